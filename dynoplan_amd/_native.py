@@ -1,0 +1,155 @@
+"""ctypes view of include/dynoplan_amd.h.
+
+The structures below mirror the C declarations field by field; ctypes applies
+the same alignment rules as the C compiler, so their sizes match the header
+(checked by tests/test_native_abi.py).  Loading fails loudly when the HIP
+library is missing: there is no CPU fallback for the expansion path.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdynoplan_amd.so")
+
+DP_MAX_NX = 16
+
+DP_OK = 0
+DP_ERR_ARG = -1
+DP_ERR_HIP = -2
+DP_ERR_STATE = -3
+DP_ERR_CAPACITY = -4
+DP_ERR_IO = -5
+DP_ERR_FORMAT = -6
+DP_ERR_NOMEM = -7
+DP_ERR_NOTCONV = -8
+
+SYS_UNICYCLE1 = 0
+SYS_UNICYCLE2 = 1
+SYS_QUADROTOR = 2
+SYS_CAR_TRAILER = 3
+
+V_FREE, V_BOUNDS, V_COLLIDE, V_SURROGATE_REJECT = 0, 1, 2, 3
+R_NONE, R_NOVEL, R_DUP_VISITED, R_DUP_BATCH, R_CAPPED = 0, 1, 2, 3, 4
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [
+        ("system", C.c_int32), ("nx", C.c_int32), ("nu", C.c_int32), ("pos_dim", C.c_int32),
+        ("dt", C.c_double),
+        ("x_lb", C.c_double * DP_MAX_NX), ("x_ub", C.c_double * DP_MAX_NX),
+        ("dist_w", C.c_double * 4),
+        ("max_vel", C.c_double), ("max_angular_vel", C.c_double),
+        ("max_acc", C.c_double), ("max_angular_acc", C.c_double),
+        ("body_half", C.c_double * 2), ("trailer_half", C.c_double * 2),
+        ("hitch", C.c_double), ("wheelbase", C.c_double),
+        ("quad", C.c_double * 8),
+        ("near_eps", C.c_double),
+    ]
+
+
+class Prims(C.Structure):
+    _fields_ = [
+        ("P", C.c_int32), ("nx", C.c_int32), ("nu", C.c_int32), ("reserved", C.c_int32),
+        ("offsets", C.POINTER(C.c_int32)),
+        ("states", C.POINTER(C.c_double)),
+        ("actions", C.POINTER(C.c_double)),
+        ("cost", C.POINTER(C.c_double)),
+    ]
+
+
+class World(C.Structure):
+    _fields_ = [
+        ("n_boxes", C.c_int32), ("n_spheres", C.c_int32),
+        ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+        ("boxes", C.POINTER(C.c_double)),
+        ("spheres", C.POINTER(C.c_double)),
+    ]
+
+
+class ExpandParams(C.Structure):
+    _fields_ = [
+        ("delta", C.c_float), ("alpha", C.c_float),
+        ("delta_factor_goal", C.c_double), ("cost_delta_factor", C.c_double),
+        ("limit_branching_factor", C.c_int32), ("num_check_goal", C.c_int32),
+        ("check_intermediate_goal", C.c_int32), ("use_surrogate", C.c_int32),
+        ("fp32_rollout", C.c_int32), ("reserved", C.c_int32),
+        ("goal", C.c_double * DP_MAX_NX),
+    ]
+
+
+class Counts(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in (
+        "nodes", "pairs", "bounds", "collide", "free_", "surrogate_reject", "novel",
+        "dup_visited", "dup_batch", "capped", "goal_hits", "checked", "recheck_fp64",
+        "resolve_spins")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+class PairsOut(C.Structure):
+    _fields_ = [
+        ("capacity", C.c_int64),
+        ("node", C.POINTER(C.c_int32)), ("prim", C.POINTER(C.c_int32)),
+        ("verdict", C.POINTER(C.c_int8)), ("first_k", C.POINTER(C.c_int32)),
+        ("goal_index", C.POINTER(C.c_int32)), ("resolution", C.POINTER(C.c_int8)),
+        ("final_state", C.POINTER(C.c_double)),
+        ("g", C.POINTER(C.c_double)), ("h", C.POINTER(C.c_double)),
+    ]
+
+
+# name -> (restype, argtypes); every symbol include/dynoplan_amd.h declares.
+SIGNATURES = {
+    "dp_last_error": (C.c_char_p, []),
+    "dp_system_default": (C.c_int, [C.c_int32, C.POINTER(SystemDesc)]),
+    "dp_create": (C.c_void_p, [C.POINTER(SystemDesc), C.c_int32]),
+    "dp_destroy": (None, [C.c_void_p]),
+    "dp_upload_primitives": (C.c_int, [C.c_void_p, C.POINTER(Prims), C.c_int32]),
+    "dp_upload_world": (C.c_int, [C.c_void_p, C.POINTER(World)]),
+    "dp_visited_reset": (C.c_int, [C.c_void_p]),
+    "dp_visited_append": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int64]),
+    "dp_visited_append_novel": (C.c_int, [C.c_void_p]),
+    "dp_visited_size": (C.c_int64, [C.c_void_p]),
+    "dp_visited_radius": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int64, C.c_double,
+                                    C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int64]),
+    "dp_reserve": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
+    "dp_upload_nodes": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32]),
+    "dp_expand": (C.c_int, [C.c_void_p, C.POINTER(ExpandParams), C.c_void_p]),
+    "dp_expand_batch": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32,
+                                  C.POINTER(ExpandParams), C.c_void_p]),
+    "dp_get_counts": (C.c_int, [C.c_void_p, C.POINTER(Counts)]),
+    "dp_fetch_pairs": (C.c_int, [C.c_void_p, C.POINTER(PairsOut)]),
+    "dp_stream": (C.c_void_p, [C.c_void_p]),
+    "dp_last_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+}
+
+_lib = None
+
+
+class DynoplanError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dynoplan_amd error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Load the HIP library (build it with ``python -m dynoplan_amd.build``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m dynoplan_amd.build` "
+                          "(the expansion path has no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != DP_OK:
+        raise DynoplanError(rc, load().dp_last_error().decode())
+    return rc

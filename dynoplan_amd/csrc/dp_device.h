@@ -1,0 +1,172 @@
+// dp_device.h -- device-side system semantics of the expansion path.
+//
+// Every function here is the gfx950 counterpart of a dynobench::Model_robot
+// method or nigh metric the reference calls on the hot path.  They are
+// compiled with -ffp-contract=off and written as explicit sequences of
+// IEEE operations so that FP64 results are bit-identical to any other
+// implementation that rounds the same expression in the same order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/dynoplan_amd.h"
+
+#define DP_PI 3.141592653589793
+#define DP_TWO_PI 6.283185307179586
+#define DP_HALF_PI 1.5707963267948966
+
+#define DP_ASIN_TERMS 30
+#define DP_TRIG_TERMS 10
+
+// System constants visible to every kernel.  Uploaded once per context.
+struct dp_dev_sys {
+  int32_t system, nx, nu, pos_dim;
+  double dt;
+  double lo[DP_MAX_NX], hi[DP_MAX_NX];  // bounds incl. world position bounds
+  double w[4];
+  double max_vel, max_w, max_acc, max_wacc;
+  double body_hl, body_hw, trailer_hl, trailer_hw, hitch;
+  double radius;
+  double quad[8];
+  double asin_c[DP_ASIN_TERMS];
+  double cos_c[DP_TRIG_TERMS];
+  double sinc_c[DP_TRIG_TERMS];
+  float near_eps;
+};
+
+__device__ __forceinline__ double dp_so2(double a, double b) {
+  double d = fabs(a - b);
+  return d > DP_PI ? DP_TWO_PI - d : d;
+}
+
+__device__ __forceinline__ double dp_l2_2(double ax, double ay, double bx, double by) {
+  double dx = ax - bx, dy = ay - by;
+  double s = dx * dx;
+  s = s + dy * dy;
+  return sqrt(s);
+}
+
+__device__ __forceinline__ double dp_l2_3(const double* a, const double* b) {
+  double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+  double s = d0 * d0;
+  s = s + d1 * d1;
+  s = s + d2 * d2;
+  return sqrt(s);
+}
+
+__device__ __forceinline__ double dp_asin_series(const dp_dev_sys& S, double t) {
+  double z = t * t;
+  double p = S.asin_c[DP_ASIN_TERMS - 1];
+#pragma unroll
+  for (int n = DP_ASIN_TERMS - 2; n >= 0; --n) p = p * z + S.asin_c[n];
+  return t * p;
+}
+
+// acos on [0,1] through asin series whose argument stays <= 1/2.
+__device__ __forceinline__ double dp_acos01(const dp_dev_sys& S, double x) {
+  if (x >= 1.0) return 0.0;
+  if (x >= 0.5) {
+    double t = sqrt((1.0 - x) * 0.5);
+    return 2.0 * dp_asin_series(S, t);
+  }
+  return DP_HALF_PI - dp_asin_series(S, x);
+}
+
+__device__ __forceinline__ double dp_quat_angle(const dp_dev_sys& S, const double* a, const double* b) {
+  double dot = a[0] * b[0];
+  dot = dot + a[1] * b[1];
+  dot = dot + a[2] * b[2];
+  dot = dot + a[3] * b[3];
+  dot = fabs(dot);
+  if (dot > 1.0) dot = 1.0;
+  return dp_acos01(S, dot);
+}
+
+// robot->distance (nigh_custom_spaces.hpp:391-573 key spaces).
+__device__ __forceinline__ double dp_distance(const dp_dev_sys& S, const double* a, const double* b) {
+  switch (S.system) {
+    case DP_SYS_UNICYCLE1:
+      return S.w[0] * dp_l2_2(a[0], a[1], b[0], b[1]) + S.w[1] * dp_so2(a[2], b[2]);
+    case DP_SYS_UNICYCLE2: {
+      double d = S.w[0] * dp_l2_2(a[0], a[1], b[0], b[1]) + S.w[1] * dp_so2(a[2], b[2]);
+      d = d + S.w[2] * fabs(a[3] - b[3]);
+      d = d + S.w[3] * fabs(a[4] - b[4]);
+      return d;
+    }
+    case DP_SYS_CAR_TRAILER: {
+      double d = S.w[0] * dp_l2_2(a[0], a[1], b[0], b[1]) + S.w[1] * dp_so2(a[2], b[2]);
+      d = d + S.w[2] * dp_so2(a[3], b[3]);
+      return d;
+    }
+    default: {  // quadrotor
+      double d = S.w[0] * dp_l2_3(a, b) + S.w[1] * dp_quat_angle(S, a + 3, b + 3);
+      d = d + S.w[2] * dp_l2_3(a + 7, b + 7);
+      d = d + S.w[3] * dp_l2_3(a + 10, b + 10);
+      return d;
+    }
+  }
+}
+
+__device__ __forceinline__ double dp_max(double a, double b) { return b > a ? b : a; }
+
+// robot->lower_bound_time.
+__device__ __forceinline__ double dp_lower_bound_time(const dp_dev_sys& S, const double* a, const double* b) {
+  switch (S.system) {
+    case DP_SYS_UNICYCLE1:
+    case DP_SYS_CAR_TRAILER:
+      return dp_max(dp_l2_2(a[0], a[1], b[0], b[1]) / S.max_vel, dp_so2(a[2], b[2]) / S.max_w);
+    case DP_SYS_UNICYCLE2: {
+      double t = dp_max(dp_l2_2(a[0], a[1], b[0], b[1]) / S.max_vel, dp_so2(a[2], b[2]) / S.max_w);
+      t = dp_max(t, fabs(a[3] - b[3]) / S.max_acc);
+      t = dp_max(t, fabs(a[4] - b[4]) / S.max_wacc);
+      return t;
+    }
+    default:
+      return dp_max(dp_l2_3(a, b) / S.max_vel, dp_l2_3(a + 7, b + 7) / S.max_acc);
+  }
+}
+
+__device__ __forceinline__ bool dp_is_offset_dim(const dp_dev_sys& S, int c) {
+  if (S.system == DP_SYS_QUADROTOR) return c <= 2 || (c >= 7 && c <= 9);
+  return c <= 1;
+}
+
+// Separating-axis test, oriented box vs axis-aligned box; touching collides.
+__device__ __forceinline__ bool dp_obb_aabb(double cx, double cy, double c, double s, double hl, double hw,
+                                            double ox, double oy, double hx, double hy) {
+  double tx = ox - cx;
+  double ty = oy - cy;
+  double ac = fabs(c);
+  double as = fabs(s);
+  if (fabs(tx) > hx + (hl * ac + hw * as)) return false;
+  if (fabs(ty) > hy + (hl * as + hw * ac)) return false;
+  double pu = tx * c + ty * s;
+  if (fabs(pu) > hl + (hx * ac + hy * as)) return false;
+  double pv = ty * c - tx * s;
+  if (fabs(pv) > hw + (hx * as + hy * ac)) return false;
+  return true;
+}
+
+__device__ __forceinline__ bool dp_sphere_box(double px, double py, double pz, double r, const double* b) {
+  double dx = fabs(px - b[0]) - b[3];
+  double dy = fabs(py - b[1]) - b[4];
+  double dz = fabs(pz - b[2]) - b[5];
+  dx = dx > 0.0 ? dx : 0.0;
+  dy = dy > 0.0 ? dy : 0.0;
+  dz = dz > 0.0 ? dz : 0.0;
+  double d2 = dx * dx + dy * dy;
+  d2 = d2 + dz * dz;
+  return d2 <= r * r;
+}
+
+__device__ __forceinline__ bool dp_sphere_sphere(double px, double py, double pz, double r, const double* o) {
+  double dx = px - o[0], dy = py - o[1], dz = pz - o[2];
+  double d2 = dx * dx + dy * dy;
+  d2 = d2 + dz * dz;
+  double rr = r + o[3];
+  return d2 <= rr * rr;
+}
+
+// Wave-level helpers (wave64).
+__device__ __forceinline__ uint64_t dp_ballot(bool p) { return __ballot(p); }
+__device__ __forceinline__ int dp_lane() { return __lane_id(); }

@@ -1,0 +1,161 @@
+"""Python host view of the C-ABI: one GPU expansion context.
+
+``GpuExpander`` mirrors the objects the reference's expansion loop holds
+(dbastar.cpp:523-919): the primitive store behind T_m (``upload_primitives``),
+the environment (``upload_world``), the visited tree T_n
+(``visited_append`` / ``visited_append_novel``) and one batched pass of the
+inner loop (``expand``).  Arrays cross the boundary as plain pointers.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class GpuExpander:
+    def __init__(self, desc, device=0):
+        self.lib = N.load()
+        self.desc = desc
+        self.nx = desc.nx
+        ctx = self.lib.dp_create(C.byref(desc), device)
+        if not ctx:
+            raise N.DynoplanError(N.DP_ERR_HIP, self.lib.dp_last_error().decode())
+        self.ctx = C.c_void_p(ctx)
+        self._keep = []
+
+    def close(self):
+        if self.ctx:
+            self.lib.dp_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- inputs -------------------------------------------------------------
+    def upload_primitives(self, offsets, states, actions=None, max_motions=0):
+        offsets = np.ascontiguousarray(offsets, np.int32)
+        states = np.ascontiguousarray(states, np.float64)
+        p = N.Prims()
+        p.P = offsets.size - 1
+        p.nx = states.shape[1]
+        p.nu = actions.shape[1] if actions is not None and actions.ndim == 2 else 0
+        p.offsets = _iptr(offsets)
+        p.states = _dptr(states)
+        if actions is not None:
+            actions = np.ascontiguousarray(actions, np.float64)
+            p.actions = _dptr(actions)
+        N.check(self.lib.dp_upload_primitives(self.ctx, C.byref(p), int(max_motions)))
+
+    def upload_world(self, lo, hi, boxes, spheres=None):
+        boxes = np.ascontiguousarray(boxes, np.float64).reshape(-1, 6)
+        spheres = np.ascontiguousarray(spheres if spheres is not None else np.zeros((0, 4)), np.float64).reshape(-1, 4)
+        w = N.World()
+        w.n_boxes = boxes.shape[0]
+        w.n_spheres = spheres.shape[0]
+        for k in range(3):
+            w.lo[k] = float(lo[k])
+            w.hi[k] = float(hi[k])
+        w.boxes = _dptr(boxes)
+        w.spheres = _dptr(spheres)
+        N.check(self.lib.dp_upload_world(self.ctx, C.byref(w)))
+
+    def visited_reset(self):
+        N.check(self.lib.dp_visited_reset(self.ctx))
+
+    def visited_append(self, states):
+        states = np.ascontiguousarray(states, np.float64).reshape(-1, self.nx)
+        N.check(self.lib.dp_visited_append(self.ctx, _dptr(states), states.shape[0]))
+
+    def visited_append_novel(self):
+        N.check(self.lib.dp_visited_append_novel(self.ctx))
+
+    def visited_size(self):
+        return int(self.lib.dp_visited_size(self.ctx))
+
+    def reserve(self, max_nodes, max_pairs):
+        N.check(self.lib.dp_reserve(self.ctx, int(max_nodes), int(max_pairs)))
+
+    def upload_nodes(self, nodes, g_parent=None):
+        nodes = np.ascontiguousarray(nodes, np.float64).reshape(-1, self.nx)
+        g = np.ascontiguousarray(g_parent if g_parent is not None else np.zeros(nodes.shape[0]), np.float64)
+        N.check(self.lib.dp_upload_nodes(self.ctx, _dptr(nodes), _dptr(g), nodes.shape[0]))
+
+    # ---- expansion ------------------------------------------------------------
+    def expand_async(self, params, stream=None):
+        """Enqueue one expansion of the uploaded nodes on ``stream`` (a
+        hipStream_t as int, or None for the context stream)."""
+        N.check(self.lib.dp_expand(self.ctx, C.byref(params), C.c_void_p(stream) if stream else None))
+
+    def expand(self, nodes, g_parent, params):
+        nodes = np.ascontiguousarray(nodes, np.float64).reshape(-1, self.nx)
+        g = np.ascontiguousarray(g_parent, np.float64)
+        N.check(self.lib.dp_expand_batch(self.ctx, _dptr(nodes), _dptr(g), nodes.shape[0], C.byref(params), None))
+        return self.counts()
+
+    def counts(self):
+        c = N.Counts()
+        N.check(self.lib.dp_get_counts(self.ctx, C.byref(c)))
+        return c.as_dict()
+
+    def timing(self):
+        ms = (C.c_float * 4)()
+        N.check(self.lib.dp_last_timing(self.ctx, ms))
+        return {"applicability": ms[0], "check": ms[1], "resolve": ms[2], "total": ms[3]}
+
+    def stream(self):
+        return self.lib.dp_stream(self.ctx)
+
+    def fetch(self):
+        n = self.counts()["pairs"]
+        out = {
+            "node": np.zeros(n, np.int32), "prim": np.zeros(n, np.int32), "verdict": np.zeros(n, np.int8),
+            "first_k": np.zeros(n, np.int32), "goal_index": np.zeros(n, np.int32),
+            "resolution": np.zeros(n, np.int8), "final_state": np.zeros((n, self.nx)),
+            "g": np.zeros(n), "h": np.zeros(n),
+        }
+        po = pairs_out(out, n)
+        N.check(self.lib.dp_fetch_pairs(self.ctx, C.byref(po)))
+        return out
+
+
+def pairs_out(out, n):
+    po = N.PairsOut()
+    po.capacity = n
+    po.node = _iptr(out["node"])
+    po.prim = _iptr(out["prim"])
+    po.verdict = out["verdict"].ctypes.data_as(C.POINTER(C.c_int8))
+    po.first_k = _iptr(out["first_k"])
+    po.goal_index = _iptr(out["goal_index"])
+    po.resolution = out["resolution"].ctypes.data_as(C.POINTER(C.c_int8))
+    po.final_state = _dptr(out["final_state"])
+    po.g = _dptr(out["g"])
+    po.h = _dptr(out["h"])
+    return po
+
+
+def expander_for(workload, device=0):
+    """Context loaded with a synth.Workload's primitives, world and visited set."""
+    ex = GpuExpander(workload.desc, device)
+    ex.upload_primitives(workload.offsets, workload.states, workload.actions)
+    ex.upload_world(workload.lo, workload.hi, workload.boxes, workload.spheres)
+    if workload.visited.shape[0]:
+        ex.visited_append(workload.visited)
+    return ex

@@ -1,0 +1,308 @@
+"""Seed-deterministic synthetic workloads of the expansion path.
+
+Each generator returns plain numpy arrays in the layouts of
+include/dynoplan_amd.h (row-major states, int32 offsets).  The workloads
+follow the configuration list the benchmark is quoted on:
+
+  unicycle1_small   unicycle1_v0, 1000 primitives of 5-20 steps, 10x10 m map,
+                    20 boxes (sides U[0.5,1.5]), 256 frontier nodes in free
+                    space, delta 0.3, FP64
+  unicycle1_scale   5000 primitives of 5-30 steps, 4096 nodes, 200 boxes on
+                    20x20 m, 100k random visited states, delta 0.3, FP64
+  quadrotor         quadrotor_v0, 10k primitives of 10-50 steps, 10x10x5 m
+                    world, 100 boxes + 50 spheres (r U[0.2,0.8]), 0.1 m robot,
+                    2048 nodes
+  car_trailer       car + 1 trailer, 5000 primitives of 10-40 steps, 300
+                    boxes on 30x30 m, 4096 nodes
+
+Primitives are canonical (start position at the origin, as every primitive
+of the reference's data/ fixtures) and are rolled out with the system
+dynamics in FP64; controls are piecewise constant.
+"""
+import math
+
+import numpy as np
+
+from . import _native as N
+
+TWO_PI = 2.0 * math.pi
+
+
+def wrap_angle(a):
+    """Wrap to (-pi, pi]."""
+    a = np.where(a > math.pi, a - TWO_PI, a)
+    return np.where(a <= -math.pi, a + TWO_PI, a)
+
+
+def _piecewise_controls(rng, P, L, lo, hi, max_pieces=3):
+    """P x L x nu piecewise-constant controls with 1..max_pieces pieces."""
+    lo = np.asarray(lo, float)
+    hi = np.asarray(hi, float)
+    nu = lo.size
+    u = np.empty((P, L, nu))
+    pieces = rng.integers(1, max_pieces + 1, size=P)
+    for j in range(P):
+        cuts = np.sort(rng.integers(1, L, size=pieces[j] - 1)) if L > 1 else np.array([], int)
+        bounds = np.concatenate([[0], cuts, [L]])
+        for a, b in zip(bounds[:-1], bounds[1:]):
+            u[j, a:b] = rng.uniform(lo, hi)
+    return u
+
+
+def _pack(states_list, actions_list):
+    lens = np.array([s.shape[0] for s in states_list], np.int32)
+    offsets = np.zeros(len(lens) + 1, np.int32)
+    np.cumsum(lens, out=offsets[1:])
+    states = np.ascontiguousarray(np.concatenate(states_list, axis=0))
+    actions = np.ascontiguousarray(np.concatenate(actions_list, axis=0))
+    return offsets, states, actions
+
+
+def unicycle1_primitives(rng, P, lmin, lmax, dt=0.1, vmax=0.5, wmax=0.5):
+    """x+ = x + dt v cos(th), y+ = y + dt v sin(th), th+ = wrap(th + dt w)."""
+    L = rng.integers(lmin, lmax + 1, size=P)
+    Lmax = int(L.max())
+    u = _piecewise_controls(rng, P, Lmax, [-vmax, -wmax], [vmax, wmax])
+    x = np.zeros((P, Lmax + 1, 3))
+    x[:, 0, 2] = wrap_angle(rng.uniform(-math.pi, math.pi, size=P))
+    for k in range(Lmax):
+        th = x[:, k, 2]
+        x[:, k + 1, 0] = x[:, k, 0] + dt * (u[:, k, 0] * np.cos(th))
+        x[:, k + 1, 1] = x[:, k, 1] + dt * (u[:, k, 0] * np.sin(th))
+        x[:, k + 1, 2] = wrap_angle(th + dt * u[:, k, 1])
+    return _pack([x[j, : L[j] + 1] for j in range(P)], [u[j, : L[j]] for j in range(P)])
+
+
+def unicycle2_primitives(rng, P, lmin, lmax, dt=0.1, amax=0.25, vmax=0.5):
+    """unicycle2_v0: (x y th v w), controls (a, alpha)."""
+    L = rng.integers(lmin, lmax + 1, size=P)
+    Lmax = int(L.max())
+    u = _piecewise_controls(rng, P, Lmax, [-amax, -amax], [amax, amax])
+    x = np.zeros((P, Lmax + 1, 5))
+    x[:, 0, 2] = wrap_angle(rng.uniform(-math.pi, math.pi, size=P))
+    x[:, 0, 3] = rng.uniform(-0.3, 0.3, size=P)
+    x[:, 0, 4] = rng.uniform(-0.3, 0.3, size=P)
+    for k in range(Lmax):
+        th, v, w = x[:, k, 2], x[:, k, 3], x[:, k, 4]
+        x[:, k + 1, 0] = x[:, k, 0] + dt * (v * np.cos(th))
+        x[:, k + 1, 1] = x[:, k, 1] + dt * (v * np.sin(th))
+        x[:, k + 1, 2] = wrap_angle(th + dt * w)
+        x[:, k + 1, 3] = np.clip(v + dt * u[:, k, 0], -vmax, vmax)
+        x[:, k + 1, 4] = np.clip(w + dt * u[:, k, 1], -vmax, vmax)
+    return _pack([x[j, : L[j] + 1] for j in range(P)], [u[j, : L[j]] for j in range(P)])
+
+
+def car_trailer_primitives(rng, P, lmin, lmax, dt=0.1, wheelbase=0.25, hitch=0.5, vmax=0.5, phimax=math.pi / 3):
+    """car_first_order_with_1_trailers: (x y th0 th1), controls (v, phi)."""
+    L = rng.integers(lmin, lmax + 1, size=P)
+    Lmax = int(L.max())
+    u = _piecewise_controls(rng, P, Lmax, [-vmax, -phimax], [vmax, phimax])
+    x = np.zeros((P, Lmax + 1, 4))
+    x[:, 0, 2] = wrap_angle(rng.uniform(-math.pi, math.pi, size=P))
+    x[:, 0, 3] = wrap_angle(x[:, 0, 2] + rng.uniform(-0.5, 0.5, size=P))
+    for k in range(Lmax):
+        th0, th1 = x[:, k, 2], x[:, k, 3]
+        v, phi = u[:, k, 0], u[:, k, 1]
+        x[:, k + 1, 0] = x[:, k, 0] + dt * (v * np.cos(th0))
+        x[:, k + 1, 1] = x[:, k, 1] + dt * (v * np.sin(th0))
+        x[:, k + 1, 2] = wrap_angle(th0 + dt * (v / wheelbase * np.tan(phi)))
+        x[:, k + 1, 3] = wrap_angle(th1 + dt * (v / hitch * np.sin(th0 - th1)))
+    return _pack([x[j, : L[j] + 1] for j in range(P)], [u[j, : L[j]] for j in range(P)])
+
+
+def random_boxes(rng, n, lo, hi, side_lo=0.5, side_hi=1.5, dim=2):
+    b = np.zeros((n, 6))
+    for a in range(dim):
+        b[:, a] = rng.uniform(lo[a], hi[a], size=n)
+        b[:, 3 + a] = 0.5 * rng.uniform(side_lo, side_hi, size=n)
+    return b
+
+
+def planar_footprint_collides(desc, boxes, states):
+    """Vectorised separating-axis test of planar footprints against AABBs
+    (used only to place frontier nodes in free space)."""
+    s = np.asarray(states, float)
+    hit = np.zeros(s.shape[0], bool)
+
+    def obb(cx, cy, th, hl, hw):
+        c, sn = np.cos(th)[:, None], np.sin(th)[:, None]
+        tx = boxes[None, :, 0] - cx[:, None]
+        ty = boxes[None, :, 1] - cy[:, None]
+        hx, hy = boxes[None, :, 3], boxes[None, :, 4]
+        ac, asn = np.abs(c), np.abs(sn)
+        sep = np.abs(tx) > hx + (hl * ac + hw * asn)
+        sep |= np.abs(ty) > hy + (hl * asn + hw * ac)
+        sep |= np.abs(tx * c + ty * sn) > hl + (hx * ac + hy * asn)
+        sep |= np.abs(ty * c - tx * sn) > hw + (hx * asn + hy * ac)
+        return (~sep).any(axis=1)
+
+    hit |= obb(s[:, 0], s[:, 1], s[:, 2], desc.body_half[0], desc.body_half[1])
+    if desc.system == N.SYS_CAR_TRAILER:
+        th1 = s[:, 3]
+        hit |= obb(s[:, 0] - desc.hitch * np.cos(th1), s[:, 1] - desc.hitch * np.sin(th1), th1,
+                   desc.trailer_half[0], desc.trailer_half[1])
+    return hit
+
+
+def free_planar_nodes(rng, desc, boxes, lo, hi, n, extra=None):
+    out = []
+    while sum(len(o) for o in out) < n:
+        m = 4 * n
+        s = np.zeros((m, desc.nx))
+        s[:, 0] = rng.uniform(lo[0], hi[0], size=m)
+        s[:, 1] = rng.uniform(lo[1], hi[1], size=m)
+        s[:, 2] = rng.uniform(-math.pi, math.pi, size=m)
+        if extra is not None:
+            extra(s)
+        ok = ~planar_footprint_collides(desc, boxes, s)
+        out.append(s[ok])
+    return np.ascontiguousarray(np.concatenate(out)[:n])
+
+
+def system_desc(system):
+    lib = N.load()
+    d = N.SystemDesc()
+    N.check(lib.dp_system_default(system, d))
+    return d
+
+
+def default_desc_static(system):
+    """The same defaults as dp_system_default, without loading the library
+    (the CPU tests build descriptions for the oracle this way)."""
+    d = N.SystemDesc()
+    for k in range(N.DP_MAX_NX):
+        d.x_lb[k] = -math.inf
+        d.x_ub[k] = math.inf
+    d.system = system
+    d.near_eps = 1e-4
+    if system == N.SYS_UNICYCLE1:
+        d.nx, d.nu, d.pos_dim, d.dt = 3, 2, 2, 0.1
+        d.dist_w[0], d.dist_w[1] = 1.0, 0.5
+        d.max_vel, d.max_angular_vel = 0.5, 0.5
+        d.body_half[0], d.body_half[1] = 0.25, 0.125
+    elif system == N.SYS_UNICYCLE2:
+        d.nx, d.nu, d.pos_dim, d.dt = 5, 2, 2, 0.1
+        d.x_lb[3], d.x_ub[3], d.x_lb[4], d.x_ub[4] = -0.5, 0.5, -0.5, 0.5
+        for k, w in enumerate((1.0, 0.5, 0.25, 0.25)):
+            d.dist_w[k] = w
+        d.max_vel, d.max_angular_vel, d.max_acc, d.max_angular_acc = 0.5, 0.5, 0.25, 0.25
+        d.body_half[0], d.body_half[1] = 0.25, 0.125
+    elif system == N.SYS_CAR_TRAILER:
+        d.nx, d.nu, d.pos_dim, d.dt = 4, 2, 2, 0.1
+        for k, w in enumerate((1.0, 0.5, 0.5)):
+            d.dist_w[k] = w
+        d.max_vel, d.max_angular_vel = 0.5, 0.5
+        d.body_half[0], d.body_half[1] = 0.25, 0.125
+        d.trailer_half[0], d.trailer_half[1] = 0.15, 0.125
+        d.hitch, d.wheelbase = 0.5, 0.25
+    elif system == N.SYS_QUADROTOR:
+        d.nx, d.nu, d.pos_dim, d.dt = 13, 4, 3, 0.01
+        for k in range(7, 10):
+            d.x_lb[k], d.x_ub[k] = -4.0, 4.0
+        for k in range(10, 13):
+            d.x_lb[k], d.x_ub[k] = -8.0, 8.0
+        for k, w in enumerate((1.0, 0.5, 0.1, 0.05)):
+            d.dist_w[k] = w
+        d.max_vel, d.max_acc, d.max_angular_vel = 4.0, 8.0, 8.0
+        d.body_half[0] = 0.1
+        for k, v in enumerate((0.034, 9.81, 0.046 * 0.7071067811865476, 0.006,
+                               16.571710e-6, 16.655602e-6, 29.261652e-6)):
+            d.quad[k] = v
+    else:
+        raise ValueError(system)
+    return d
+
+
+def make_params(delta=0.3, alpha=0.5, goal=None, limit_branching_factor=20, num_check_goal=4,
+                check_intermediate_goal=True, delta_factor_goal=1.0, cost_delta_factor=1.0,
+                use_surrogate=False, fp32_rollout=False):
+    """Options_dbastar defaults (dbastar/options.hpp:105-156)."""
+    p = N.ExpandParams()
+    p.delta = delta
+    p.alpha = alpha
+    p.delta_factor_goal = delta_factor_goal
+    p.cost_delta_factor = cost_delta_factor
+    p.limit_branching_factor = limit_branching_factor
+    p.num_check_goal = num_check_goal
+    p.check_intermediate_goal = int(check_intermediate_goal)
+    p.use_surrogate = int(use_surrogate)
+    p.fp32_rollout = int(fp32_rollout)
+    if goal is not None:
+        for k, v in enumerate(goal):
+            p.goal[k] = float(v)
+    return p
+
+
+class Workload(dict):
+    """Inputs of one expansion batch: desc, offsets, states, actions, lo, hi,
+    boxes, spheres, nodes, g_parent, visited, params, name."""
+
+    def __getattr__(self, k):
+        try:
+            return self[k]
+        except KeyError as e:
+            raise AttributeError(k) from e
+
+
+def _planar_workload(name, system, seed, P, lmin, lmax, extent, n_boxes, N_nodes, n_visited,
+                     visited_from_nodes=False, limit=20, prim_fn=None, desc=None):
+    rng = np.random.default_rng(seed)
+    desc = desc or default_desc_static(system)
+    offsets, states, actions = prim_fn(rng, P, lmin, lmax)
+    lo = np.array([0.0, 0.0, 0.0])
+    hi = np.array([extent, extent, 0.0])
+    boxes = random_boxes(rng, n_boxes, lo, hi)
+    extra = None
+    if system == N.SYS_CAR_TRAILER:
+        def extra(s):
+            s[:, 3] = wrap_angle(s[:, 2] + rng.uniform(-0.5, 0.5, size=s.shape[0]))
+    elif system == N.SYS_UNICYCLE2:
+        def extra(s):
+            s[:, 3] = rng.uniform(-0.3, 0.3, size=s.shape[0])
+            s[:, 4] = rng.uniform(-0.3, 0.3, size=s.shape[0])
+    nodes = free_planar_nodes(rng, desc, boxes, lo + 0.5, hi - 0.5, N_nodes, extra)
+    g_parent = rng.uniform(0.0, 10.0, size=N_nodes)
+    if visited_from_nodes:
+        visited = nodes.copy()
+    else:
+        visited = np.zeros((n_visited, desc.nx))
+        visited[:, 0] = rng.uniform(lo[0], hi[0], size=n_visited)
+        visited[:, 1] = rng.uniform(lo[1], hi[1], size=n_visited)
+        visited[:, 2] = rng.uniform(-math.pi, math.pi, size=n_visited)
+        if extra is not None:
+            extra(visited)
+    goal = free_planar_nodes(rng, desc, boxes, lo + 0.5, hi - 0.5, 1, extra)[0]
+    params = make_params(goal=goal, limit_branching_factor=limit)
+    return Workload(name=name, desc=desc, offsets=offsets, states=states, actions=actions, lo=lo, hi=hi,
+                    boxes=boxes, spheres=np.zeros((0, 4)), nodes=nodes, g_parent=g_parent,
+                    visited=np.ascontiguousarray(visited), params=params, seed=seed)
+
+
+def unicycle1_small(seed=1, **kw):
+    args = dict(P=1000, lmin=5, lmax=20, extent=10.0, n_boxes=20, N_nodes=256, n_visited=0,
+                visited_from_nodes=True)
+    args.update(kw)
+    return _planar_workload("unicycle1_v0 plain expansion N=256 P=1000 O=20", N.SYS_UNICYCLE1, seed,
+                            prim_fn=unicycle1_primitives, **args)
+
+
+def unicycle1_scale(seed=2, **kw):
+    args = dict(P=5000, lmin=5, lmax=30, extent=20.0, n_boxes=200, N_nodes=4096, n_visited=100_000)
+    args.update(kw)
+    return _planar_workload("unicycle1_v0 at scale N=4096 P=5000 O=200 V=100k", N.SYS_UNICYCLE1, seed,
+                            prim_fn=unicycle1_primitives, **args)
+
+
+def unicycle2_small(seed=5, **kw):
+    args = dict(P=1000, lmin=5, lmax=20, extent=10.0, n_boxes=20, N_nodes=256, n_visited=0,
+                visited_from_nodes=True)
+    args.update(kw)
+    return _planar_workload("unicycle2_v0 N=256 P=1000 O=20", N.SYS_UNICYCLE2, seed,
+                            prim_fn=unicycle2_primitives, **args)
+
+
+def car_trailer(seed=4, **kw):
+    args = dict(P=5000, lmin=10, lmax=40, extent=30.0, n_boxes=300, N_nodes=4096, n_visited=100_000)
+    args.update(kw)
+    return _planar_workload("car+1 trailer N=4096 P=5000 O=300", N.SYS_CAR_TRAILER, seed,
+                            prim_fn=car_trailer_primitives, **args)

@@ -1,0 +1,224 @@
+/*
+ * dynoplan_amd.h -- C-ABI of the MI355X db-A* node-expansion path.
+ *
+ * One call of dp_expand() does, for a batch of N frontier nodes, what the
+ * reference's inner expansion loop does for one node at a time
+ * (src/dbastar/dbastar.cpp:806-919):
+ *
+ *   canonical_state / offset           dbastar.hpp:273-274   (Model_robot)
+ *   applicability query  nearestR(c, alpha*delta) over primitive starts
+ *                                       dbastar.hpp:281, dbastar.cpp:585-586
+ *   transform_primitive + bounds        dbastar.cpp:402-446, dbastar.hpp:166-215
+ *   collision check (first index)       dbastar.cpp:459-489
+ *   check_goal                          dbastar.cpp:492-512, 825-829
+ *   g / h                               dbastar.cpp:833-847
+ *   duplicate test + insert             dbastar.cpp:850-881
+ *   branching cap                       dbastar.cpp:916-918
+ *
+ * Batch semantics (the canonical order): pairs (node i, primitive j) are
+ * visited in ascending (i, j) order.  Every applicable pair receives a
+ * verdict (FREE / BOUNDS / COLLIDE / SURROGATE_REJECT) before any cap.  The
+ * duplicate decision of a FREE pair is taken against the visited set plus
+ * every NOVEL pair that precedes it in canonical order, exactly as the
+ * reference inserts each novel node into T_n at once (dbastar.cpp:875-876).
+ * A FREE pair of node i is CAPPED when limit_branching_factor NOVEL pairs of
+ * node i precede it (the reference's loop break).  The reference shuffles
+ * the applicable list (dbastar.hpp:302-303); the canonical order is the
+ * identity shuffle.
+ *
+ * Conventions: every entry point returns DP_OK (0) or a negative DP_ERR_*;
+ * dp_last_error() gives a thread-local message.  No exceptions cross the
+ * boundary.  A context is used by one host thread at a time; work is issued
+ * on the stream passed in (NULL = the context's own stream).  Device memory
+ * is owned by the context; host buffers are owned by the caller.
+ */
+#ifndef DYNOPLAN_AMD_H
+#define DYNOPLAN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  DP_OK = 0,
+  DP_ERR_ARG = -1,      /* invalid argument (sizes, NULL pointers, alpha not in (0,1), ...) */
+  DP_ERR_HIP = -2,      /* a HIP runtime call failed */
+  DP_ERR_STATE = -3,    /* call out of order (no primitives / world uploaded, ...) */
+  DP_ERR_CAPACITY = -4, /* output capacity exceeded */
+  DP_ERR_IO = -5,       /* file could not be read */
+  DP_ERR_FORMAT = -6,   /* file content malformed */
+  DP_ERR_NOMEM = -7,    /* allocation failed */
+  DP_ERR_NOTCONV = -8   /* a bounded device loop did not finish (reported, never hangs) */
+};
+
+/* ---- systems (dynobench::Model_robot instances on the path) ------------ */
+enum dp_system_id {
+  DP_SYS_UNICYCLE1 = 0,   /* unicycle1_v0: x y theta | v w                     */
+  DP_SYS_UNICYCLE2 = 1,   /* unicycle2_v0: x y theta v w | a alpha             */
+  DP_SYS_QUADROTOR = 2,   /* quad3d / quadrotor_v0: p(3) q(xyzw) v(3) w(3) | 4 rotors */
+  DP_SYS_CAR_TRAILER = 3  /* car_first_order_with_1_trailers: x y th0 th1 | v phi */
+};
+
+#define DP_MAX_NX 16
+
+/* Per-pair verdict (pre-cap). */
+enum { DP_V_FREE = 0, DP_V_BOUNDS = 1, DP_V_COLLIDE = 2, DP_V_SURROGATE_REJECT = 3 };
+/* Duplicate/insert resolution of a FREE pair. */
+enum { DP_R_NONE = 0, DP_R_NOVEL = 1, DP_R_DUP_VISITED = 2, DP_R_DUP_BATCH = 3, DP_R_CAPPED = 4 };
+
+/* System description: the fields of dynobench::Model_robot the path reads
+ * (nx, nu, ref_dt, x_lb/x_ub for is_state_valid, distance_weights,
+ * lower_bound_time speeds, collision geometry).  dp_system_default() fills
+ * the values this library uses for each system; position bounds are taken
+ * from the world (robot_factory(..., p_lb, p_ub) in the reference). */
+typedef struct dp_system_desc {
+  int32_t system;              /* dp_system_id */
+  int32_t nx, nu;
+  int32_t pos_dim;             /* 2 (planar) or 3 */
+  double dt;                   /* ref_dt */
+  double x_lb[DP_MAX_NX];      /* is_state_valid lower bounds (position rows overwritten by the world) */
+  double x_ub[DP_MAX_NX];
+  double dist_w[4];            /* distance_weights, one per metric component group */
+  double max_vel, max_angular_vel, max_acc, max_angular_acc; /* lower_bound_time */
+  double body_half[2];         /* planar: robot box half length/width; quadrotor: [0] = sphere radius */
+  double trailer_half[2];      /* car+trailer: trailer box half length/width */
+  double hitch;                /* car+trailer: hitch length; trailer centre at p - hitch*(cos th1, sin th1) */
+  double wheelbase;            /* car+trailer: tractor wheelbase */
+  double quad[8];              /* quadrotor: mass, g, arm, t2t, Jx, Jy, Jz, -          */
+  double near_eps;             /* FP32 path: margin (m) under which a state is re-checked in FP64 */
+} dp_system_desc;
+
+/* Motion primitives (dynoplan::Motion, ompl/robots.h:206-235), row-major as
+ * loaded from file.  Primitive j owns states [offsets[j], offsets[j+1]) and
+ * actions [offsets[j]-j, offsets[j+1]-j-1). */
+typedef struct dp_prims {
+  int32_t P;
+  int32_t nx, nu;
+  int32_t reserved;
+  const int32_t* offsets;      /* P+1 */
+  const double* states;        /* [offsets[P]][nx] */
+  const double* actions;       /* [offsets[P]-P][nu] (may be NULL for translate-only systems) */
+  const double* cost;          /* [P] or NULL (then ref_dt * #actions, robots.cpp:3153-3159) */
+} dp_prims;
+
+/* Environment (dynobench env: bounds + obstacles). */
+typedef struct dp_world {
+  int32_t n_boxes, n_spheres;
+  double lo[3], hi[3];         /* environment min/max; position bounds of is_state_valid */
+  const double* boxes;         /* [n_boxes][6]  cx cy cz hx hy hz (planar: z ignored) */
+  const double* spheres;       /* [n_spheres][4] cx cy cz r */
+} dp_world;
+
+/* Options_dbastar fields the expansion reads (dbastar/options.hpp:105-156). */
+typedef struct dp_expand_params {
+  float delta;                     /* discontinuity bound (float, as the reference) */
+  float alpha;                     /* in (0,1) */
+  double delta_factor_goal;        /* goal radius = delta_factor_goal * delta */
+  double cost_delta_factor;
+  int32_t limit_branching_factor;  /* <= 0: no cap */
+  int32_t num_check_goal;          /* intermediate goal checks (reference: 4) */
+  int32_t check_intermediate_goal; /* reference: true */
+  int32_t use_surrogate;           /* learned pre-filter (needs a trained surrogate) */
+  int32_t fp32_rollout;            /* quadrotor: FP32 rollout + FP64 near-contact re-check */
+  int32_t reserved;
+  double goal[DP_MAX_NX];
+} dp_expand_params;
+
+/* Per-batch counters.  "pairs" is the unit of the throughput metric: one
+ * (frontier node, applicable primitive) pair that received a verdict. */
+typedef struct dp_counts {
+  int64_t nodes, pairs;
+  int64_t bounds, collide, free_, surrogate_reject;
+  int64_t novel, dup_visited, dup_batch, capped;
+  int64_t goal_hits;        /* FREE pairs with chosen_index != -1 */
+  int64_t checked;          /* pairs that reached the collision check (num_col_motions) */
+  int64_t recheck_fp64;     /* FP32 pairs re-evaluated in FP64 near contact */
+  int64_t resolve_spins;    /* diagnostic: waits in the ordered duplicate resolution */
+} dp_counts;
+
+/* Caller-owned host buffers for dp_fetch_pairs (any pointer may be NULL). */
+typedef struct dp_pairs_out {
+  int64_t capacity;
+  int32_t* node;
+  int32_t* prim;
+  int8_t* verdict;          /* DP_V_* */
+  int32_t* first_k;         /* BOUNDS/COLLIDE: first offending state index, else -1 */
+  int32_t* goal_index;      /* chosen_index of check_goal, -1 if none */
+  int8_t* resolution;       /* DP_R_* */
+  double* final_state;      /* [capacity][nx]: state stored in the new node (FREE pairs) */
+  double* g;                /* FREE pairs: gScore */
+  double* h;                /* FREE pairs: hScore */
+} dp_pairs_out;
+
+/* ---- entry points ------------------------------------------------------ */
+typedef struct dp_ctx dp_ctx;
+
+/* Thread-local description of the last error. */
+const char* dp_last_error(void);
+
+/* Defaults for a system (replaces dynobench::robot_factory + model YAML). */
+int dp_system_default(int32_t system, dp_system_desc* out);
+
+/* Context on HIP device `device`.  Returns NULL on failure (see dp_last_error). */
+dp_ctx* dp_create(const dp_system_desc* sys, int32_t device);
+void dp_destroy(dp_ctx* ctx);
+
+/* Upload primitives (replaces T_m->add over motions, dbastar.cpp:564-575,
+ * and the per-motion fcl shapes of compute_col_shape, robots.cpp:3190-3215).
+ * Only the first max_motions are used (<= 0: all). */
+int dp_upload_primitives(dp_ctx* ctx, const dp_prims* prims, int32_t max_motions);
+
+/* Upload the environment (load_env; obstacles become LDS-staged geometry). */
+int dp_upload_world(dp_ctx* ctx, const dp_world* world);
+
+/* Visited-node set for the duplicate test (T_n, dbastar.cpp:578-579). */
+int dp_visited_reset(dp_ctx* ctx);
+int dp_visited_append(dp_ctx* ctx, const double* states /*[n][nx]*/, int64_t n);
+/* Append the NOVEL pairs of the last dp_expand() (T_n->add, dbastar.cpp:875-876). */
+int dp_visited_append_novel(dp_ctx* ctx);
+int64_t dp_visited_size(const dp_ctx* ctx);
+
+/* Radius query over the visited set (T_n->nearestR): for each query state,
+ * the indices of visited states with distance <= radius, CSR output.
+ * out_offsets has n+1 entries; out_idx capacity idx_cap.  Used for rewiring. */
+int dp_visited_radius(dp_ctx* ctx, const double* queries /*[n][nx]*/, int64_t n, double radius,
+                      int64_t* out_offsets, int64_t* out_idx, int64_t idx_cap);
+
+/* Pre-size device buffers (optional; dp_expand grows them on demand). */
+int dp_reserve(dp_ctx* ctx, int32_t max_nodes, int64_t max_pairs);
+
+/* Copy N frontier nodes (row-major [N][nx]) and their gScore to the device. */
+int dp_upload_nodes(dp_ctx* ctx, const double* nodes, const double* g_parent, int32_t N);
+
+/* Expand the uploaded nodes: enqueue all kernels on `stream` (hipStream_t,
+ * NULL = context stream) and return without synchronising. */
+int dp_expand(dp_ctx* ctx, const dp_expand_params* params, void* stream);
+
+/* dp_upload_nodes + dp_expand + synchronise: the batched counterpart of one
+ * pass of the reference's expansion loop. */
+int dp_expand_batch(dp_ctx* ctx, const double* nodes, const double* g_parent, int32_t N,
+                    const dp_expand_params* params, void* stream);
+
+/* Counters of the last dp_expand (synchronises the stream). */
+int dp_get_counts(dp_ctx* ctx, dp_counts* out);
+
+/* Copy the per-pair results of the last dp_expand into caller buffers, in
+ * canonical (node, primitive) order.  Returns DP_ERR_CAPACITY if the batch
+ * has more pairs than out->capacity. */
+int dp_fetch_pairs(dp_ctx* ctx, dp_pairs_out* out);
+
+/* The stream the context issues its own work on (hipStream_t). */
+void* dp_stream(dp_ctx* ctx);
+
+/* Device time (ms) of the kernels of the last dp_expand, split by stage
+ * (HIP events on the expansion stream). stages: 0 applicability, 1 check,
+ * 2 duplicate resolution, 3 total. */
+int dp_last_timing(dp_ctx* ctx, float* ms4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DYNOPLAN_AMD_H */
