@@ -26,6 +26,10 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def workloads():
     from dynoplan_amd import synth
     return {
@@ -106,15 +110,18 @@ def main():
     ex.upload_nodes(wl.nodes, wl.g_parent)
     ex.reserve(wl.nodes.shape[0], wl.nodes.shape[0] * (wl.offsets.size - 1))
 
+    progress(f"workload {wl.name}: {wl.nodes.shape[0]} nodes, {wl.offsets.size - 1} primitives")
     for _ in range(args.warmup):
         ex.expand_async(wl.params)
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        progress(f"warm-up step done: {ex.timing()}")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ex.expand_async(wl.params)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     counts = ex.counts()
+    progress(f"timed {args.steps} steps in {elapsed:.3f} s: {counts['pairs']} pairs per step")
     pairs = counts["pairs"]
     value = pairs * args.steps / elapsed
 
@@ -155,11 +162,14 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": abytes},
     }
+    progress(f"stage ms {stage_ms}")
     if not args.no_baselines:
         pg = plain_gpu(ex, wl, args.plain_nodes)
+        progress(f"plain GPU path: {pg}")
         line["plain_gpu"] = pg
         line["speedup_vs_plain_gpu"] = value / pg["value"]
         cb = cpu_baseline(wl, args.cpu_nodes)
+        progress(f"cpu baseline: {cb}")
         line["cpu_baseline"] = cb
         line["speedup_vs_cpu"] = value / cb["value"]
     ex.close()

@@ -81,7 +81,7 @@ class Counts(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
         "nodes", "pairs", "bounds", "collide", "free_", "surrogate_reject", "novel",
         "dup_visited", "dup_batch", "capped", "goal_hits", "checked", "recheck_fp64",
-        "resolve_spins")]
+        "resolve_rounds")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

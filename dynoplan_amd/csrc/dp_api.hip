@@ -202,6 +202,7 @@ dp_ctx* dp_create(const dp_system_desc* sys, int32_t device) {
   ok = ok && hipMemset(c->d_counters, 0, sizeof(long long) * DPC_COUNT) == hipSuccess;
   ok = ok && hipMemset(c->d_total, 0, sizeof(int32_t)) == hipSuccess;
   ok = ok && upload_sys(c) == DP_OK;
+  ok = ok && hipHostMalloc((void**)&c->h_pinned, sizeof(long long) * 4) == hipSuccess;
   if (!ok) {
     if (g_err.empty()) dp_set_error("context creation failed");
     dp_destroy(c);
@@ -245,6 +246,14 @@ void dp_destroy(dp_ctx* c) {
   dev_free(&c->d_cont_start);
   dev_free(&c->d_cont_cursor);
   dev_free(&c->d_cont_idx);
+  dev_free(&c->d_cont_x);
+  dev_free(&c->d_conf_off);
+  dev_free(&c->d_conf);
+  dev_free(&c->d_tiles);
+  dev_free(&c->d_node_cursor);
+  dev_free(&c->d_node_count);
+  if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+  c->h_pinned = nullptr;
   dev_free(&c->d_counters);
   dev_free(&c->d_total);
   for (int k = 0; k < 5; ++k)
@@ -467,6 +476,8 @@ int dp_reserve(dp_ctx* c, int32_t max_nodes, int64_t max_pairs) {
     if ((rc = dev_alloc(&c->d_gpar, max_nodes))) return rc;
     if ((rc = dev_alloc(&c->d_app, (int64_t)max_nodes + 1))) return rc;
     if ((rc = dev_alloc(&c->d_node_novel, (int64_t)max_nodes + 1))) return rc;
+    if ((rc = dev_alloc(&c->d_node_cursor, max_nodes))) return rc;
+    if ((rc = dev_alloc(&c->d_node_count, max_nodes))) return rc;
     c->N_cap = max_nodes;
   }
   if (max_pairs > c->pair_cap) {
@@ -483,6 +494,8 @@ int dp_reserve(dp_ctx* c, int32_t max_nodes, int64_t max_pairs) {
     if ((rc = dev_alloc(&c->d_g, max_pairs))) return rc;
     if ((rc = dev_alloc(&c->d_h, max_pairs))) return rc;
     if ((rc = dev_alloc(&c->d_cont_idx, max_pairs))) return rc;
+    if ((rc = dev_alloc(&c->d_cont_x, (int64_t)nx * max_pairs))) return rc;
+    if ((rc = dev_alloc(&c->d_conf_off, max_pairs + 1))) return rc;
     c->pair_cap = max_pairs;
   }
   return DP_OK;
@@ -566,6 +579,17 @@ int dp_get_counts(dp_ctx* c, dp_counts* o) {
   long long h[DPC_COUNT];
   int rc = read_counters(c, h);
   if (rc) return rc;
+  // Resolution rounds run ahead blind; finish any node still undecided.
+  for (int extra = 0; h[DPC_NODES_LEFT] > 0 && c->N > 0; ++extra) {
+    if (extra > 1000) {
+      dp_set_error("duplicate resolution did not converge");
+      return DP_ERR_NOTCONV;
+    }
+    rc = dp_launch_more_rounds(c, 32, c->stream);
+    if (rc) return rc;
+    rc = read_counters(c, h);
+    if (rc) return rc;
+  }
   std::memset(o, 0, sizeof(*o));
   o->nodes = c->N;
   o->pairs = h[DPC_PAIRS];
@@ -580,13 +604,13 @@ int dp_get_counts(dp_ctx* c, dp_counts* o) {
   o->goal_hits = h[DPC_GOAL];
   o->checked = h[DPC_CHECKED];
   o->recheck_fp64 = h[DPC_RECHECK];
-  o->resolve_spins = h[DPC_SPINS];
+  o->resolve_rounds = h[DPC_ROUNDS];
   if (h[DPC_OVERFLOW]) {
     dp_set_error("pair capacity exceeded");
     return DP_ERR_CAPACITY;
   }
   if (h[DPC_ERRFLAG]) {
-    dp_set_error("ordered duplicate resolution hit its spin bound");
+    dp_set_error("device error flag set");
     return DP_ERR_NOTCONV;
   }
   return DP_OK;

@@ -24,16 +24,20 @@ void dp_set_error(const std::string& msg);
 // Device-side counters, one slot per dp_counts field (int64 each) plus flags.
 enum {
   DPC_PAIRS = 0, DPC_BOUNDS, DPC_COLLIDE, DPC_FREE, DPC_SURR, DPC_NOVEL, DPC_DUPV, DPC_DUPB,
-  DPC_CAPPED, DPC_GOAL, DPC_CHECKED, DPC_RECHECK, DPC_SPINS, DPC_ERRFLAG, DPC_OVERFLOW,
-  DPC_TICKET, DPC_COUNT
+  DPC_CAPPED, DPC_GOAL, DPC_CHECKED, DPC_RECHECK, DPC_ROUNDS, DPC_ERRFLAG, DPC_OVERFLOW,
+  DPC_NODES_LEFT, DPC_CONF, DPC_COUNT
 };
 
-// Uniform grid over the position components (2 or 3), cell = r/w0 padded.
+// Uniform grid used by the duplicate tests.  Axes 0,1 are the position
+// (x, y); axis 2 is z for 3-D systems and the heading theta (state[2],
+// periodic) for planar ones.  Cells are at least r/w0 wide on the position
+// axes and r/w1 on the heading axis, so every state within metric distance
+// r of a query lies in the 3x3x3 block of cells around the query's cell.
 struct dp_grid_geom {
   double lo[3];
-  double inv_cell;  // 1 / cell
+  double inv_cell[3];
   int32_t dims[3];
-  int32_t pos_dim;
+  int32_t periodic2;
   int64_t ncells;
 };
 
@@ -41,6 +45,42 @@ __host__ __device__ inline int32_t dp_cell_axis(double x, double lo, double inv_
   double f = (x - lo) * inv_cell;
   int32_t c = f < 0.0 ? 0 : (f >= (double)(dim - 1) ? dim - 1 : (int32_t)f);
   return c;
+}
+
+__host__ __device__ inline int32_t dp_cell_theta(double th, double inv_cell, int32_t dim) {
+  double t = th;
+  if (!(t >= -DP_PI && t < DP_PI)) t = t - DP_TWO_PI * floor((t + DP_PI) / DP_TWO_PI);
+  return dp_cell_axis(t, -DP_PI, inv_cell, dim);
+}
+
+// Base cell coordinates of state x (components 0, 1, 2).
+__host__ __device__ inline void dp_cell_coords(const dp_grid_geom& G, const double* x, int32_t* c) {
+  c[0] = dp_cell_axis(x[0], G.lo[0], G.inv_cell[0], G.dims[0]);
+  c[1] = dp_cell_axis(x[1], G.lo[1], G.inv_cell[1], G.dims[1]);
+  c[2] = G.periodic2 ? dp_cell_theta(x[2], G.inv_cell[2], G.dims[2])
+                     : dp_cell_axis(x[2], G.lo[2], G.inv_cell[2], G.dims[2]);
+}
+
+__host__ __device__ inline int64_t dp_cell_linear(const dp_grid_geom& G, const int32_t* c) {
+  return ((int64_t)c[2] * G.dims[1] + c[1]) * G.dims[0] + c[0];
+}
+
+// Neighbour cq (0..26) of base cell c, or -1 when outside the grid (or a
+// repeat of the base cell on a single-bin heading axis).
+__host__ __device__ inline int64_t dp_cell_neighbor(const dp_grid_geom& G, const int32_t* c, int cq) {
+  const int32_t d0 = cq % 3 - 1, d1 = (cq / 3) % 3 - 1, d2 = cq / 9 - 1;
+  const int32_t n0 = c[0] + d0, n1 = c[1] + d1;
+  if (n0 < 0 || n0 >= G.dims[0] || n1 < 0 || n1 >= G.dims[1]) return -1;
+  int32_t n2 = c[2] + d2;
+  if (G.dims[2] == 1) {
+    if (d2 != 0) return -1;
+    n2 = 0;
+  } else if (G.periodic2) {
+    n2 = n2 < 0 ? n2 + G.dims[2] : (n2 >= G.dims[2] ? n2 - G.dims[2] : n2);
+  } else if (n2 < 0 || n2 >= G.dims[2]) {
+    return -1;
+  }
+  return ((int64_t)n2 * G.dims[1] + n1) * G.dims[0] + n0;
 }
 
 struct dp_ctx {
@@ -106,6 +146,17 @@ struct dp_ctx {
   int32_t* d_cont_idx = nullptr;    // [pair_cap]
   int64_t cont_start_cap = 0, cont_cursor_cap = 0;
   int32_t* d_node_novel = nullptr;  // [N_cap+1] NOVEL pairs per node, then offsets
+  double* d_cont_x = nullptr;       // [nx][pair_cap] contender final states sorted by cell
+  int32_t* d_conf_off = nullptr;    // [pair_cap+1] earlier-conflict list offsets per pair
+  int32_t* d_conf = nullptr;        // [conf_cap] earlier conflicting contenders
+  int64_t conf_cap = 0;
+  int32_t* d_tiles = nullptr;       // [tiles+1] partial sums of the large scan
+  int64_t tiles_cap = 0;
+  int32_t* d_node_cursor = nullptr; // [N_cap] first undecided pair per node
+  int32_t* d_node_count = nullptr;  // [N_cap] NOVEL pairs decided per node
+  long long* h_pinned = nullptr;    // pinned host words for device totals
+  int32_t rounds = 32;              // resolution rounds launched per batch
+  int32_t round_limit = 0;          // limit_branching_factor of the batch in flight
 
   // counters
   long long* d_counters = nullptr;  // [DPC_COUNT]
@@ -118,7 +169,44 @@ struct dp_ctx {
   int64_t last_pairs = -1;
 };
 
-// ---- launchers (k_expand.hip) -------------------------------------------
-int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s);
+#define DP_HIP_RET(expr)                                                        \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      dp_set_error(std::string(#expr) + ": " + hipGetErrorString(e_));          \
+      return DP_ERR_HIP;                                                        \
+    }                                                                           \
+  } while (0)
+
+inline int dp_grid_blocks(int64_t work, int per_block, int max_blocks) {
+  int64_t b = (work + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > max_blocks) b = max_blocks;
+  return (int)b;
+}
+
+int dp_ensure_i32(int32_t** p, int64_t* cap, int64_t need);
+int dp_ensure_f64(double** p, int64_t* cap, int64_t need);
+
+// ---- k_util.hip ---------------------------------------------------------
+void dp_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+void dp_zero_counters(long long* c, hipStream_t s);
+// Exclusive scan of data[0..n-1] into data[0..n]; data[n] = total.  Small n
+// (one workgroup).  total (optional) receives min(total, cap); counters[slot]
+// receives the total when slot >= 0.
+void dp_scan_small(int32_t* data, int64_t n, int32_t* total, int64_t cap, long long* counters, int slot,
+                   hipStream_t s);
+// Same for large arrays whose length lives on the device (*n_dev, at most
+// n_max): three passes over tiles of 4096 elements.
+int dp_scan_large(dp_ctx* c, int32_t* data, const int32_t* n_dev, int64_t n_max, long long* counters, int slot,
+                  hipStream_t s);
+
+// ---- k_grid.hip ---------------------------------------------------------
+dp_grid_geom dp_make_grid(const dp_ctx* c, double r);
 int dp_launch_visited_rebuild(dp_ctx* c, double radius, hipStream_t s);
 int dp_launch_visited_append_novel(dp_ctx* c, int64_t novel_total, hipStream_t s);
+int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStream_t s);
+int dp_launch_more_rounds(dp_ctx* c, int rounds, hipStream_t s);
+
+// ---- k_expand.hip -------------------------------------------------------
+int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s);

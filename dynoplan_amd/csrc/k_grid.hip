@@ -1,0 +1,461 @@
+// k_grid.hip -- duplicate tests of the expansion: the visited-set grid (T_n),
+// the in-batch contender grid, earlier-conflict lists and the round-based
+// resolution that reproduces the reference's sequential insert semantics
+// (dbastar.cpp:850-918) for a whole batch.
+//
+// Sequential semantics restated: in canonical (node, primitive) order, a FREE
+// pair of node i is CAPPED once limit NOVEL pairs of node i precede it;
+// otherwise it is NOVEL when goal-forced (chosen_index != -1) or when no
+// visited state and no earlier NOVEL pair lies within (1-alpha)*delta;
+// otherwise it is a duplicate.
+//
+// Parallel form: each FREE pair that could become NOVEL ("contender") gets
+// the list of earlier contenders within the radius.  Rounds then advance a
+// cursor per node through its pairs in order; a pair is decided as soon as
+// every earlier conflicting contender is decided (or one is NOVEL).  The
+// smallest undecided pair of the batch is always decidable, so the rounds
+// terminate; measured depth on the 4096-node unicycle batch is ~20-30.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "dp_internal.h"
+
+dp_grid_geom dp_make_grid(const dp_ctx* c, double r) {
+  dp_grid_geom G{};
+  const bool planar = c->desc.pos_dim == 2;
+  double cell = (r / c->desc.dist_w[0]) * (1.0 + 1e-9);
+  int32_t nth = 1;
+  if (planar && c->desc.dist_w[1] > 0.0) {
+    double bw = (r / c->desc.dist_w[1]) * (1.0 + 1e-6);
+    double k = std::floor(DP_TWO_PI / bw);
+    nth = k >= 3.0 ? (int32_t)std::min(k, 4096.0) : 1;
+  }
+  const int64_t max_cells = 1ll << 24;
+  for (;;) {
+    int64_t n = 1;
+    for (int d = 0; d < 2 + (planar ? 0 : 1); ++d) {
+      double ext = c->world_hi[d] - c->world_lo[d];
+      int64_t k = (int64_t)(ext / cell) + 3;
+      G.dims[d] = (int32_t)k;
+      G.lo[d] = c->world_lo[d] - cell;
+      G.inv_cell[d] = 1.0 / cell;
+      n *= k;
+    }
+    if (planar) {
+      G.dims[2] = nth;
+      G.lo[2] = -DP_PI;
+      G.inv_cell[2] = (double)nth / DP_TWO_PI;
+      G.periodic2 = 1;
+      n *= nth;
+    }
+    if (n <= max_cells) {
+      G.ncells = n;
+      break;
+    }
+    cell *= 1.5;  // coarser cells stay exact, only slower
+    if (planar && nth > 1) {
+      nth = nth / 2;  // wider heading bins stay >= r/w1
+      if (nth < 3) nth = 1;
+    }
+  }
+  return G;
+}
+
+namespace {
+
+template <int NX>
+__device__ __forceinline__ int64_t cell_of_soa(const dp_grid_geom& G, const double* pts, int64_t stride, int64_t p) {
+  double x[3] = {pts[p], pts[stride + p], pts[2 * stride + p]};
+  int32_t c[3];
+  dp_cell_coords(G, x, c);
+  return dp_cell_linear(G, c);
+}
+
+// ---- visited set ---------------------------------------------------------
+template <int NX>
+__global__ void k_vis_count(dp_grid_geom G, const double* __restrict__ pts, int64_t stride, int64_t n,
+                            int32_t* __restrict__ cellof, int32_t* __restrict__ counts) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = cell_of_soa<NX>(G, pts, stride, p);
+    cellof[p] = (int32_t)c;
+    atomicAdd(&counts[c], 1);
+  }
+}
+
+template <int NX>
+__global__ void k_vis_scatter(const double* __restrict__ pts, int64_t stride, int64_t n,
+                              const int32_t* __restrict__ cellof, const int32_t* __restrict__ start,
+                              int32_t* __restrict__ cursor, double* __restrict__ out) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t c = cellof[p];
+    const int32_t pos = start[c] + atomicAdd(&cursor[c], 1);
+#pragma unroll
+    for (int k = 0; k < NX; ++k) out[(int64_t)k * stride + pos] = pts[(int64_t)k * stride + p];
+  }
+}
+
+// ---- contenders --------------------------------------------------------------
+__device__ __forceinline__ bool is_contender(int8_t verdict, int8_t flags) {
+  return verdict == DP_V_FREE && ((flags & 2) || !(flags & 1));
+}
+
+template <int NX>
+__global__ void k_cont_count(dp_grid_geom G, const int32_t* __restrict__ total, const int8_t* __restrict__ verdict,
+                             const int8_t* __restrict__ flags, const double* __restrict__ fin, int64_t pcap,
+                             int32_t* __restrict__ counts) {
+  const int64_t n = *total;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+    if (!is_contender(verdict[q], flags[q])) continue;
+    atomicAdd(&counts[cell_of_soa<NX>(G, fin, pcap, q)], 1);
+  }
+}
+
+template <int NX>
+__global__ void k_cont_scatter(dp_grid_geom G, const int32_t* __restrict__ total, const int8_t* __restrict__ verdict,
+                               const int8_t* __restrict__ flags, const double* __restrict__ fin, int64_t pcap,
+                               const int32_t* __restrict__ start, int32_t* __restrict__ cursor,
+                               int32_t* __restrict__ idx, double* __restrict__ xs) {
+  const int64_t n = *total;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+    if (!is_contender(verdict[q], flags[q])) continue;
+    const int64_t c = cell_of_soa<NX>(G, fin, pcap, q);
+    const int32_t pos = start[c] + atomicAdd(&cursor[c], 1);
+    idx[pos] = (int32_t)q;
+#pragma unroll
+    for (int k = 0; k < NX; ++k) xs[(int64_t)k * pcap + pos] = fin[(int64_t)k * pcap + q];
+  }
+}
+
+// Earlier-conflict lists: for each pair that needs them (FREE, neither
+// goal-forced nor a duplicate of the visited set), the contenders o < q
+// within r.  FILL=false writes counts, FILL=true the lists.
+template <int NX, bool FILL>
+__global__ void k_conf(const dp_dev_sys* __restrict__ Sp, dp_grid_geom G, const int32_t* __restrict__ total,
+                       const int8_t* __restrict__ verdict, const int8_t* __restrict__ flags,
+                       const double* __restrict__ fin, int64_t pcap, const int32_t* __restrict__ cstart,
+                       const int32_t* __restrict__ cidx, const double* __restrict__ cx, double r,
+                       int32_t* __restrict__ conf_off, int32_t* __restrict__ conf, int64_t conf_cap,
+                       long long* __restrict__ counters) {
+  const dp_dev_sys& S = *Sp;
+  const int64_t n = *total;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+    const int8_t v = verdict[q], f = flags[q];
+    const bool checker = v == DP_V_FREE && !(f & 2) && !(f & 1);
+    int32_t cnt = 0;
+    if (checker) {
+      double x[NX];
+#pragma unroll
+      for (int k = 0; k < NX; ++k) x[k] = fin[(int64_t)k * pcap + q];
+      int32_t c[3];
+      dp_cell_coords(G, x, c);
+      int64_t w = FILL ? (int64_t)conf_off[q] : 0;
+      for (int cq = 0; cq < 27; ++cq) {
+        const int64_t cell = dp_cell_neighbor(G, c, cq);
+        if (cell < 0) continue;
+        const int32_t e = cstart[cell + 1];
+        for (int32_t t = cstart[cell]; t < e; ++t) {
+          const int32_t o = cidx[t];
+          if (o >= q) continue;
+          double y[NX];
+#pragma unroll
+          for (int k = 0; k < NX; ++k) y[k] = cx[(int64_t)k * pcap + t];
+          if (!(dp_distance(S, x, y) <= r)) continue;
+          if (FILL) {
+            if (w < conf_cap) conf[w] = o;
+            ++w;
+          }
+          ++cnt;
+        }
+      }
+    }
+    if (!FILL) conf_off[q] = cnt;
+  }
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
+  uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+  uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Counters were zeroed at the start of the batch; nodes with pairs to decide
+// are counted into DPC_NODES_LEFT, empty nodes are finished here.
+__global__ void k_round_init(int32_t N, const int32_t* __restrict__ app, const int32_t* __restrict__ total,
+                             int32_t* __restrict__ cursor, int32_t* __restrict__ count,
+                             int32_t* __restrict__ node_novel, long long* counters) {
+  const int32_t tot = *total;
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    const int32_t b = app[i] < tot ? app[i] : tot;
+    const int32_t e = app[i + 1] < tot ? app[i + 1] : tot;
+    cursor[i] = b;
+    count[i] = 0;
+    node_novel[i] = 0;
+    if (b < e) atomicAdd((unsigned long long*)&counters[DPC_NODES_LEFT], 1ull);
+  }
+}
+
+struct round_args {
+  int32_t N;
+  const int32_t* app;
+  const int32_t* total;
+  const int8_t* verdict;
+  const int8_t* flags;
+  const int32_t* conf_off;
+  const int32_t* conf;
+  int32_t* res;
+  int32_t* cursor;
+  int32_t* count;
+  int32_t* node_novel;
+  int32_t limit;
+  long long* counters;
+};
+
+// One resolution round: one wave per node advances the node's cursor through
+// its pairs in canonical order until a pair waits on an undecided earlier
+// contender of another node.
+__global__ __launch_bounds__(64) void k_round(round_args a) {
+  const int lane = threadIdx.x;
+  if (__hip_atomic_load(&a.counters[DPC_NODES_LEFT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+  const int32_t node = blockIdx.x;
+  if (node >= a.N) return;
+  if (node == 0 && lane == 0) atomicAdd((unsigned long long*)&a.counters[DPC_ROUNDS], 1ull);
+  const int32_t tot = *a.total;
+  const int32_t end = a.app[node + 1] < tot ? a.app[node + 1] : tot;
+  int32_t cur = a.cursor[node];
+  if (cur >= end) return;
+  int32_t count = a.count[node];
+  long long n_nov = 0, n_cap = 0, n_dv = 0, n_db = 0;
+  bool stopped = false;
+  for (int32_t cb = cur; cb < end; cb += 64) {
+    const int32_t p = cb + lane;
+    const bool fr = p < end && a.verdict[p] == DP_V_FREE;
+    const int8_t fl = fr ? a.flags[p] : 0;
+    const bool forced = (fl & 2) != 0;
+    const bool dupv = (fl & 1) != 0;
+    bool ext_dup = false, ext_block = false;
+    uint64_t same = 0;
+    if (fr && !forced && !dupv) {
+      const int32_t e1 = a.conf_off[p + 1];
+      for (int32_t e = a.conf_off[p]; e < e1 && !ext_dup; ++e) {
+        const int32_t o = a.conf[e];
+        if (o >= cb) {
+          same |= 1ull << (o - cb);
+          continue;
+        }
+        const int r = __hip_atomic_load(&a.res[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (r == DP_R_NOVEL) ext_dup = true;
+        else if (r == DP_R_UNDECIDED) ext_block = true;
+      }
+    }
+    const uint64_t free_m = __ballot(fr);
+    const uint64_t forced_m = __ballot(fr && forced);
+    const uint64_t dupv_m = __ballot(fr && dupv && !forced);
+    const uint64_t dup_m = __ballot(ext_dup);
+    const uint64_t block_m = __ballot(ext_block && !ext_dup);
+    uint64_t novel_m = 0, capped_m = 0, dupb_m = 0, dupvr_m = 0;
+    int stop_lane = 64;
+    uint64_t rem = free_m;
+    while (rem) {
+      const int b = __ffsll((long long)rem) - 1;
+      rem &= rem - 1ull;
+      const uint64_t bit = 1ull << b;
+      if (a.limit > 0 && count + __popcll(novel_m) >= a.limit) capped_m |= bit;
+      else if (forced_m & bit) novel_m |= bit;
+      else if (dupv_m & bit) dupvr_m |= bit;
+      else if (dup_m & bit) dupb_m |= bit;
+      else if (block_m & bit) {
+        stop_lane = b;
+        break;
+      } else if (readlane64(same, b) & novel_m) dupb_m |= bit;
+      else novel_m |= bit;
+    }
+    const uint64_t bit = 1ull << lane;
+    if (fr && ((novel_m | capped_m | dupb_m | dupvr_m) & bit)) {
+      const int r = (capped_m & bit) ? DP_R_CAPPED
+                    : (novel_m & bit) ? DP_R_NOVEL
+                    : (dupvr_m & bit) ? DP_R_DUP_VISITED
+                                      : DP_R_DUP_BATCH;
+      __hip_atomic_store(&a.res[p], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    count += __popcll(novel_m);
+    n_nov += __popcll(novel_m);
+    n_cap += __popcll(capped_m);
+    n_dv += __popcll(dupvr_m);
+    n_db += __popcll(dupb_m);
+    if (stop_lane < 64) {
+      cur = cb + stop_lane;
+      stopped = true;
+      break;
+    }
+  }
+  if (lane == 0) {
+    a.cursor[node] = stopped ? cur : end;
+    a.count[node] = count;
+    if (!stopped) {
+      if (a.node_novel) a.node_novel[node] = count;
+      atomicAdd((unsigned long long*)&a.counters[DPC_NODES_LEFT], (unsigned long long)-1ll);
+    }
+    if (n_nov) atomicAdd((unsigned long long*)&a.counters[DPC_NOVEL], (unsigned long long)n_nov);
+    if (n_cap) atomicAdd((unsigned long long*)&a.counters[DPC_CAPPED], (unsigned long long)n_cap);
+    if (n_dv) atomicAdd((unsigned long long*)&a.counters[DPC_DUPV], (unsigned long long)n_dv);
+    if (n_db) atomicAdd((unsigned long long*)&a.counters[DPC_DUPB], (unsigned long long)n_db);
+  }
+}
+
+// Append the NOVEL final states of the batch to the visited set in canonical
+// order; dst holds the exclusive scan of NOVEL pairs per node.
+template <int NX>
+__global__ __launch_bounds__(64) void k_gather_novel(int32_t N, const int32_t* __restrict__ app,
+                                                     const int32_t* __restrict__ total,
+                                                     const int32_t* __restrict__ res, const int32_t* __restrict__ dst,
+                                                     const double* __restrict__ fin, int64_t pcap,
+                                                     double* __restrict__ vis, int64_t vstride, int64_t vbase) {
+  const int32_t node = blockIdx.x;
+  if (node >= N) return;
+  const int lane = threadIdx.x;
+  const int32_t tot = *total;
+  const int32_t begin = app[node] < tot ? app[node] : tot;
+  const int32_t end = app[node + 1] < tot ? app[node + 1] : tot;
+  int64_t w = vbase + dst[node];
+  for (int32_t cb = begin; cb < end; cb += 64) {
+    const int32_t p = cb + lane;
+    const bool nov = p < end && res[p] == DP_R_NOVEL;
+    const uint64_t m = __ballot(nov);
+    if (nov) {
+      const int64_t pos = w + __popcll(m & ((1ull << lane) - 1ull));
+#pragma unroll
+      for (int c = 0; c < NX; ++c) vis[(int64_t)c * vstride + pos] = fin[(int64_t)c * pcap + p];
+    }
+    w += __popcll(m);
+  }
+}
+
+template <typename F>
+int dispatch_nx(int nx, F&& f) {
+  switch (nx) {
+    case 3: return f(std::integral_constant<int, 3>{});
+    case 4: return f(std::integral_constant<int, 4>{});
+    case 5: return f(std::integral_constant<int, 5>{});
+    case 13: return f(std::integral_constant<int, 13>{});
+  }
+  dp_set_error("unsupported nx");
+  return DP_ERR_ARG;
+}
+
+}  // namespace
+
+int dp_launch_visited_rebuild(dp_ctx* c, double radius, hipStream_t s) {
+  c->grid = dp_make_grid(c, radius);
+  c->grid_radius = radius;
+  const int64_t nc = c->grid.ncells;
+  int rc = dp_ensure_i32(&c->d_vis_start, &c->vis_cells_cap, nc + 1);
+  if (rc) return rc;
+  dp_fill_i32(c->d_vis_start, nc + 1, 0, s);
+  if (c->nv > 0) {
+    rc = dp_ensure_i32(&c->d_cont_cursor, &c->cont_cursor_cap, nc + 1);
+    if (rc) return rc;
+    rc = dispatch_nx(c->desc.nx, [&](auto NXc) -> int {
+      constexpr int NX = decltype(NXc)::value;
+      k_vis_count<NX><<<dp_grid_blocks(c->nv, 256, 8192), 256, 0, s>>>(c->grid, c->d_vis, c->nv_cap, c->nv,
+                                                                      c->d_vis_cell, c->d_vis_start);
+      dp_scan_small(c->d_vis_start, nc, nullptr, INT64_MAX, c->d_counters, -1, s);
+      dp_fill_i32(c->d_cont_cursor, nc, 0, s);
+      k_vis_scatter<NX><<<dp_grid_blocks(c->nv, 256, 8192), 256, 0, s>>>(c->d_vis, c->nv_cap, c->nv, c->d_vis_cell,
+                                                                        c->d_vis_start, c->d_cont_cursor,
+                                                                        c->d_vis_sorted);
+      return DP_OK;
+    });
+    if (rc) return rc;
+  }
+  DP_HIP_RET(hipGetLastError());
+  c->grid_dirty = false;
+  return DP_OK;
+}
+
+int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStream_t s) {
+  const int64_t nc = c->grid.ncells;
+  const int32_t N = c->N;
+  int rc = dp_ensure_i32(&c->d_cont_start, &c->cont_start_cap, nc + 1);
+  if (!rc) rc = dp_ensure_i32(&c->d_cont_cursor, &c->cont_cursor_cap, nc + 1);
+  if (rc) return rc;
+  const int pb = dp_grid_blocks(c->pair_cap, 256, 8192);
+  rc = dispatch_nx(c->desc.nx, [&](auto NXc) -> int {
+    constexpr int NX = decltype(NXc)::value;
+    dp_fill_i32(c->d_cont_start, nc + 1, 0, s);
+    k_cont_count<NX><<<pb, 256, 0, s>>>(c->grid, c->d_total, c->d_verdict, c->d_flags, c->d_final, c->pair_cap,
+                                        c->d_cont_start);
+    dp_scan_small(c->d_cont_start, nc, nullptr, INT64_MAX, c->d_counters, -1, s);
+    dp_fill_i32(c->d_cont_cursor, nc, 0, s);
+    k_cont_scatter<NX><<<pb, 256, 0, s>>>(c->grid, c->d_total, c->d_verdict, c->d_flags, c->d_final, c->pair_cap,
+                                          c->d_cont_start, c->d_cont_cursor, c->d_cont_idx, c->d_cont_x);
+    k_conf<NX, false><<<pb, 256, 0, s>>>(c->d_sys, c->grid, c->d_total, c->d_verdict, c->d_flags, c->d_final,
+                                         c->pair_cap, c->d_cont_start, c->d_cont_idx, c->d_cont_x, r_dup,
+                                         c->d_conf_off, nullptr, 0, c->d_counters);
+    return DP_OK;
+  });
+  if (rc) return rc;
+  rc = dp_scan_large(c, c->d_conf_off, c->d_total, c->pair_cap, c->d_counters, DPC_CONF, s);
+  if (rc) return rc;
+  // The list length decides the allocation: one device->host word per batch.
+  DP_HIP_RET(hipMemcpyAsync(c->h_pinned, c->d_counters + DPC_CONF, sizeof(long long), hipMemcpyDeviceToHost, s));
+  DP_HIP_RET(hipStreamSynchronize(s));
+  const int64_t nconf = c->h_pinned[0];
+  if (nconf > c->conf_cap) {
+    rc = dp_ensure_i32(&c->d_conf, &c->conf_cap, nconf + nconf / 4 + 1024);
+    if (rc) return rc;
+  }
+  rc = dispatch_nx(c->desc.nx, [&](auto NXc) -> int {
+    constexpr int NX = decltype(NXc)::value;
+    k_conf<NX, true><<<pb, 256, 0, s>>>(c->d_sys, c->grid, c->d_total, c->d_verdict, c->d_flags, c->d_final,
+                                        c->pair_cap, c->d_cont_start, c->d_cont_idx, c->d_cont_x, r_dup,
+                                        c->d_conf_off, c->d_conf, c->conf_cap, c->d_counters);
+    return DP_OK;
+  });
+  if (rc) return rc;
+  k_round_init<<<dp_grid_blocks(N, 256, 1024), 256, 0, s>>>(N, c->d_app, c->d_total, c->d_node_cursor,
+                                                            c->d_node_count, c->d_node_novel, c->d_counters);
+  c->round_limit = p->limit_branching_factor;
+  rc = dp_launch_more_rounds(c, c->rounds, s);
+  if (rc) return rc;
+  DP_HIP_RET(hipGetLastError());
+  return DP_OK;
+}
+
+int dp_launch_more_rounds(dp_ctx* c, int rounds, hipStream_t s) {
+  round_args a{};
+  a.N = c->N;
+  a.app = c->d_app;
+  a.total = c->d_total;
+  a.verdict = c->d_verdict;
+  a.flags = c->d_flags;
+  a.conf_off = c->d_conf_off;
+  a.conf = c->d_conf;
+  a.res = c->d_res;
+  a.cursor = c->d_node_cursor;
+  a.count = c->d_node_count;
+  a.node_novel = c->d_node_novel;
+  a.limit = c->round_limit;
+  a.counters = c->d_counters;
+  for (int r = 0; r < rounds; ++r) k_round<<<c->N, 64, 0, s>>>(a);
+  DP_HIP_RET(hipGetLastError());
+  return DP_OK;
+}
+
+int dp_launch_visited_append_novel(dp_ctx* c, int64_t novel_total, hipStream_t s) {
+  // Caller guarantees visited capacity >= nv + novel_total.
+  const int32_t N = c->N;
+  if (N <= 0 || novel_total <= 0) return DP_OK;
+  dp_scan_small(c->d_node_novel, N, nullptr, INT64_MAX, c->d_counters, -1, s);
+  int rc = dispatch_nx(c->desc.nx, [&](auto NXc) -> int {
+    constexpr int NX = decltype(NXc)::value;
+    k_gather_novel<NX><<<N, 64, 0, s>>>(N, c->d_app, c->d_total, c->d_res, c->d_node_novel, c->d_final,
+                                        c->pair_cap, c->d_vis, c->nv_cap, c->nv);
+    return DP_OK;
+  });
+  if (rc) return rc;
+  DP_HIP_RET(hipGetLastError());
+  c->nv += novel_total;
+  c->grid_dirty = true;
+  return DP_OK;
+}

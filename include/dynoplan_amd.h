@@ -136,7 +136,7 @@ typedef struct dp_counts {
   int64_t goal_hits;        /* FREE pairs with chosen_index != -1 */
   int64_t checked;          /* pairs that reached the collision check (num_col_motions) */
   int64_t recheck_fp64;     /* FP32 pairs re-evaluated in FP64 near contact */
-  int64_t resolve_spins;    /* diagnostic: waits in the ordered duplicate resolution */
+  int64_t resolve_rounds;   /* diagnostic: rounds of the batch duplicate resolution */
 } dp_counts;
 
 /* Caller-owned host buffers for dp_fetch_pairs (any pointer may be NULL). */

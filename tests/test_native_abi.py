@@ -1,0 +1,90 @@
+"""The C-ABI library loads and exports what include/dynoplan_amd.h declares
+(no GPU needed; no compute call is made)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from dynoplan_amd import _native as N
+from dynoplan_amd import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "dynoplan_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dp_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(N.LIB_PATH), "build with `python -m dynoplan_amd.build` first"
+    out = subprocess.run(["nm", "-D", "--defined-only", N.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (dp_[a-z0-9_]+)\b", out))
+    decl = declared_functions()
+    assert len(decl) >= 19
+    missing = [f for f in decl if f not in exported]
+    assert not missing, missing
+    assert set(decl) == set(N.SIGNATURES), "ctypes signature table out of date"
+
+
+def test_library_loads_with_signatures():
+    lib = N.load()
+    for name in N.SIGNATURES:
+        assert getattr(lib, name) is not None
+
+
+STRUCTS = {
+    "dp_system_desc": (N.SystemDesc, ["dt", "x_lb", "x_ub", "dist_w", "max_vel", "body_half", "quad", "near_eps"]),
+    "dp_prims": (N.Prims, ["offsets", "states", "actions", "cost"]),
+    "dp_world": (N.World, ["lo", "hi", "boxes", "spheres"]),
+    "dp_expand_params": (N.ExpandParams, ["alpha", "delta_factor_goal", "limit_branching_factor", "fp32_rollout",
+                                          "goal"]),
+    "dp_counts": (N.Counts, ["pairs", "checked", "resolve_rounds"]),
+    "dp_pairs_out": (N.PairsOut, ["node", "verdict", "final_state", "h"]),
+}
+
+
+def test_struct_layout_matches_c_compiler(tmp_path):
+    lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void){"]
+    for sname, (_, fields) in STRUCTS.items():
+        lines.append(f'printf("{sname} %zu\\n", sizeof({sname}));')
+        for f in fields:
+            lines.append(f'printf("{sname}.{f} %zu\\n", offsetof({sname}, {f}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True).stdout.splitlines())
+    for sname, (cls, fields) in STRUCTS.items():
+        assert int(got[sname]) == C.sizeof(cls), sname
+        for f in fields:
+            assert int(got[f"{sname}.{f}"]) == getattr(cls, f).offset, f"{sname}.{f}"
+
+
+@pytest.mark.parametrize("system", [N.SYS_UNICYCLE1, N.SYS_UNICYCLE2, N.SYS_QUADROTOR, N.SYS_CAR_TRAILER])
+def test_system_defaults_match_static_copy(system):
+    """The oracle tests build descriptions without the library; both must
+    agree field by field with dp_system_default."""
+    lib = N.load()
+    d = N.SystemDesc()
+    assert lib.dp_system_default(system, C.byref(d)) == N.DP_OK
+    s = synth.default_desc_static(system)
+    assert bytes(d) == bytes(s)
+
+
+def test_unknown_system_and_null_errors():
+    lib = N.load()
+    d = N.SystemDesc()
+    assert lib.dp_system_default(99, C.byref(d)) == N.DP_ERR_ARG
+    assert b"unknown" in lib.dp_last_error()
+    assert lib.dp_system_default(0, None) == N.DP_ERR_ARG
+    bad = N.SystemDesc()
+    lib.dp_system_default(N.SYS_UNICYCLE1, C.byref(bad))
+    bad.nx = 5  # system id / nx mismatch is refused before any device call
+    assert not lib.dp_create(C.byref(bad), 0)
+    assert b"mismatch" in lib.dp_last_error()

@@ -139,8 +139,8 @@ def test_deterministic_and_append_novel(need_gpu):
         a = ex.fetch()
         c2 = ex.expand(wl.nodes, wl.g_parent, wl.params)
         b = ex.fetch()
-        assert {k: v for k, v in c1.items() if k != "resolve_spins"} == \
-            {k: v for k, v in c2.items() if k != "resolve_spins"}
+        assert {k: v for k, v in c1.items() if k != "resolve_rounds"} == \
+            {k: v for k, v in c2.items() if k != "resolve_rounds"}
         for k in a:
             assert np.array_equal(a[k], b[k], equal_nan=a[k].dtype == np.float64), k
         nv0 = ex.visited_size()
