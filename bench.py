@@ -36,14 +36,19 @@ def workloads():
         "unicycle1_scale": synth.unicycle1_scale,
         "unicycle1_small": synth.unicycle1_small,
         "car_trailer": synth.car_trailer,
+        "quadrotor": synth.quadrotor,
     }
 
 
 def algorithmic_bytes(wl, prim):
-    """Per pair: the translated primitive's states streamed (nx doubles per
-    state) plus an 8-byte verdict record."""
-    lens = np.diff(wl.offsets.astype(np.int64))
-    return float(np.sum(lens[prim] * wl.desc.nx * 8 + 8))
+    """Bytes one pair must stream.  Translate systems: the primitive's states
+    (nx FP64 per state) plus an 8-byte verdict.  Quadrotor: the FP32 actions
+    (4 x 4 B per step) plus the FP32 start state (13 x 4 B) plus the verdict."""
+    from dynoplan_amd import _native as N
+    lens = np.diff(wl.offsets.astype(np.int64))[prim]
+    if wl.desc.system == N.SYS_QUADROTOR:
+        return float(np.sum((lens - 1) * 16 + 52 + 8))
+    return float(np.sum(lens * wl.desc.nx * 8 + 8))
 
 
 def cpu_baseline(wl, n_nodes):
@@ -105,6 +110,8 @@ def main():
     from dynoplan_amd.expander import expander_for
 
     wl = workloads()[args.workload]()
+    from dynoplan_amd import _native as N
+    is_quad = wl.desc.system == N.SYS_QUADROTOR
     torch.cuda.init()
     ex = expander_for(wl, device=0)
     ex.upload_nodes(wl.nodes, wl.g_parent)
@@ -149,7 +156,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 rollout + f64 re-check" if is_quad else "f64",
         "data": "synthetic",
         "config": {"workload": wl.name, "nodes": int(wl.nodes.shape[0]), "primitives": int(wl.offsets.size - 1),
                    "obstacles": int(wl.boxes.shape[0] + wl.spheres.shape[0]), "visited": int(wl.visited.shape[0]),
@@ -158,7 +165,7 @@ def main():
                    "pairs_per_step": int(pairs)},
         "counts": counts,
         "stage_ms": stage_ms,
-        "roofline": {"bound": "hbm", "kernel": "k_check_planar", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": "k_quad32+k_quad64" if is_quad else "k_check_planar", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": abytes},
     }

@@ -121,6 +121,7 @@ SIGNATURES = {
     "dp_fetch_pairs": (C.c_int, [C.c_void_p, C.POINTER(PairsOut)]),
     "dp_stream": (C.c_void_p, [C.c_void_p]),
     "dp_last_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "dp_debug_fp32_positions": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int64]),
 }
 
 _lib = None

@@ -252,6 +252,8 @@ void dp_destroy(dp_ctx* c) {
   dev_free(&c->d_tiles);
   dev_free(&c->d_node_cursor);
   dev_free(&c->d_node_count);
+  dev_free(&c->d_chunk);
+  dev_free(&c->d_dbg32);
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
   c->h_pinned = nullptr;
   dev_free(&c->d_counters);
@@ -369,6 +371,12 @@ int dp_upload_primitives(dp_ctx* c, const dp_prims* pr, int32_t max_motions) {
     if ((rc = dev_alloc(&c->d_act32, (int64_t)asoa32.size()))) return rc;
     HIP_CHECK(hipMemcpy(c->d_act, asoa.data(), sizeof(double) * asoa.size(), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(c->d_act32, asoa32.data(), sizeof(float) * asoa32.size(), hipMemcpyHostToDevice));
+    double reach = 0.0;
+    for (int32_t j = 0; j < P; ++j)
+      for (int32_t s = off[j]; s < off[j + 1]; ++s)
+        for (int k = 0; k < 3; ++k)
+          reach = std::max(reach, std::fabs(st[(int64_t)s * nx + k] - st[(int64_t)off[j] * nx + k]));
+    c->prim_reach = reach;
   }
   c->P = P;
   c->S = S;
@@ -478,6 +486,7 @@ int dp_reserve(dp_ctx* c, int32_t max_nodes, int64_t max_pairs) {
     if ((rc = dev_alloc(&c->d_node_novel, (int64_t)max_nodes + 1))) return rc;
     if ((rc = dev_alloc(&c->d_node_cursor, max_nodes))) return rc;
     if ((rc = dev_alloc(&c->d_node_count, max_nodes))) return rc;
+    if ((rc = dev_alloc(&c->d_chunk, (int64_t)max_nodes + 1))) return rc;
     c->N_cap = max_nodes;
   }
   if (max_pairs > c->pair_cap) {
@@ -496,6 +505,7 @@ int dp_reserve(dp_ctx* c, int32_t max_nodes, int64_t max_pairs) {
     if ((rc = dev_alloc(&c->d_cont_idx, max_pairs))) return rc;
     if ((rc = dev_alloc(&c->d_cont_x, (int64_t)nx * max_pairs))) return rc;
     if ((rc = dev_alloc(&c->d_conf_off, max_pairs + 1))) return rc;
+    if (c->desc.system == DP_SYS_QUADROTOR && (rc = dev_alloc(&c->d_dbg32, 3 * max_pairs))) return rc;
     c->pair_cap = max_pairs;
   }
   return DP_OK;
@@ -609,7 +619,7 @@ int dp_get_counts(dp_ctx* c, dp_counts* o) {
     dp_set_error("pair capacity exceeded");
     return DP_ERR_CAPACITY;
   }
-  if (h[DPC_ERRFLAG]) {
+  if (h[DPC_ERRFLAG] & ~2ll) {
     dp_set_error("device error flag set");
     return DP_ERR_NOTCONV;
   }
@@ -731,6 +741,20 @@ int dp_visited_radius(dp_ctx* c, const double* queries, int64_t n, double radius
     dp_set_error("radius query output capacity exceeded");
     return DP_ERR_CAPACITY;
   }
+  return DP_OK;
+}
+
+int dp_debug_fp32_positions(dp_ctx* c, float* out, int64_t n) {
+  ARG_CHECK(c && out && n >= 0, "invalid argument");
+  if (!c->d_dbg32) {
+    dp_set_error("no FP32 rollout data (quadrotor contexts only)");
+    return DP_ERR_STATE;
+  }
+  ARG_CHECK(n <= c->pair_cap, "n exceeds the pair capacity");
+  HIP_CHECK(hipSetDevice(c->device));
+  HIP_CHECK(hipDeviceSynchronize());
+  for (int k = 0; k < 3; ++k)
+    HIP_CHECK(hipMemcpy(out + k * n, c->d_dbg32 + (size_t)k * c->pair_cap, sizeof(float) * n, hipMemcpyDeviceToHost));
   return DP_OK;
 }
 

@@ -15,9 +15,9 @@
 // Pad added to swept boxes of the broad phase (m): bounds the rounding of
 // "canonical sweep + offset" versus per-state "state + offset" by 1e6x.
 #define DP_SWEEP_PAD 1e-9
-// Bound on spin iterations of one wave in the ordered resolution; reaching
-// it sets an error flag instead of waiting forever.
-#define DP_SPIN_LIMIT (1 << 24)
+// Bound on back-off iterations of one wave in the ticketed resolution;
+// reaching it leaves the node to the host-driven rounds instead of waiting.
+#define DP_SPIN_LIMIT (1 << 20)
 
 void dp_set_error(const std::string& msg);
 
@@ -25,7 +25,7 @@ void dp_set_error(const std::string& msg);
 enum {
   DPC_PAIRS = 0, DPC_BOUNDS, DPC_COLLIDE, DPC_FREE, DPC_SURR, DPC_NOVEL, DPC_DUPV, DPC_DUPB,
   DPC_CAPPED, DPC_GOAL, DPC_CHECKED, DPC_RECHECK, DPC_ROUNDS, DPC_ERRFLAG, DPC_OVERFLOW,
-  DPC_NODES_LEFT, DPC_CONF, DPC_COUNT
+  DPC_NODES_LEFT, DPC_CONF, DPC_TICKET, DPC_COUNT
 };
 
 // Uniform grid used by the duplicate tests.  Axes 0,1 are the position
@@ -156,6 +156,9 @@ struct dp_ctx {
   int32_t* d_node_count = nullptr;  // [N_cap] NOVEL pairs decided per node
   long long* h_pinned = nullptr;    // pinned host words for device totals
   int32_t rounds = 32;              // resolution rounds launched per batch
+  int32_t* d_chunk = nullptr;       // [N_cap+1] quadrotor: 64-pair chunks per node, scanned
+  float* d_dbg32 = nullptr;         // [3][pair_cap] quadrotor: FP32 final positions (error report)
+  double prim_reach = 0.0;          // max over primitives of max_k |p_k - p_0|_inf
   int32_t round_limit = 0;          // limit_branching_factor of the batch in flight
 
   // counters
@@ -207,6 +210,9 @@ int dp_launch_visited_rebuild(dp_ctx* c, double radius, hipStream_t s);
 int dp_launch_visited_append_novel(dp_ctx* c, int64_t novel_total, hipStream_t s);
 int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStream_t s);
 int dp_launch_more_rounds(dp_ctx* c, int rounds, hipStream_t s);
+
+// ---- k_quad.hip ---------------------------------------------------------
+int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, double r_dup, hipStream_t s);
 
 // ---- k_expand.hip -------------------------------------------------------
 int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s);

@@ -401,8 +401,14 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   a.counters = c->d_counters;
 
   if (c->desc.system == DP_SYS_QUADROTOR) {
-    dp_set_error("quadrotor expansion kernel not available in this build");
-    return DP_ERR_ARG;
+    rc = dp_launch_quad(c, p, goal_bound, r_dup, s);
+    if (rc) return rc;
+    DP_HIP_RET(hipEventRecord(c->ev[2], s));
+    rc = dp_launch_resolve(c, p, r_dup, s);
+    if (rc) return rc;
+    DP_HIP_RET(hipEventRecord(c->ev[3], s));
+    c->timing_valid = true;
+    return DP_OK;
   }
   const int lpp = c->maxlen <= 16 ? 16 : (c->maxlen <= 32 ? 32 : 64);
   const int seg = 64 / lpp;

@@ -211,20 +211,16 @@ struct round_args {
   long long* counters;
 };
 
-// One resolution round: one wave per node advances the node's cursor through
-// its pairs in canonical order until a pair waits on an undecided earlier
-// contender of another node.
-__global__ __launch_bounds__(64) void k_round(round_args a) {
-  const int lane = threadIdx.x;
-  if (__hip_atomic_load(&a.counters[DPC_NODES_LEFT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
-  const int32_t node = blockIdx.x;
-  if (node >= a.N) return;
-  if (node == 0 && lane == 0) atomicAdd((unsigned long long*)&a.counters[DPC_ROUNDS], 1ull);
+// Advance node `node`'s cursor through its pairs in canonical order until a
+// pair waits on an undecided earlier contender of another node.  Returns
+// true when the node is finished.  Executed by one whole wave.
+// cur / count are wave-uniform registers carried across retries; they are
+// also stored for the round kernel, which reloads them in a later launch.
+__device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, int32_t& count) {
+  const int lane = threadIdx.x & 63;
   const int32_t tot = *a.total;
   const int32_t end = a.app[node + 1] < tot ? a.app[node + 1] : tot;
-  int32_t cur = a.cursor[node];
-  if (cur >= end) return;
-  int32_t count = a.count[node];
+  if (cur >= end) return true;
   long long n_nov = 0, n_cap = 0, n_dv = 0, n_db = 0;
   bool stopped = false;
   for (int32_t cb = cur; cb < end; cb += 64) {
@@ -289,8 +285,9 @@ __global__ __launch_bounds__(64) void k_round(round_args a) {
       break;
     }
   }
+  if (!stopped) cur = end;
   if (lane == 0) {
-    a.cursor[node] = stopped ? cur : end;
+    a.cursor[node] = cur;
     a.count[node] = count;
     if (!stopped) {
       if (a.node_novel) a.node_novel[node] = count;
@@ -301,6 +298,41 @@ __global__ __launch_bounds__(64) void k_round(round_args a) {
     if (n_dv) atomicAdd((unsigned long long*)&a.counters[DPC_DUPV], (unsigned long long)n_dv);
     if (n_db) atomicAdd((unsigned long long*)&a.counters[DPC_DUPB], (unsigned long long)n_db);
   }
+  return !stopped;
+}
+
+// One resolution round (host-driven completion path): one wave per node.
+__global__ __launch_bounds__(64) void k_round(round_args a) {
+  if (__hip_atomic_load(&a.counters[DPC_NODES_LEFT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+  const int32_t node = blockIdx.x;
+  if (node >= a.N) return;
+  if (node == 0 && threadIdx.x == 0) atomicAdd((unsigned long long*)&a.counters[DPC_ROUNDS], 1ull);
+  int32_t cur = a.cursor[node], count = a.count[node];
+  (void)advance_node(a, node, cur, count);
+}
+
+// Persistent resolution: waves claim nodes in ascending order through a
+// ticket and advance them until finished, backing off while blocked.  A
+// wave only waits for nodes with smaller tickets, all held by resident
+// waves, so the smallest unfinished node never waits.  The spin is bounded:
+// a wave that exhausts it leaves its node to the round kernel and raises a
+// flag the host checks.
+__global__ __launch_bounds__(64) void k_resolve_ticketed(round_args a) {
+  __shared__ int32_t s_node;
+  if (threadIdx.x == 0) s_node = (int32_t)atomicAdd((unsigned long long*)&a.counters[DPC_TICKET], 1ull);
+  __syncthreads();
+  const int32_t node = s_node;
+  if (node >= a.N) return;
+  int32_t spins = 0;
+  int32_t cur = a.cursor[node], count = a.count[node];
+  while (!advance_node(a, node, cur, count)) {
+    if (++spins > DP_SPIN_LIMIT) {
+      if (threadIdx.x == 0) atomicOr((unsigned long long*)&a.counters[DPC_ERRFLAG], 2ull);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+  if (threadIdx.x == 0 && spins) atomicAdd((unsigned long long*)&a.counters[DPC_ROUNDS], (unsigned long long)spins);
 }
 
 // Append the NOVEL final states of the batch to the visited set in canonical
@@ -344,6 +376,24 @@ int dispatch_nx(int nx, F&& f) {
 }
 
 }  // namespace
+
+static round_args make_round_args(dp_ctx* c) {
+  round_args a{};
+  a.N = c->N;
+  a.app = c->d_app;
+  a.total = c->d_total;
+  a.verdict = c->d_verdict;
+  a.flags = c->d_flags;
+  a.conf_off = c->d_conf_off;
+  a.conf = c->d_conf;
+  a.res = c->d_res;
+  a.cursor = c->d_node_cursor;
+  a.count = c->d_node_count;
+  a.node_novel = c->d_node_novel;
+  a.limit = c->round_limit;
+  a.counters = c->d_counters;
+  return a;
+}
 
 int dp_launch_visited_rebuild(dp_ctx* c, double radius, hipStream_t s) {
   c->grid = dp_make_grid(c, radius);
@@ -416,27 +466,16 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
   k_round_init<<<dp_grid_blocks(N, 256, 1024), 256, 0, s>>>(N, c->d_app, c->d_total, c->d_node_cursor,
                                                             c->d_node_count, c->d_node_novel, c->d_counters);
   c->round_limit = p->limit_branching_factor;
-  rc = dp_launch_more_rounds(c, c->rounds, s);
-  if (rc) return rc;
+  {
+    round_args ra = make_round_args(c);
+    k_resolve_ticketed<<<N, 64, 0, s>>>(ra);
+  }
   DP_HIP_RET(hipGetLastError());
   return DP_OK;
 }
 
 int dp_launch_more_rounds(dp_ctx* c, int rounds, hipStream_t s) {
-  round_args a{};
-  a.N = c->N;
-  a.app = c->d_app;
-  a.total = c->d_total;
-  a.verdict = c->d_verdict;
-  a.flags = c->d_flags;
-  a.conf_off = c->d_conf_off;
-  a.conf = c->d_conf;
-  a.res = c->d_res;
-  a.cursor = c->d_node_cursor;
-  a.count = c->d_node_count;
-  a.node_novel = c->d_node_novel;
-  a.limit = c->round_limit;
-  a.counters = c->d_counters;
+  round_args a = make_round_args(c);
   for (int r = 0; r < rounds; ++r) k_round<<<c->N, 64, 0, s>>>(a);
   DP_HIP_RET(hipGetLastError());
   return DP_OK;

@@ -1,0 +1,618 @@
+// k_quad.hip -- quadrotor_v0 expansion: rollout of the primitive's actions
+// from the translated start state (transform_primitive with the position /
+// velocity offset, dbastar.hpp:166-215), bounds on every state, sphere robot
+// against boxes and spheres with the first colliding state
+// (check_lazy_trajectory, dbastar.cpp:391-490), then check_goal, g/h and the
+// duplicate test on the FP64 states.
+//
+// Two kernels:
+//   k_quad32   one wave per (node, chunk of 64 pairs): FP32 rollout relative
+//              to the node, obstacles of the node's reach box staged in LDS
+//              as FP32 offsets from the node, per-state clearance.  A state
+//              whose clearance (or bound margin) is within near_eps is
+//              "near"; a verdict that a near state could change is deferred
+//              to the FP64 pass.  Quaternions are renormalised in FP64.
+//   k_quad64   one lane per deferred pair and per FREE pair: the FP64 rollout
+//              in the same operation order as the CPU oracle (bit-identical
+//              states), full FP64 re-check for deferred pairs, then goal
+//              check, g, h and the visited-set duplicate test.
+// With fp32_rollout = 0, k_quad64 evaluates every pair (exact FP64 mode).
+#include <hip/hip_runtime.h>
+#include <limits.h>
+
+#include "dp_internal.h"
+
+// flag bits of d_flags (bit0 dup-vs-visited, bit1 goal-forced are set later)
+#define QF_NEAR 4
+#define QF_NEED64 8
+
+struct dp_quad_args {
+  const dp_dev_sys* sys;
+  const double* nodes;
+  int32_t Ncap, N;
+  const double* gpar;
+  const int32_t* off;
+  const double* st;       // [13][S]
+  int32_t S;
+  const double* act;      // [4][A]
+  const float* act32;     // [4][A]
+  int32_t A;
+  int32_t P;
+  const int32_t* goalidx;
+  int32_t num_goal, check_goal;
+  const double* obs;      // [nb][6] then [ns][4]
+  int32_t nb, ns;
+  double prim_reach;      // max over primitives of max_k |p_k - p_0|_inf
+  double tmax;            // longest primitive duration
+  const double* vis;
+  int64_t vstride, nv;
+  const int32_t* vstart;
+  dp_grid_geom grid;
+  const int32_t* app;     // pair offsets per node [N+1]
+  const int32_t* chunk_off;  // [N+1] wave chunks per node, scanned
+  const int32_t* pnode;
+  const int32_t* pprim;
+  const int32_t* total;
+  int8_t* verdict;
+  int32_t* firstk;
+  int32_t* goal_out;
+  int32_t* res;
+  int8_t* flags;
+  double* fin;
+  int64_t pcap;
+  double* gout;
+  double* hout;
+  float* dbg32;           // [3][pcap] FP32 final position (absolute) of FREE pairs, or null
+  double goal[DP_MAX_NX];
+  double goal_bound, r_dup, cdf;
+  int32_t exact;
+  long long* counters;
+};
+
+namespace {
+
+constexpr int kQWaves = 4;
+
+// ---- FP64 step: the oracle's operation order --------------------------------
+__device__ __forceinline__ void quad_cos_sinc(const dp_dev_sys& S, double z, double* c, double* sinc) {
+  double pc = S.cos_c[DP_TRIG_TERMS - 1];
+  double ps = S.sinc_c[DP_TRIG_TERMS - 1];
+#pragma unroll
+  for (int n = DP_TRIG_TERMS - 2; n >= 0; --n) {
+    pc = pc * z + S.cos_c[n];
+    ps = ps * z + S.sinc_c[n];
+  }
+  *c = pc;
+  *sinc = ps;
+}
+
+__device__ __forceinline__ void quad_step64(const dp_dev_sys& S, const double* s, const double* u, double* o) {
+  const double m = S.quad[0], g = S.quad[1], arm = S.quad[2], t2t = S.quad[3];
+  const double Jx = S.quad[4], Jy = S.quad[5], Jz = S.quad[6];
+  const double dt = S.dt;
+  const double qx = s[3], qy = s[4], qz = s[5], qw = s[6];
+  const double vx = s[7], vy = s[8], vz = s[9];
+  const double wx = s[10], wy = s[11], wz = s[12];
+  double g4 = g * 0.25;
+  double usum = ((u[0] + u[1]) + u[2]) + u[3];
+  double f = g4 * usum;
+  double F = m * g4;
+  double tx = (arm * F) * (((u[2] + u[3]) - u[0]) - u[1]);
+  double ty = (arm * F) * (((u[1] + u[2]) - u[0]) - u[3]);
+  double tz = (t2t * F) * (((u[1] + u[3]) - u[0]) - u[2]);
+  double rzx = 2.0 * (qx * qz + qw * qy);
+  double rzy = 2.0 * (qy * qz - qw * qx);
+  double rzz = 1.0 - 2.0 * (qx * qx + qy * qy);
+  double ax = f * rzx;
+  double ay = f * rzy;
+  double az = f * rzz - g;
+  double jwx = Jx * wx, jwy = Jy * wy, jwz = Jz * wz;
+  double cx = wy * jwz - wz * jwy;
+  double cy = wz * jwx - wx * jwz;
+  double cz = wx * jwy - wy * jwx;
+  double dwx = (tx - cx) / Jx;
+  double dwy = (ty - cy) / Jy;
+  double dwz = (tz - cz) / Jz;
+  o[0] = s[0] + dt * vx;
+  o[1] = s[1] + dt * vy;
+  o[2] = s[2] + dt * vz;
+  o[7] = vx + dt * ax;
+  o[8] = vy + dt * ay;
+  o[9] = vz + dt * az;
+  o[10] = wx + dt * dwx;
+  o[11] = wy + dt * dwy;
+  o[12] = wz + dt * dwz;
+  double h = 0.5 * dt;
+  double z = (h * h) * ((wx * wx + wy * wy) + wz * wz);
+  double ec, esinc;
+  quad_cos_sinc(S, z, &ec, &esinc);
+  double k = h * esinc;
+  double ex = k * wx, ey = k * wy, ez = k * wz, ew = ec;
+  double nw = ((qw * ew - qx * ex) - qy * ey) - qz * ez;
+  double nx = ((qw * ex + qx * ew) + qy * ez) - qz * ey;
+  double ny = ((qw * ey - qx * ez) + qy * ew) + qz * ex;
+  double nz = ((qw * ez + qx * ey) - qy * ex) + qz * ew;
+  double n2 = ((nx * nx + ny * ny) + nz * nz) + nw * nw;
+  double inv = 1.0 / sqrt(n2);
+  o[3] = nx * inv;
+  o[4] = ny * inv;
+  o[5] = nz * inv;
+  o[6] = nw * inv;
+}
+
+__device__ __forceinline__ bool state_valid64(const dp_dev_sys& S, const double* s) {
+  bool ok = true;
+#pragma unroll
+  for (int c = 0; c < 13; ++c) ok = ok && (s[c] >= S.lo[c] && s[c] <= S.hi[c]);
+  return ok;
+}
+
+__device__ __forceinline__ bool collides64(const dp_dev_sys& S, const double* obs, int nb, int ns, const double* s) {
+  const double r = S.radius;
+  for (int o = 0; o < nb; ++o)
+    if (dp_sphere_box(s[0], s[1], s[2], r, obs + 6 * o)) return true;
+  const double* sp = obs + 6 * nb;
+  for (int o = 0; o < ns; ++o)
+    if (dp_sphere_sphere(s[0], s[1], s[2], r, sp + 4 * o)) return true;
+  return false;
+}
+
+// ---- FP32 pass -------------------------------------------------------------------
+struct quad_f32 {
+  float g4, g, armF, t2tF, Jx, Jy, Jz, dt, h;
+  float cosc[6], sincc[6];
+};
+
+__device__ __forceinline__ void quad_step32(const quad_f32& K, float* s, const float* u) {
+  const float qx = s[3], qy = s[4], qz = s[5], qw = s[6];
+  const float vx = s[7], vy = s[8], vz = s[9];
+  const float wx = s[10], wy = s[11], wz = s[12];
+  const float f = K.g4 * (((u[0] + u[1]) + u[2]) + u[3]);
+  const float tx = K.armF * (((u[2] + u[3]) - u[0]) - u[1]);
+  const float ty = K.armF * (((u[1] + u[2]) - u[0]) - u[3]);
+  const float tz = K.t2tF * (((u[1] + u[3]) - u[0]) - u[2]);
+  const float rzx = 2.0f * (qx * qz + qw * qy);
+  const float rzy = 2.0f * (qy * qz - qw * qx);
+  const float rzz = 1.0f - 2.0f * (qx * qx + qy * qy);
+  const float jwx = K.Jx * wx, jwy = K.Jy * wy, jwz = K.Jz * wz;
+  const float dwx = (tx - (wy * jwz - wz * jwy)) / K.Jx;
+  const float dwy = (ty - (wz * jwx - wx * jwz)) / K.Jy;
+  const float dwz = (tz - (wx * jwy - wy * jwx)) / K.Jz;
+  s[0] = s[0] + K.dt * vx;
+  s[1] = s[1] + K.dt * vy;
+  s[2] = s[2] + K.dt * vz;
+  s[7] = vx + K.dt * (f * rzx);
+  s[8] = vy + K.dt * (f * rzy);
+  s[9] = vz + K.dt * (f * rzz - K.g);
+  s[10] = wx + K.dt * dwx;
+  s[11] = wy + K.dt * dwy;
+  s[12] = wz + K.dt * dwz;
+  const float z = (K.h * K.h) * ((wx * wx + wy * wy) + wz * wz);
+  float pc = K.cosc[5], ps = K.sincc[5];
+#pragma unroll
+  for (int n = 4; n >= 0; --n) {
+    pc = pc * z + K.cosc[n];
+    ps = ps * z + K.sincc[n];
+  }
+  const float kk = K.h * ps;
+  const float ex = kk * wx, ey = kk * wy, ez = kk * wz, ew = pc;
+  const double nw = ((qw * ew - qx * ex) - qy * ey) - qz * ez;
+  const double nx = ((qw * ex + qx * ew) + qy * ez) - qz * ey;
+  const double ny = ((qw * ey - qx * ez) + qy * ew) + qz * ex;
+  const double nz = ((qw * ez + qx * ey) - qy * ex) + qz * ew;
+  // quaternion renormalised in FP64
+  const double inv = 1.0 / sqrt(((nx * nx + ny * ny) + nz * nz) + nw * nw);
+  s[3] = (float)(nx * inv);
+  s[4] = (float)(ny * inv);
+  s[5] = (float)(nz * inv);
+  s[6] = (float)(nw * inv);
+}
+
+__global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
+  extern __shared__ float qsm[];
+  const dp_dev_sys& S = *a.sys;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int per_wave = 6 * a.nb + 4 * a.ns;
+  float* cb = qsm + wave * per_wave;  // compacted boxes (6 floats) then spheres (4 floats)
+  quad_f32 K;
+  K.g = (float)S.quad[1];
+  K.g4 = (float)(S.quad[1] * 0.25);
+  K.armF = (float)(S.quad[2] * (S.quad[0] * (S.quad[1] * 0.25)));
+  K.t2tF = (float)(S.quad[3] * (S.quad[0] * (S.quad[1] * 0.25)));
+  K.Jx = (float)S.quad[4];
+  K.Jy = (float)S.quad[5];
+  K.Jz = (float)S.quad[6];
+  K.dt = (float)S.dt;
+  K.h = (float)(0.5 * S.dt);
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    K.cosc[n] = (float)S.cos_c[n];
+    K.sincc[n] = (float)S.sinc_c[n];
+  }
+  const float r = (float)S.radius;
+  const float eps = S.near_eps;
+  const float r_lo2 = (r - eps) * (r - eps), r_hi2 = (r + eps) * (r + eps), r2 = r * r;
+  const int32_t total_chunks = a.chunk_off[a.N];
+  long long c_b = 0, c_c = 0, c_k = 0;
+
+  for (int32_t w = blockIdx.x * kQWaves + wave; w < total_chunks; w += gridDim.x * kQWaves) {
+    // node of chunk w: last i with chunk_off[i] <= w
+    int32_t lo = 0, hi = a.N;
+    while (hi - lo > 1) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (a.chunk_off[mid] <= w) lo = mid;
+      else hi = mid;
+    }
+    const int32_t node = lo;
+    const int32_t tot = *a.total;
+    const int32_t pbeg = a.app[node] < tot ? a.app[node] : tot;
+    const int32_t pend = a.app[node + 1] < tot ? a.app[node + 1] : tot;
+    const int32_t q = pbeg + (w - a.chunk_off[node]) * 64 + lane;
+    const bool vp = q < pend;
+    double np[3], nvel[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      np[c] = a.nodes[(int64_t)c * a.Ncap + node];
+      nvel[c] = a.nodes[(int64_t)(7 + c) * a.Ncap + node];
+    }
+    // reach box of the node's rollouts (positions relative to the node)
+    const double vmax = fmax(fabs(nvel[0]), fmax(fabs(nvel[1]), fabs(nvel[2])));
+    const double reach = a.prim_reach + a.tmax * vmax + 0.5;
+    const float freach = (float)reach;
+    // stage candidate obstacles relative to the node, ballot-compacted
+    int nbc = 0, nsc = 0;
+    for (int o0 = 0; o0 < a.nb; o0 += 64) {
+      const int o = o0 + lane;
+      bool keep = false;
+      float rel[6];
+      if (o < a.nb) {
+        const double* b = a.obs + 6 * o;
+        keep = true;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double d = b[c] - np[c];
+          keep = keep && fabs(d) <= reach + b[3 + c] + S.radius;
+          rel[c] = (float)d;
+          rel[3 + c] = (float)b[3 + c];
+        }
+      }
+      const uint64_t m = __ballot(keep);
+      if (keep) {
+        float* dst = cb + 6 * (nbc + __popcll(m & ((1ull << lane) - 1ull)));
+#pragma unroll
+        for (int c = 0; c < 6; ++c) dst[c] = rel[c];
+      }
+      nbc += __popcll(m);
+    }
+    float* cs = cb + 6 * a.nb;
+    for (int o0 = 0; o0 < a.ns; o0 += 64) {
+      const int o = o0 + lane;
+      bool keep = false;
+      float rel[4];
+      if (o < a.ns) {
+        const double* b = a.obs + 6 * a.nb + 4 * o;
+        keep = true;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double d = b[c] - np[c];
+          keep = keep && fabs(d) <= reach + b[3] + S.radius;
+          rel[c] = (float)d;
+        }
+        rel[3] = (float)b[3];
+      }
+      const uint64_t m = __ballot(keep);
+      if (keep) {
+        float* dst = cs + 4 * (nsc + __popcll(m & ((1ull << lane) - 1ull)));
+#pragma unroll
+        for (int c = 0; c < 4; ++c) dst[c] = rel[c];
+      }
+      nsc += __popcll(m);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    if (vp) {
+      const int32_t j = a.pprim[q];
+      const int32_t s0 = a.off[j];
+      const int32_t L1 = a.off[j + 1] - s0;
+      const int32_t a0 = s0 - j;
+      float s[13];
+      // start: relative position = stored p0; velocity = stored v0 + node velocity
+#pragma unroll
+      for (int c = 0; c < 13; ++c) {
+        const double v = a.st[(int64_t)c * a.S + s0];
+        s[c] = (c >= 7 && c <= 9) ? (float)(v + nvel[c - 7]) : (float)v;
+      }
+      float plo[3], phi[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        plo[c] = (float)(S.lo[c] - np[c]);
+        phi[c] = (float)(S.hi[c] - np[c]);
+      }
+      int first_bad = INT_MAX, first_col = INT_MAX;
+      bool near_bound_any = false, near_col_before = false, near = false;
+      for (int k = 0; k < L1; ++k) {
+        if (k > 0) {
+          float u[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) u[c] = a.act32[(int64_t)c * a.A + a0 + k - 1];
+          quad_step32(K, s, u);
+        }
+        // bounds with margin
+        bool bad = false, nb_ = false;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          bad = bad || !(s[c] >= plo[c] && s[c] <= phi[c]);
+          nb_ = nb_ || fabsf(s[c] - plo[c]) <= eps || fabsf(s[c] - phi[c]) <= eps;
+          nb_ = nb_ || fabsf(s[c]) > freach;  // left the staged reach box
+        }
+#pragma unroll
+        for (int c = 7; c < 13; ++c) {
+          const float lo_ = (float)S.lo[c], hi_ = (float)S.hi[c];
+          bad = bad || !(s[c] >= lo_ && s[c] <= hi_);
+          nb_ = nb_ || fabsf(s[c] - lo_) <= eps || fabsf(s[c] - hi_) <= eps;
+        }
+        near_bound_any = near_bound_any || nb_;
+        if (bad) {
+          first_bad = k;
+          break;  // later states cannot change a bounds verdict
+        }
+        if (first_col == INT_MAX) {
+          bool col = false, nc = false;
+          for (int t = 0; t < nbc; ++t) {
+            const float* b = cb + 6 * t;
+            float dx = fabsf(s[0] - b[0]) - b[3];
+            float dy = fabsf(s[1] - b[1]) - b[4];
+            float dz = fabsf(s[2] - b[2]) - b[5];
+            dx = dx > 0.f ? dx : 0.f;
+            dy = dy > 0.f ? dy : 0.f;
+            dz = dz > 0.f ? dz : 0.f;
+            const float d2 = (dx * dx + dy * dy) + dz * dz;
+            col = col || d2 <= r2;
+            nc = nc || (d2 >= r_lo2 && d2 <= r_hi2);
+          }
+          for (int t = 0; t < nsc; ++t) {
+            const float* b = cs + 4 * t;
+            const float dx = s[0] - b[0], dy = s[1] - b[1], dz = s[2] - b[2];
+            const float d2 = (dx * dx + dy * dy) + dz * dz;
+            const float rr = r + b[3];
+            col = col || d2 <= rr * rr;
+            nc = nc || (d2 >= (rr - eps) * (rr - eps) && d2 <= (rr + eps) * (rr + eps));
+          }
+          if (nc) near_col_before = true;
+          if (col) first_col = k;
+        }
+      }
+      int8_t vd;
+      int32_t fk;
+      if (first_bad != INT_MAX) {
+        vd = DP_V_BOUNDS;
+        fk = first_bad;
+        near = near_bound_any;
+      } else if (first_col != INT_MAX) {
+        vd = DP_V_COLLIDE;
+        fk = first_col;
+        near = near_bound_any || near_col_before;
+      } else {
+        vd = DP_V_FREE;
+        fk = -1;
+        near = near_bound_any || near_col_before;
+      }
+      if (a.dbg32) {
+        const bool f = vd == DP_V_FREE;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a.dbg32[c * a.pcap + q] = f ? (float)(np[c] + (double)s[c]) : __builtin_nanf("");
+      }
+      if (near) {
+        a.flags[q] = QF_NEAR;
+      } else if (vd == DP_V_FREE) {
+        a.flags[q] = QF_NEED64;
+      } else {
+        a.flags[q] = 0;
+        a.verdict[q] = vd;
+        a.firstk[q] = fk;
+        a.goal_out[q] = -1;
+        a.res[q] = DP_R_NONE;
+        c_b += vd == DP_V_BOUNDS;
+        c_c += vd == DP_V_COLLIDE;
+        c_k += vd == DP_V_COLLIDE;
+      }
+    }
+    // the staging area is rewritten by the next chunk
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if (c_b) atomicAdd((unsigned long long*)&a.counters[DPC_BOUNDS], (unsigned long long)c_b);
+  if (c_c) atomicAdd((unsigned long long*)&a.counters[DPC_COLLIDE], (unsigned long long)c_c);
+  if (c_k) atomicAdd((unsigned long long*)&a.counters[DPC_CHECKED], (unsigned long long)c_k);
+}
+
+// ---- FP64 pass -------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
+  const dp_dev_sys& S = *a.sys;
+  const int32_t tot = *a.total;
+  long long c_b = 0, c_c = 0, c_f = 0, c_g = 0, c_k = 0, c_r = 0;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+    int8_t fl = a.exact ? (int8_t)QF_NEAR : a.flags[q];
+    if (!(fl & (QF_NEAR | QF_NEED64))) continue;
+    const bool full = (fl & QF_NEAR) != 0;
+    if (full && !a.exact) ++c_r;
+    const int32_t i = a.pnode[q], j = a.pprim[q];
+    const int32_t s0 = a.off[j];
+    const int32_t L1 = a.off[j + 1] - s0;
+    const int32_t a0 = s0 - j;
+    double node[13];
+#pragma unroll
+    for (int c = 0; c < 13; ++c) node[c] = a.nodes[(int64_t)c * a.Ncap + i];
+    double s[13], t0[13], x[13];
+#pragma unroll
+    for (int c = 0; c < 13; ++c) {
+      const double v = a.st[(int64_t)c * a.S + s0];
+      s[c] = dp_is_offset_dim(S, c) ? v + node[c] : v;
+      t0[c] = s[c];
+    }
+    int first_bad = INT_MAX, first_col = INT_MAX;
+    int chosen = -1, nn = 0;
+    const int nng = a.check_goal ? a.num_goal : 0;
+    for (int k = 0; k < L1; ++k) {
+      if (k > 0) {
+        double u[4], o[13];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) u[c] = a.act[(int64_t)c * a.A + a0 + k - 1];
+        quad_step64(S, s, u, o);
+#pragma unroll
+        for (int c = 0; c < 13; ++c) s[c] = o[c];
+      }
+      if (full) {
+        if (first_bad == INT_MAX && !state_valid64(S, s)) first_bad = k;
+        if (first_bad != INT_MAX) break;
+        if (first_col == INT_MAX && collides64(S, a.obs, a.nb, a.ns, s)) first_col = k;
+      }
+      // check_goal at the float-rule indices, in nn order (dbastar.cpp:492-512)
+      while (chosen == -1 && nn < nng && a.goalidx[(int64_t)nn * a.P + j] == k) {
+        if (dp_distance(S, s, a.goal) <= a.goal_bound) {
+          chosen = k;
+#pragma unroll
+          for (int c = 0; c < 13; ++c) x[c] = s[c];
+        }
+        ++nn;
+      }
+    }
+    int8_t vd = DP_V_FREE;
+    int32_t fk = -1;
+    if (first_bad != INT_MAX) {
+      vd = DP_V_BOUNDS;
+      fk = first_bad;
+    } else if (first_col != INT_MAX) {
+      vd = DP_V_COLLIDE;
+      fk = first_col;
+    }
+    a.verdict[q] = vd;
+    a.firstk[q] = fk;
+    c_b += vd == DP_V_BOUNDS;
+    c_c += vd == DP_V_COLLIDE;
+    c_k += vd != DP_V_BOUNDS;
+    if (vd != DP_V_FREE) {
+      a.goal_out[q] = -1;
+      a.res[q] = DP_R_NONE;
+      a.flags[q] = 0;
+      continue;
+    }
+    if (chosen == -1) {
+#pragma unroll
+      for (int c = 0; c < 13; ++c) x[c] = s[c];
+    }
+    const double hq = dp_lower_bound_time(S, x, a.goal);
+    const double cm = chosen != -1 ? chosen * S.dt : (L1 - 1) * S.dt;
+    const double gq = a.gpar[i] + cm + a.cdf * dp_lower_bound_time(S, node, t0);
+    bool dupv = false;
+    if (a.nv > 0) {
+      int32_t cc[3];
+      dp_cell_coords(a.grid, x, cc);
+      for (int cq = 0; cq < 27 && !dupv; ++cq) {
+        const int64_t cell = dp_cell_neighbor(a.grid, cc, cq);
+        if (cell < 0) continue;
+        const int32_t e = a.vstart[cell + 1];
+        for (int32_t p = a.vstart[cell]; p < e && !dupv; ++p) {
+          double y[13];
+#pragma unroll
+          for (int c = 0; c < 13; ++c) y[c] = a.vis[(int64_t)c * a.vstride + p];
+          dupv = dp_distance(S, x, y) <= a.r_dup;
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 13; ++c) a.fin[(int64_t)c * a.pcap + q] = x[c];
+    a.gout[q] = gq;
+    a.hout[q] = hq;
+    a.goal_out[q] = chosen;
+    a.res[q] = DP_R_UNDECIDED;
+    a.flags[q] = (int8_t)((dupv ? 1 : 0) | (chosen != -1 ? 2 : 0));
+    ++c_f;
+    c_g += chosen != -1;
+  }
+  if (c_b) atomicAdd((unsigned long long*)&a.counters[DPC_BOUNDS], (unsigned long long)c_b);
+  if (c_c) atomicAdd((unsigned long long*)&a.counters[DPC_COLLIDE], (unsigned long long)c_c);
+  if (c_f) atomicAdd((unsigned long long*)&a.counters[DPC_FREE], (unsigned long long)c_f);
+  if (c_g) atomicAdd((unsigned long long*)&a.counters[DPC_GOAL], (unsigned long long)c_g);
+  if (c_k) atomicAdd((unsigned long long*)&a.counters[DPC_CHECKED], (unsigned long long)c_k);
+  if (c_r) atomicAdd((unsigned long long*)&a.counters[DPC_RECHECK], (unsigned long long)c_r);
+}
+
+__global__ void k_quad_chunks(int32_t N, const int32_t* __restrict__ app, const int32_t* __restrict__ total,
+                              int32_t* __restrict__ chunks) {
+  const int32_t tot = *total;
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    const int32_t b = app[i] < tot ? app[i] : tot;
+    const int32_t e = app[i + 1] < tot ? app[i + 1] : tot;
+    chunks[i] = (e - b + 63) / 64;
+  }
+}
+
+}  // namespace
+
+int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, double r_dup, hipStream_t s) {
+  dp_quad_args a{};
+  a.sys = c->d_sys;
+  a.nodes = c->d_nodes;
+  a.Ncap = c->N_cap;
+  a.N = c->N;
+  a.gpar = c->d_gpar;
+  a.off = c->d_off;
+  a.st = c->d_st;
+  a.S = c->S;
+  a.act = c->d_act;
+  a.act32 = c->d_act32;
+  a.A = c->S - c->P;
+  a.P = c->P;
+  a.goalidx = c->d_goalidx;
+  a.num_goal = p->num_check_goal;
+  a.check_goal = p->check_intermediate_goal && p->num_check_goal > 0;
+  a.obs = c->d_obs;
+  a.nb = c->nb;
+  a.ns = c->ns;
+  a.prim_reach = c->prim_reach;
+  a.tmax = (c->maxlen - 1) * c->desc.dt;
+  a.vis = c->d_vis_sorted;
+  a.vstride = c->nv_cap;
+  a.nv = c->nv;
+  a.vstart = c->d_vis_start;
+  a.grid = c->grid;
+  a.app = c->d_app;
+  a.chunk_off = c->d_chunk;
+  a.pnode = c->d_pnode;
+  a.pprim = c->d_pprim;
+  a.total = c->d_total;
+  a.verdict = c->d_verdict;
+  a.firstk = c->d_firstk;
+  a.goal_out = c->d_goal;
+  a.res = c->d_res;
+  a.flags = c->d_flags;
+  a.fin = c->d_final;
+  a.pcap = c->pair_cap;
+  a.gout = c->d_g;
+  a.hout = c->d_h;
+  a.dbg32 = c->d_dbg32;
+  for (int k = 0; k < DP_MAX_NX; ++k) a.goal[k] = k < 13 ? p->goal[k] : 0.0;
+  a.goal_bound = goal_bound;
+  a.r_dup = r_dup;
+  a.cdf = p->cost_delta_factor;
+  a.exact = p->fp32_rollout ? 0 : 1;
+  a.counters = c->d_counters;
+  if (p->fp32_rollout) {
+    const size_t lds = sizeof(float) * kQWaves * (6 * (size_t)c->nb + 4 * (size_t)c->ns + 1);
+    if (lds > 64 * 1024) {
+      dp_set_error("too many obstacles for the FP32 quadrotor staging (max 64 KiB LDS per workgroup)");
+      return DP_ERR_ARG;
+    }
+    k_quad_chunks<<<dp_grid_blocks(c->N, 256, 1024), 256, 0, s>>>(c->N, c->d_app, c->d_total, c->d_chunk);
+    dp_scan_small(c->d_chunk, c->N, nullptr, INT64_MAX, c->d_counters, -1, s);
+    const int64_t max_chunks = (int64_t)c->N * ((c->P + 63) / 64);
+    k_quad32<<<dp_grid_blocks(max_chunks, kQWaves, 4096), 256, lds, s>>>(a);
+  }
+  k_quad64<<<dp_grid_blocks(c->pair_cap, 256, 8192), 256, 0, s>>>(a);
+  DP_HIP_RET(hipGetLastError());
+  return DP_OK;
+}

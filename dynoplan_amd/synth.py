@@ -306,3 +306,114 @@ def car_trailer(seed=4, **kw):
     args.update(kw)
     return _planar_workload("car+1 trailer N=4096 P=5000 O=300", N.SYS_CAR_TRAILER, seed,
                             prim_fn=car_trailer_primitives, **args)
+
+
+# ---- quadrotor ------------------------------------------------------------------
+def _quat_mul(a, b):
+    x1, y1, z1, w1 = a[..., 0], a[..., 1], a[..., 2], a[..., 3]
+    x2, y2, z2, w2 = b[..., 0], b[..., 1], b[..., 2], b[..., 3]
+    return np.stack([w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                     w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2,
+                     w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2,
+                     w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2], axis=-1)
+
+
+def random_attitudes(rng, n, tilt_max=0.25):
+    """Uniform yaw times a tilt of at most tilt_max about a random horizontal axis."""
+    yaw = rng.uniform(-math.pi, math.pi, n)
+    ax = rng.normal(size=(n, 2))
+    ax /= np.linalg.norm(ax, axis=1, keepdims=True)
+    tilt = tilt_max * np.sqrt(rng.uniform(0, 1, n))
+    qy = np.stack([np.zeros(n), np.zeros(n), np.sin(yaw / 2), np.cos(yaw / 2)], 1)
+    qt = np.stack([ax[:, 0] * np.sin(tilt / 2), ax[:, 1] * np.sin(tilt / 2), np.zeros(n), np.cos(tilt / 2)], 1)
+    return _quat_mul(qy, qt)
+
+
+def quad_step_np(desc, s, u):
+    """Vectorised FP64 step of the quadrotor model (same equations as the
+    library's rollout; used to generate primitive data only)."""
+    m, g, arm, t2t, Jx, Jy, Jz = (desc.quad[k] for k in range(7))
+    dt = desc.dt
+    q = s[:, 3:7]
+    v = s[:, 7:10]
+    w = s[:, 10:13]
+    f = (g / 4) * u.sum(1)
+    F = m * g / 4
+    tau = np.stack([arm * F * (u[:, 2] + u[:, 3] - u[:, 0] - u[:, 1]),
+                    arm * F * (u[:, 1] + u[:, 2] - u[:, 0] - u[:, 3]),
+                    t2t * F * (u[:, 1] + u[:, 3] - u[:, 0] - u[:, 2])], 1)
+    qx, qy, qz, qw = q.T
+    rz = np.stack([2 * (qx * qz + qw * qy), 2 * (qy * qz - qw * qx), 1 - 2 * (qx * qx + qy * qy)], 1)
+    acc = f[:, None] * rz - np.array([0, 0, g])
+    J = np.array([Jx, Jy, Jz])
+    dw = (tau - np.cross(w, J * w)) / J
+    out = np.empty_like(s)
+    out[:, 0:3] = s[:, 0:3] + dt * v
+    out[:, 7:10] = v + dt * acc
+    out[:, 10:13] = w + dt * dw
+    th = np.linalg.norm(w, axis=1) * dt / 2
+    sinc = np.where(th > 1e-12, np.sin(th) / np.where(th > 0, th, 1), 1.0)
+    e = np.concatenate([(dt / 2) * sinc[:, None] * w, np.cos(th)[:, None]], 1)
+    nq = _quat_mul(q, e)
+    out[:, 3:7] = nq / np.linalg.norm(nq, axis=1, keepdims=True)
+    return out
+
+
+def quadrotor_primitives(rng, P, lmin, lmax, desc=None, tilt_max=0.25, w_sd=0.7, base=(0.9, 1.1), diff=0.02):
+    desc = desc or default_desc_static(N.SYS_QUADROTOR)
+    L = rng.integers(lmin, lmax + 1, size=P)
+    Lmax = int(L.max())
+    u = np.empty((P, Lmax, 4))
+    pieces = rng.integers(1, 4, size=P)
+    for j in range(P):
+        cuts = np.sort(rng.integers(1, Lmax, size=pieces[j] - 1))
+        bnd = np.concatenate([[0], cuts, [Lmax]])
+        for a, b in zip(bnd[:-1], bnd[1:]):
+            u[j, a:b] = np.clip(rng.uniform(*base) + rng.uniform(-diff, diff, 4), 0.0, 1.4)
+    x = np.zeros((P, Lmax + 1, 13))
+    x[:, 0, 3:7] = random_attitudes(rng, P, tilt_max)
+    x[:, 0, 10:13] = rng.normal(0, w_sd, (P, 3))
+    for k in range(Lmax):
+        x[:, k + 1] = quad_step_np(desc, x[:, k], u[:, k])
+    return _pack([x[j, : L[j] + 1] for j in range(P)], [u[j, : L[j]] for j in range(P)])
+
+
+def sphere_collides_np(desc, boxes, spheres, p):
+    r = desc.body_half[0]
+    c = np.clip(p[:, None, :], boxes[None, :, :3] - boxes[None, :, 3:], boxes[None, :, :3] + boxes[None, :, 3:])
+    hit = (((c - p[:, None, :]) ** 2).sum(2) <= r * r).any(1)
+    if spheres.shape[0]:
+        d2 = ((spheres[None, :, :3] - p[:, None, :]) ** 2).sum(2)
+        hit |= (d2 <= (r + spheres[None, :, 3]) ** 2).any(1)
+    return hit
+
+
+def quadrotor(seed=3, P=10000, lmin=10, lmax=50, n_boxes=100, n_spheres=50, N_nodes=2048, n_visited=20000,
+              limit=20, tilt_max=0.25, w_sd=0.7):
+    rng = np.random.default_rng(seed)
+    desc = default_desc_static(N.SYS_QUADROTOR)
+    offsets, states, actions = quadrotor_primitives(rng, P, lmin, lmax, desc, tilt_max, w_sd)
+    lo = np.array([0.0, 0.0, 0.0])
+    hi = np.array([10.0, 10.0, 5.0])
+    boxes = random_boxes(rng, n_boxes, lo, hi, 0.5, 1.5, dim=3)
+    spheres = np.c_[rng.uniform(lo, hi, (n_spheres, 3)), rng.uniform(0.2, 0.8, n_spheres)]
+
+    def states_in_free_space(n):
+        out = []
+        while sum(len(o) for o in out) < n:
+            m = 4 * n
+            s = np.zeros((m, 13))
+            s[:, :3] = rng.uniform(lo + 0.5, hi - 0.5, (m, 3))
+            s[:, 3:7] = random_attitudes(rng, m, tilt_max)
+            s[:, 7:10] = rng.uniform(-0.5, 0.5, (m, 3))
+            s[:, 10:13] = rng.normal(0, w_sd, (m, 3))
+            out.append(s[~sphere_collides_np(desc, boxes, spheres, s[:, :3])])
+        return np.ascontiguousarray(np.concatenate(out)[:n])
+
+    nodes = states_in_free_space(N_nodes)
+    visited = states_in_free_space(n_visited)
+    goal = states_in_free_space(1)[0]
+    params = make_params(goal=goal, limit_branching_factor=limit, fp32_rollout=True)
+    return Workload(name=f"quadrotor_v0 N={N_nodes} P={P} O={n_boxes}+{n_spheres} V={n_visited}", desc=desc,
+                    offsets=offsets, states=states, actions=actions, lo=lo, hi=hi, boxes=boxes, spheres=spheres,
+                    nodes=nodes, g_parent=rng.uniform(0, 10, N_nodes), visited=visited, params=params, seed=seed)

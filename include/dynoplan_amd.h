@@ -218,6 +218,12 @@ void* dp_stream(dp_ctx* ctx);
  * 2 duplicate resolution, 3 total. */
 int dp_last_timing(dp_ctx* ctx, float* ms4);
 
+/* Quadrotor FP32 rollout error report: the FP32 final position (absolute,
+ * [3][n] row per axis) of every pair the FP32 pass found FREE in the last
+ * dp_expand (other entries unspecified).  No counterpart in the reference;
+ * used to report the FP32-vs-FP64 rollout error per system. */
+int dp_debug_fp32_positions(dp_ctx* ctx, float* out, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif
