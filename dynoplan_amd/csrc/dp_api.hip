@@ -247,7 +247,7 @@ void dp_destroy(dp_ctx* c) {
   dev_free(&c->d_cont_cursor);
   dev_free(&c->d_cont_idx);
   dev_free(&c->d_cont_x);
-  dev_free(&c->d_conf_off);
+  dev_free(&c->d_conf_cnt);
   dev_free(&c->d_conf);
   dev_free(&c->d_tiles);
   dev_free(&c->d_node_cursor);
@@ -504,7 +504,8 @@ int dp_reserve(dp_ctx* c, int32_t max_nodes, int64_t max_pairs) {
     if ((rc = dev_alloc(&c->d_h, max_pairs))) return rc;
     if ((rc = dev_alloc(&c->d_cont_idx, max_pairs))) return rc;
     if ((rc = dev_alloc(&c->d_cont_x, (int64_t)nx * max_pairs))) return rc;
-    if ((rc = dev_alloc(&c->d_conf_off, max_pairs + 1))) return rc;
+    if ((rc = dev_alloc(&c->d_conf_cnt, max_pairs))) return rc;
+    if ((rc = dev_alloc(&c->d_conf, (int64_t)DP_CONF_SLOTS * max_pairs))) return rc;
     if (c->desc.system == DP_SYS_QUADROTOR && (rc = dev_alloc(&c->d_dbg32, 3 * max_pairs))) return rc;
     c->pair_cap = max_pairs;
   }

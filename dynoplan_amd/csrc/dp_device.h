@@ -107,6 +107,25 @@ __device__ __forceinline__ double dp_distance(const dp_dev_sys& S, const double*
   }
 }
 
+// dp_distance(S, a, b) <= r, decided with early exits.  The metric is a sum
+// of non-negative terms added left to right; IEEE addition is monotone in
+// each operand, so a partial sum (or the sum with a term left out) above r
+// proves the full sum is above r.  When no exit applies the full sum is
+// formed in dp_distance's exact operation order, so the decision is always
+// identical to evaluating dp_distance.
+__device__ __forceinline__ bool dp_within(const dp_dev_sys& S, const double* a, const double* b, double r) {
+  if (S.system != DP_SYS_QUADROTOR) return dp_distance(S, a, b) <= r;
+  const double A = S.w[0] * dp_l2_3(a, b);
+  if (A > r) return false;
+  const double C = S.w[2] * dp_l2_3(a + 7, b + 7);
+  const double D = S.w[3] * dp_l2_3(a + 10, b + 10);
+  if ((A + C) + D > r) return false;
+  double d = A + S.w[1] * dp_quat_angle(S, a + 3, b + 3);
+  d = d + C;
+  d = d + D;
+  return d <= r;
+}
+
 __device__ __forceinline__ double dp_max(double a, double b) { return b > a ? b : a; }
 
 // robot->lower_bound_time.

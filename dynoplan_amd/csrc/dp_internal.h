@@ -18,6 +18,8 @@
 // Bound on back-off iterations of one wave in the ticketed resolution;
 // reaching it leaves the node to the host-driven rounds instead of waiting.
 #define DP_SPIN_LIMIT (1 << 20)
+// Earlier-conflict slots stored per pair (more: the resolution scans the grid).
+#define DP_CONF_SLOTS 16
 
 void dp_set_error(const std::string& msg);
 
@@ -147,9 +149,9 @@ struct dp_ctx {
   int64_t cont_start_cap = 0, cont_cursor_cap = 0;
   int32_t* d_node_novel = nullptr;  // [N_cap+1] NOVEL pairs per node, then offsets
   double* d_cont_x = nullptr;       // [nx][pair_cap] contender final states sorted by cell
-  int32_t* d_conf_off = nullptr;    // [pair_cap+1] earlier-conflict list offsets per pair
-  int32_t* d_conf = nullptr;        // [conf_cap] earlier conflicting contenders
-  int64_t conf_cap = 0;
+  int16_t* d_conf_cnt = nullptr;    // [pair_cap] stored earlier conflicts per pair (-1: scan)
+  int32_t* d_conf = nullptr;        // [DP_CONF_SLOTS][pair_cap] earlier conflicting contenders
+  double round_r_dup = 0.0;         // duplicate radius of the batch in flight
   int32_t* d_tiles = nullptr;       // [tiles+1] partial sums of the large scan
   int64_t tiles_cap = 0;
   int32_t* d_node_cursor = nullptr; // [N_cap] first undecided pair per node
@@ -199,8 +201,8 @@ void dp_zero_counters(long long* c, hipStream_t s);
 // receives the total when slot >= 0.
 void dp_scan_small(int32_t* data, int64_t n, int32_t* total, int64_t cap, long long* counters, int slot,
                    hipStream_t s);
-// Same for large arrays whose length lives on the device (*n_dev, at most
-// n_max): three passes over tiles of 4096 elements.
+// Same for large arrays: three passes over tiles of 4096 elements.  The
+// length is *n_dev (at most n_max) when n_dev is set, else n_max.
 int dp_scan_large(dp_ctx* c, int32_t* data, const int32_t* n_dev, int64_t n_max, long long* counters, int slot,
                   hipStream_t s);
 

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(64) void k_app(const dp_dev_sys* __restrict__ Sp, c
       double b[NX];
 #pragma unroll
       for (int c = 0; c < NX; ++c) b[c] = start[(int64_t)c * P + j];
-      hit = dp_distance(S, a, b) <= r_app;
+      hit = dp_within(S, a, b, r_app);
     }
     const uint64_t m = __ballot(hit);
     if (FILL && hit) {
@@ -133,7 +133,7 @@ __device__ __forceinline__ bool grid_cell_any(const dp_dev_sys& S, const dp_grid
     double y[NX];
 #pragma unroll
     for (int k = 0; k < NX; ++k) y[k] = pts[(int64_t)k * stride + p];
-    if (dp_distance(S, x, y) <= r) return true;
+    if (dp_within(S, x, y, r)) return true;
   }
   return false;
 }
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
         const int idx = a.goalidx[(int64_t)k0 * a.P + j];
         double s[NX];
         load_translated<NX>(a, (int64_t)st0 + idx, ox, oy, s);
-        hit = dp_distance(S, s, a.goal) <= a.goal_bound;
+        hit = dp_within(S, s, a.goal, a.goal_bound);
       }
       const uint64_t m = (__ballot(hit) >> shift) & segbits;
       if (fr && m) chosen = a.goalidx[(int64_t)(__ffsll((long long)m) - 1) * a.P + j];

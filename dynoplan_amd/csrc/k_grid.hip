@@ -127,49 +127,62 @@ __global__ void k_cont_scatter(dp_grid_geom G, const int32_t* __restrict__ total
   }
 }
 
-// Earlier-conflict lists: for each pair that needs them (FREE, neither
-// goal-forced nor a duplicate of the visited set), the contenders o < q
-// within r.  FILL=false writes counts, FILL=true the lists.
-template <int NX, bool FILL>
+// Earlier contenders of pair q within r: scans the 27 cells around x.
+// Calls f(o) for every contender o < q within r; stops when f returns false.
+template <int NX, typename F>
+__device__ __forceinline__ void scan_earlier(const dp_dev_sys& S, const dp_grid_geom& G, const int32_t* cstart,
+                                             const int32_t* cidx, const double* cx, int64_t pcap, const double* x,
+                                             int64_t q, double r, F&& f) {
+  int32_t c[3];
+  dp_cell_coords(G, x, c);
+  for (int cq = 0; cq < 27; ++cq) {
+    const int64_t cell = dp_cell_neighbor(G, c, cq);
+    if (cell < 0) continue;
+    const int32_t e = cstart[cell + 1];
+    for (int32_t t = cstart[cell]; t < e; ++t) {
+      const int32_t o = cidx[t];
+      if (o >= q) continue;
+      double y[NX];
+#pragma unroll
+      for (int k = 0; k < NX; ++k) y[k] = cx[(int64_t)k * pcap + t];
+      if (!dp_within(S, x, y, r)) continue;
+      if (!f(o)) return;
+    }
+  }
+}
+
+// Earlier-conflict lists, stored inline (DP_CONF_SLOTS per pair, slot-major),
+// for the pairs a node's cursor is likely to reach: FREE pairs that are
+// neither goal-forced nor duplicates of the visited set, at position < window
+// in their node.  conf_cnt = -1 marks a pair whose list is not stored
+// (outside the window or overflowing); the resolution scans the grid for it.
+template <int NX>
 __global__ void k_conf(const dp_dev_sys* __restrict__ Sp, dp_grid_geom G, const int32_t* __restrict__ total,
+                       const int32_t* __restrict__ pnode, const int32_t* __restrict__ app,
                        const int8_t* __restrict__ verdict, const int8_t* __restrict__ flags,
                        const double* __restrict__ fin, int64_t pcap, const int32_t* __restrict__ cstart,
-                       const int32_t* __restrict__ cidx, const double* __restrict__ cx, double r,
-                       int32_t* __restrict__ conf_off, int32_t* __restrict__ conf, int64_t conf_cap,
-                       long long* __restrict__ counters) {
+                       const int32_t* __restrict__ cidx, const double* __restrict__ cx, double r, int32_t window,
+                       int16_t* __restrict__ conf_cnt, int32_t* __restrict__ conf) {
   const dp_dev_sys& S = *Sp;
   const int64_t n = *total;
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
     const int8_t v = verdict[q], f = flags[q];
     const bool checker = v == DP_V_FREE && !(f & 2) && !(f & 1);
-    int32_t cnt = 0;
-    if (checker) {
-      double x[NX];
-#pragma unroll
-      for (int k = 0; k < NX; ++k) x[k] = fin[(int64_t)k * pcap + q];
-      int32_t c[3];
-      dp_cell_coords(G, x, c);
-      int64_t w = FILL ? (int64_t)conf_off[q] : 0;
-      for (int cq = 0; cq < 27; ++cq) {
-        const int64_t cell = dp_cell_neighbor(G, c, cq);
-        if (cell < 0) continue;
-        const int32_t e = cstart[cell + 1];
-        for (int32_t t = cstart[cell]; t < e; ++t) {
-          const int32_t o = cidx[t];
-          if (o >= q) continue;
-          double y[NX];
-#pragma unroll
-          for (int k = 0; k < NX; ++k) y[k] = cx[(int64_t)k * pcap + t];
-          if (!(dp_distance(S, x, y) <= r)) continue;
-          if (FILL) {
-            if (w < conf_cap) conf[w] = o;
-            ++w;
-          }
-          ++cnt;
-        }
-      }
+    if (!checker) continue;
+    if (window > 0 && q - app[pnode[q]] >= window) {
+      conf_cnt[q] = -1;
+      continue;
     }
-    if (!FILL) conf_off[q] = cnt;
+    double x[NX];
+#pragma unroll
+    for (int k = 0; k < NX; ++k) x[k] = fin[(int64_t)k * pcap + q];
+    int cnt = 0;
+    scan_earlier<NX>(S, G, cstart, cidx, cx, pcap, x, q, r, [&](int32_t o) {
+      if (cnt < DP_CONF_SLOTS) conf[(int64_t)cnt * pcap + q] = o;
+      ++cnt;
+      return cnt <= DP_CONF_SLOTS;
+    });
+    conf_cnt[q] = (int16_t)(cnt <= DP_CONF_SLOTS ? cnt : -1);
   }
 }
 
@@ -196,13 +209,21 @@ __global__ void k_round_init(int32_t N, const int32_t* __restrict__ app, const i
 }
 
 struct round_args {
+  const dp_dev_sys* sys;
   int32_t N;
   const int32_t* app;
   const int32_t* total;
   const int8_t* verdict;
   const int8_t* flags;
-  const int32_t* conf_off;
+  const int16_t* conf_cnt;
   const int32_t* conf;
+  const double* fin;
+  int64_t pcap;
+  dp_grid_geom grid;
+  const int32_t* cstart;
+  const int32_t* cidx;
+  const double* cx;
+  double r_dup;
   int32_t* res;
   int32_t* cursor;
   int32_t* count;
@@ -216,6 +237,7 @@ struct round_args {
 // true when the node is finished.  Executed by one whole wave.
 // cur / count are wave-uniform registers carried across retries; they are
 // also stored for the round kernel, which reloads them in a later launch.
+template <int NX>
 __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, int32_t& count) {
   const int lane = threadIdx.x & 63;
   const int32_t tot = *a.total;
@@ -226,22 +248,36 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
   for (int32_t cb = cur; cb < end; cb += 64) {
     const int32_t p = cb + lane;
     const bool fr = p < end && a.verdict[p] == DP_V_FREE;
+    if (a.limit > 0 && count >= a.limit) {
+      // the reference has left this node's loop: every later FREE pair is capped
+      if (fr) __hip_atomic_store(&a.res[p], (int)DP_R_CAPPED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      n_cap += __popcll(__ballot(fr));
+      continue;
+    }
     const int8_t fl = fr ? a.flags[p] : 0;
     const bool forced = (fl & 2) != 0;
     const bool dupv = (fl & 1) != 0;
     bool ext_dup = false, ext_block = false;
     uint64_t same = 0;
     if (fr && !forced && !dupv) {
-      const int32_t e1 = a.conf_off[p + 1];
-      for (int32_t e = a.conf_off[p]; e < e1 && !ext_dup; ++e) {
-        const int32_t o = a.conf[e];
+      auto visit = [&](int32_t o) {
         if (o >= cb) {
           same |= 1ull << (o - cb);
-          continue;
+          return true;
         }
         const int r = __hip_atomic_load(&a.res[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (r == DP_R_NOVEL) ext_dup = true;
         else if (r == DP_R_UNDECIDED) ext_block = true;
+        return !ext_dup;
+      };
+      const int cnt = a.conf_cnt[p];
+      if (cnt >= 0) {
+        for (int e = 0; e < cnt && !ext_dup; ++e) visit(a.conf[(int64_t)e * a.pcap + p]);
+      } else {
+        double x[NX];
+#pragma unroll
+        for (int k = 0; k < NX; ++k) x[k] = a.fin[(int64_t)k * a.pcap + p];
+        scan_earlier<NX>(*a.sys, a.grid, a.cstart, a.cidx, a.cx, a.pcap, x, p, a.r_dup, visit);
       }
     }
     const uint64_t free_m = __ballot(fr);
@@ -302,13 +338,14 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
 }
 
 // One resolution round (host-driven completion path): one wave per node.
+template <int NX>
 __global__ __launch_bounds__(64) void k_round(round_args a) {
   if (__hip_atomic_load(&a.counters[DPC_NODES_LEFT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
   const int32_t node = blockIdx.x;
   if (node >= a.N) return;
   if (node == 0 && threadIdx.x == 0) atomicAdd((unsigned long long*)&a.counters[DPC_ROUNDS], 1ull);
   int32_t cur = a.cursor[node], count = a.count[node];
-  (void)advance_node(a, node, cur, count);
+  (void)advance_node<NX>(a, node, cur, count);
 }
 
 // Persistent resolution: waves claim nodes in ascending order through a
@@ -317,6 +354,7 @@ __global__ __launch_bounds__(64) void k_round(round_args a) {
 // waves, so the smallest unfinished node never waits.  The spin is bounded:
 // a wave that exhausts it leaves its node to the round kernel and raises a
 // flag the host checks.
+template <int NX>
 __global__ __launch_bounds__(64) void k_resolve_ticketed(round_args a) {
   __shared__ int32_t s_node;
   if (threadIdx.x == 0) s_node = (int32_t)atomicAdd((unsigned long long*)&a.counters[DPC_TICKET], 1ull);
@@ -325,7 +363,7 @@ __global__ __launch_bounds__(64) void k_resolve_ticketed(round_args a) {
   if (node >= a.N) return;
   int32_t spins = 0;
   int32_t cur = a.cursor[node], count = a.count[node];
-  while (!advance_node(a, node, cur, count)) {
+  while (!advance_node<NX>(a, node, cur, count)) {
     if (++spins > DP_SPIN_LIMIT) {
       if (threadIdx.x == 0) atomicOr((unsigned long long*)&a.counters[DPC_ERRFLAG], 2ull);
       break;
@@ -379,13 +417,21 @@ int dispatch_nx(int nx, F&& f) {
 
 static round_args make_round_args(dp_ctx* c) {
   round_args a{};
+  a.sys = c->d_sys;
   a.N = c->N;
   a.app = c->d_app;
   a.total = c->d_total;
   a.verdict = c->d_verdict;
   a.flags = c->d_flags;
-  a.conf_off = c->d_conf_off;
+  a.conf_cnt = c->d_conf_cnt;
   a.conf = c->d_conf;
+  a.fin = c->d_final;
+  a.pcap = c->pair_cap;
+  a.grid = c->grid;
+  a.cstart = c->d_cont_start;
+  a.cidx = c->d_cont_idx;
+  a.cx = c->d_cont_x;
+  a.r_dup = c->round_r_dup;
   a.res = c->d_res;
   a.cursor = c->d_node_cursor;
   a.count = c->d_node_count;
@@ -409,7 +455,7 @@ int dp_launch_visited_rebuild(dp_ctx* c, double radius, hipStream_t s) {
       constexpr int NX = decltype(NXc)::value;
       k_vis_count<NX><<<dp_grid_blocks(c->nv, 256, 8192), 256, 0, s>>>(c->grid, c->d_vis, c->nv_cap, c->nv,
                                                                       c->d_vis_cell, c->d_vis_start);
-      dp_scan_small(c->d_vis_start, nc, nullptr, INT64_MAX, c->d_counters, -1, s);
+      (void)dp_scan_large(c, c->d_vis_start, nullptr, nc, c->d_counters, -1, s);
       dp_fill_i32(c->d_cont_cursor, nc, 0, s);
       k_vis_scatter<NX><<<dp_grid_blocks(c->nv, 256, 8192), 256, 0, s>>>(c->d_vis, c->nv_cap, c->nv, c->d_vis_cell,
                                                                         c->d_vis_start, c->d_cont_cursor,
@@ -429,54 +475,47 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
   int rc = dp_ensure_i32(&c->d_cont_start, &c->cont_start_cap, nc + 1);
   if (!rc) rc = dp_ensure_i32(&c->d_cont_cursor, &c->cont_cursor_cap, nc + 1);
   if (rc) return rc;
+  c->round_limit = p->limit_branching_factor;
+  c->round_r_dup = r_dup;
+  // Conflict lists are stored for the first `window` pairs of each node; a
+  // node's cursor passes that point only when the branching cap has not been
+  // reached (positions of the cap on the reference workloads: median 22-87,
+  // 90th percentile 28-260).  Without a cap every pair is covered.
+  const int32_t window = p->limit_branching_factor > 0
+                             ? (c->desc.system == DP_SYS_QUADROTOR ? 64 : 256)
+                             : 0;
   const int pb = dp_grid_blocks(c->pair_cap, 256, 8192);
   rc = dispatch_nx(c->desc.nx, [&](auto NXc) -> int {
     constexpr int NX = decltype(NXc)::value;
     dp_fill_i32(c->d_cont_start, nc + 1, 0, s);
     k_cont_count<NX><<<pb, 256, 0, s>>>(c->grid, c->d_total, c->d_verdict, c->d_flags, c->d_final, c->pair_cap,
                                         c->d_cont_start);
-    dp_scan_small(c->d_cont_start, nc, nullptr, INT64_MAX, c->d_counters, -1, s);
+    (void)dp_scan_large(c, c->d_cont_start, nullptr, nc, c->d_counters, -1, s);
     dp_fill_i32(c->d_cont_cursor, nc, 0, s);
     k_cont_scatter<NX><<<pb, 256, 0, s>>>(c->grid, c->d_total, c->d_verdict, c->d_flags, c->d_final, c->pair_cap,
                                           c->d_cont_start, c->d_cont_cursor, c->d_cont_idx, c->d_cont_x);
-    k_conf<NX, false><<<pb, 256, 0, s>>>(c->d_sys, c->grid, c->d_total, c->d_verdict, c->d_flags, c->d_final,
-                                         c->pair_cap, c->d_cont_start, c->d_cont_idx, c->d_cont_x, r_dup,
-                                         c->d_conf_off, nullptr, 0, c->d_counters);
-    return DP_OK;
-  });
-  if (rc) return rc;
-  rc = dp_scan_large(c, c->d_conf_off, c->d_total, c->pair_cap, c->d_counters, DPC_CONF, s);
-  if (rc) return rc;
-  // The list length decides the allocation: one device->host word per batch.
-  DP_HIP_RET(hipMemcpyAsync(c->h_pinned, c->d_counters + DPC_CONF, sizeof(long long), hipMemcpyDeviceToHost, s));
-  DP_HIP_RET(hipStreamSynchronize(s));
-  const int64_t nconf = c->h_pinned[0];
-  if (nconf > c->conf_cap) {
-    rc = dp_ensure_i32(&c->d_conf, &c->conf_cap, nconf + nconf / 4 + 1024);
-    if (rc) return rc;
-  }
-  rc = dispatch_nx(c->desc.nx, [&](auto NXc) -> int {
-    constexpr int NX = decltype(NXc)::value;
-    k_conf<NX, true><<<pb, 256, 0, s>>>(c->d_sys, c->grid, c->d_total, c->d_verdict, c->d_flags, c->d_final,
-                                        c->pair_cap, c->d_cont_start, c->d_cont_idx, c->d_cont_x, r_dup,
-                                        c->d_conf_off, c->d_conf, c->conf_cap, c->d_counters);
-    return DP_OK;
-  });
-  if (rc) return rc;
-  k_round_init<<<dp_grid_blocks(N, 256, 1024), 256, 0, s>>>(N, c->d_app, c->d_total, c->d_node_cursor,
-                                                            c->d_node_count, c->d_node_novel, c->d_counters);
-  c->round_limit = p->limit_branching_factor;
-  {
+    k_conf<NX><<<pb, 256, 0, s>>>(c->d_sys, c->grid, c->d_total, c->d_pnode, c->d_app, c->d_verdict, c->d_flags,
+                                  c->d_final, c->pair_cap, c->d_cont_start, c->d_cont_idx, c->d_cont_x, r_dup,
+                                  window, c->d_conf_cnt, c->d_conf);
+    k_round_init<<<dp_grid_blocks(N, 256, 1024), 256, 0, s>>>(N, c->d_app, c->d_total, c->d_node_cursor,
+                                                              c->d_node_count, c->d_node_novel, c->d_counters);
     round_args ra = make_round_args(c);
-    k_resolve_ticketed<<<N, 64, 0, s>>>(ra);
-  }
+    k_resolve_ticketed<NX><<<N, 64, 0, s>>>(ra);
+    return DP_OK;
+  });
+  if (rc) return rc;
   DP_HIP_RET(hipGetLastError());
   return DP_OK;
 }
 
 int dp_launch_more_rounds(dp_ctx* c, int rounds, hipStream_t s) {
   round_args a = make_round_args(c);
-  for (int r = 0; r < rounds; ++r) k_round<<<c->N, 64, 0, s>>>(a);
+  int rc = dispatch_nx(c->desc.nx, [&](auto NXc) -> int {
+    constexpr int NX = decltype(NXc)::value;
+    for (int r = 0; r < rounds; ++r) k_round<NX><<<c->N, 64, 0, s>>>(a);
+    return DP_OK;
+  });
+  if (rc) return rc;
   DP_HIP_RET(hipGetLastError());
   return DP_OK;
 }

@@ -472,7 +472,7 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
       }
       // check_goal at the float-rule indices, in nn order (dbastar.cpp:492-512)
       while (chosen == -1 && nn < nng && a.goalidx[(int64_t)nn * a.P + j] == k) {
-        if (dp_distance(S, s, a.goal) <= a.goal_bound) {
+        if (dp_within(S, s, a.goal, a.goal_bound)) {
           chosen = k;
 #pragma unroll
           for (int c = 0; c < 13; ++c) x[c] = s[c];
@@ -519,7 +519,7 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
           double y[13];
 #pragma unroll
           for (int c = 0; c < 13; ++c) y[c] = a.vis[(int64_t)c * a.vstride + p];
-          dupv = dp_distance(S, x, y) <= a.r_dup;
+          dupv = dp_within(S, x, y, a.r_dup);
         }
       }
     }

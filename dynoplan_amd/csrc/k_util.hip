@@ -61,9 +61,10 @@ constexpr int kTileItems = 16;
 constexpr int kTile = kTileThreads * kTileItems;  // 4096
 
 // Pass 1: per-tile exclusive scan in place, tile sums to tiles[].
-__global__ __launch_bounds__(kTileThreads) void k_scan_tiles(int32_t* data, const int32_t* n_dev, int32_t* tiles) {
+__global__ __launch_bounds__(kTileThreads) void k_scan_tiles(int32_t* data, const int32_t* n_dev, int64_t n_host,
+                                                             int32_t* tiles) {
   __shared__ long long s[kTileThreads];
-  const int64_t n = *n_dev;
+  const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
   const int64_t base = (int64_t)blockIdx.x * kTile;
   if (base >= n) {
     if (threadIdx.x == 0) tiles[blockIdx.x] = 0;
@@ -97,9 +98,10 @@ __global__ __launch_bounds__(kTileThreads) void k_scan_tiles(int32_t* data, cons
 }
 
 // Pass 3: add tile offsets; write data[n] = total.
-__global__ __launch_bounds__(kTileThreads) void k_scan_add(int32_t* data, const int32_t* n_dev, const int32_t* tiles,
-                                                          int64_t ntiles, long long* counters, int slot) {
-  const int64_t n = *n_dev;
+__global__ __launch_bounds__(kTileThreads) void k_scan_add(int32_t* data, const int32_t* n_dev, int64_t n_host,
+                                                          const int32_t* tiles, int64_t ntiles, long long* counters,
+                                                          int slot) {
+  const int64_t n = n_dev ? (int64_t)*n_dev : n_host;
   const int64_t base = (int64_t)blockIdx.x * kTile;
   if (base < n) {
     const int32_t off = tiles[blockIdx.x];
@@ -134,15 +136,18 @@ int dp_scan_large(dp_ctx* c, int32_t* data, const int32_t* n_dev, int64_t n_max,
   const int64_t ntiles = (n_max + kTile - 1) / kTile;
   int rc = dp_ensure_i32(&c->d_tiles, &c->tiles_cap, ntiles + 1);
   if (rc) return rc;
-  k_scan_tiles<<<(int)ntiles, kTileThreads, 0, s>>>(data, n_dev, c->d_tiles);
+  k_scan_tiles<<<(int)ntiles, kTileThreads, 0, s>>>(data, n_dev, n_max, c->d_tiles);
   k_scan_small<<<1, 1024, 0, s>>>(c->d_tiles, ntiles, nullptr, INT64_MAX, counters, -1);
-  k_scan_add<<<(int)ntiles, kTileThreads, 0, s>>>(data, n_dev, c->d_tiles, ntiles, counters, slot);
+  k_scan_add<<<(int)ntiles, kTileThreads, 0, s>>>(data, n_dev, n_max, c->d_tiles, ntiles, counters, slot);
   DP_HIP_RET(hipGetLastError());
   return DP_OK;
 }
 
+// Growth frees a buffer earlier kernels of the batch may still read, so the
+// device is drained first (growth happens on the first batches only).
 int dp_ensure_i32(int32_t** p, int64_t* cap, int64_t need) {
   if (*cap >= need && *p) return DP_OK;
+  DP_HIP_RET(hipDeviceSynchronize());
   if (*p) (void)hipFree(*p);
   *p = nullptr;
   *cap = 0;
@@ -153,6 +158,7 @@ int dp_ensure_i32(int32_t** p, int64_t* cap, int64_t need) {
 
 int dp_ensure_f64(double** p, int64_t* cap, int64_t need) {
   if (*cap >= need && *p) return DP_OK;
+  DP_HIP_RET(hipDeviceSynchronize());
   if (*p) (void)hipFree(*p);
   *p = nullptr;
   *cap = 0;
