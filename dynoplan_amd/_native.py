@@ -98,6 +98,16 @@ class PairsOut(C.Structure):
     ]
 
 
+class PrimSet(C.Structure):
+    _fields_ = [
+        ("P", C.c_int32), ("nx", C.c_int32), ("nu", C.c_int32), ("reserved", C.c_int32),
+        ("offsets", C.POINTER(C.c_int32)), ("states", C.POINTER(C.c_double)),
+        ("actions", C.POINTER(C.c_double)), ("cost", C.POINTER(C.c_double)),
+    ]
+
+
+FMT_AUTO, FMT_YAML, FMT_JSON, FMT_MSGPACK, FMT_BOOST = 0, 1, 2, 3, 4
+
 # name -> (restype, argtypes); every symbol include/dynoplan_amd.h declares.
 SIGNATURES = {
     "dp_last_error": (C.c_char_p, []),
@@ -122,6 +132,10 @@ SIGNATURES = {
     "dp_stream": (C.c_void_p, [C.c_void_p]),
     "dp_last_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "dp_debug_fp32_positions": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int64]),
+    "dp_load_primitives": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_uint64,
+                                     C.POINTER(PrimSet)]),
+    "dp_free_primitives": (None, [C.POINTER(PrimSet)]),
+    "dp_save_primitives": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(PrimSet)]),
 }
 
 _lib = None

@@ -218,6 +218,35 @@ void* dp_stream(dp_ctx* ctx);
  * 2 duplicate resolution, 3 total. */
 int dp_last_timing(dp_ctx* ctx, float* ms4);
 
+/* ---- primitive files (load_motion_primitives_new, src/ompl/robots.cpp:3061-3174) ---- */
+enum { DP_FMT_AUTO = 0, DP_FMT_YAML = 1, DP_FMT_JSON = 2, DP_FMT_MSGPACK = 3, DP_FMT_BOOST = 4 };
+
+/* Host arrays of a loaded primitive set, in the dp_prims layout (allocated
+ * by dp_load_primitives, released by dp_free_primitives). */
+typedef struct dp_prim_set {
+  int32_t P, nx, nu, reserved;
+  int32_t* offsets;            /* P+1 */
+  double* states;              /* [offsets[P]][nx] */
+  double* actions;             /* [offsets[P]-P][nu] */
+  double* cost;                /* [P] */
+} dp_prim_set;
+
+/* Load a primitive file as load_motion_primitives_new does: format from the
+ * extension when DP_FMT_AUTO (robots.cpp:3070-3080), truncation to
+ * max_motions (<= 0: all, :3104-3105), first/last state + noise*U[-1,1]
+ * drawn from a seeded stream (the reference adds 1e-7 from std::rand,
+ * :3113-3127; noise = 0 disables it), quaternion normalisation for the
+ * quadrotor (:3130-3137), cost = ref_dt * #actions when the first cost
+ * exceeds 1e5 (:3153-3159).  Primitive j gets idx = j (:3170-3173). */
+int dp_load_primitives(const char* path, int32_t format, int32_t system, int32_t max_motions, double noise,
+                       uint64_t seed, dp_prim_set* out);
+void dp_free_primitives(dp_prim_set* set);
+
+/* Write a primitive set as Boost binary (.bin), msgpack or YAML; the
+ * readers above load these files back bit-exactly (format conversion of
+ * main_primitives' bintoyaml / yamltobin modes, main_primitives.cpp:4-20). */
+int dp_save_primitives(const char* path, int32_t format, const dp_prim_set* set);
+
 /* Quadrotor FP32 rollout error report: the FP32 final position (absolute,
  * [3][n] row per axis) of every pair the FP32 pass found FREE in the last
  * dp_expand (other entries unspecified).  No counterpart in the reference;

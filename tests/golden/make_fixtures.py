@@ -9,6 +9,7 @@ script, so the tests never read the reference at run time.
   unicycle1_ref_prims.bin   the Boost binary twin of the same 100 primitives
   unicycle1_ref10.npz       data/motions__unicycle1_v0__20-07-2023--16-04-22.yaml
   unicycle1_ref10.bin       its Boost binary twin
+  unicycle1_ref10.yaml      the YAML file itself (format pin of the YAML reader)
   quadrotor_initguess.npz   benchmark_initguess/quadrotor_v0/*/*.yaml solution
                             trajectories (states, actions, primitive junctions)
 """
@@ -49,6 +50,7 @@ def main():
     offsets, states, actions = pack(trajs)
     np.savez_compressed(os.path.join(OUT, "unicycle1_ref10.npz"), offsets=offsets, states=states, actions=actions)
     shutil.copyfile(y[: -len(".yaml")] + ".bin", os.path.join(OUT, "unicycle1_ref10.bin"))
+    shutil.copyfile(y, os.path.join(OUT, "unicycle1_ref10.yaml"))
 
     names, st, ac, splits, offs = [], [], [], [], [0]
     for p in sorted(glob.glob(os.path.join(REF, "benchmark_initguess/quadrotor_v0/*/*.yaml"))):
