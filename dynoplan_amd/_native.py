@@ -108,6 +108,15 @@ class PrimSet(C.Structure):
 
 FMT_AUTO, FMT_YAML, FMT_JSON, FMT_MSGPACK, FMT_BOOST = 0, 1, 2, 3, 4
 
+
+class SurrogateStats(C.Structure):
+    _fields_ = [("threshold", C.c_float), ("reserved", C.c_float)] + [
+        (n, C.c_int64) for n in ("pairs", "feasible", "infeasible", "rejectable", "colliding", "rejectable_colliding")]
+
+    def as_dict(self):
+        return {n: (float(getattr(self, n)) if n == "threshold" else int(getattr(self, n)))
+                for n, _ in self._fields_ if n != "reserved"}
+
 # name -> (restype, argtypes); every symbol include/dynoplan_amd.h declares.
 SIGNATURES = {
     "dp_last_error": (C.c_char_p, []),
@@ -136,6 +145,18 @@ SIGNATURES = {
                                      C.POINTER(PrimSet)]),
     "dp_free_primitives": (None, [C.POINTER(PrimSet)]),
     "dp_save_primitives": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(PrimSet)]),
+    "dp_surrogate_init": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "dp_surrogate_train": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int32, C.POINTER(ExpandParams),
+                                     C.c_int32, C.c_int32, C.c_float, C.c_uint64, C.POINTER(C.c_float)]),
+    "dp_surrogate_calibrate": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int32, C.POINTER(ExpandParams),
+                                         C.c_float, C.POINTER(SurrogateStats)]),
+    "dp_surrogate_set_threshold": (C.c_int, [C.c_void_p, C.c_float]),
+    "dp_surrogate_scores": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int64]),
+    "dp_surrogate_param_count": (C.c_int64, []),
+    "dp_surrogate_grad": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                    C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "dp_debug_gemm_bf16": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32, C.c_int32,
+                                     C.c_int32, C.POINTER(C.c_float)]),
 }
 
 _lib = None

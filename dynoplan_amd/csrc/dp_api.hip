@@ -215,6 +215,7 @@ void dp_destroy(dp_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  dp_mlp_free(c);
   dev_free(&c->d_sys);
   dev_free(&c->d_off);
   dev_free(&c->d_st);

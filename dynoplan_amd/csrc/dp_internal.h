@@ -85,6 +85,37 @@ __host__ __device__ inline int64_t dp_cell_neighbor(const dp_grid_geom& G, const
   return ((int64_t)n2 * G.dims[1] + n1) * G.dims[0] + n0;
 }
 
+// Learned surrogate state (k_mlp.hip).
+struct dp_mlp {
+  int32_t P = 0;
+  int64_t step = 0;
+  bool ready = false;        // trained and calibrated
+  bool cache_valid = false;  // per-primitive first-layer terms up to date
+  float threshold = -INFINITY;
+  float* w = nullptr;  int64_t w_cap = 0;    // FP32 master parameters
+  float* g = nullptr;  int64_t g_cap = 0;    // gradients
+  float* m1 = nullptr; int64_t m1_cap = 0;   // Adam moments
+  float* m2 = nullptr; int64_t m2_cap = 0;
+  uint16_t* wb = nullptr; int64_t wb_cap = 0;  // BF16 parameters
+  float* emb = nullptr; int64_t emb_cap = 0;   // [P][64] x {value, grad, m, v}
+  uint16_t* embb = nullptr; int64_t embb_cap = 0;
+  // activations / scratch of training
+  float *h1 = nullptr, *h2 = nullptr, *h3 = nullptr, *dh = nullptr, *zb = nullptr;
+  int64_t h1_cap = 0, h2_cap = 0, h3_cap = 0, dh_cap = 0, zb_cap = 0;
+  uint16_t *h1b = nullptr, *h2b = nullptr, *h3b = nullptr, *dhb = nullptr, *xb = nullptr;
+  int64_t h1b_cap = 0, h2b_cap = 0, h3b_cap = 0, dhb_cap = 0, xb_cap = 0;
+  // inference
+  uint16_t* patch = nullptr; int64_t patch_cap = 0;  // [N][1024]
+  float* An = nullptr; int64_t An_cap = 0;            // [N][256]
+  float* Bp = nullptr; int64_t Bp_cap = 0;            // [P][256]
+  float* score = nullptr; int64_t score_cap = 0;      // [pair_cap]
+  // training set
+  int32_t *snode = nullptr, *sprim = nullptr, *sid = nullptr;
+  float* slabel = nullptr;
+  int64_t s_cap = 0, sid_cap = 0;
+  float* loss = nullptr;
+};
+
 struct dp_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -161,6 +192,7 @@ struct dp_ctx {
   int32_t* d_chunk = nullptr;       // [N_cap+1] quadrotor: 64-pair chunks per node, scanned
   float* d_dbg32 = nullptr;         // [3][pair_cap] quadrotor: FP32 final positions (error report)
   double prim_reach = 0.0;          // max over primitives of max_k |p_k - p_0|_inf
+  dp_mlp mlp;
   int32_t round_limit = 0;          // limit_branching_factor of the batch in flight
 
   // counters
@@ -215,6 +247,16 @@ int dp_launch_more_rounds(dp_ctx* c, int rounds, hipStream_t s);
 
 // ---- k_quad.hip ---------------------------------------------------------
 int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, double r_dup, hipStream_t s);
+
+// ---- k_mlp.hip ----------------------------------------------------------
+int dp_mlp_alloc(dp_ctx* c, int32_t P, uint64_t seed);
+int64_t dp_mlp_param_count();
+int dp_mlp_fwd_bwd(dp_ctx* c, const uint16_t* X, const float* y, int32_t B, const int32_t* sample,
+                   const int32_t* sprim, float* loss, hipStream_t s);
+// Score the pairs of the batch (after k_app fill); rejects below the
+// threshold when `reject`.  Pairs left for the exact check get verdict -1.
+int dp_launch_surrogate(dp_ctx* c, bool reject, hipStream_t s);
+void dp_mlp_free(dp_ctx* c);
 
 // ---- k_expand.hip -------------------------------------------------------
 int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s);

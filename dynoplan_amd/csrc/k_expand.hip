@@ -107,6 +107,7 @@ struct dp_check_args {
   double* hout;
   double goal[DP_MAX_NX];
   double goal_bound, r_dup, cdf;
+  int32_t surrogate;  // pairs already marked DP_V_SURROGATE_REJECT are skipped
   long long* counters;
 };
 
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
   for (int64_t q0 = ((int64_t)blockIdx.x * kWaves + wave) * SEG; q0 < total;
        q0 += (int64_t)gridDim.x * kWaves * SEG) {
     const int64_t q = q0 + seg;
-    const bool vp = q < total;
+    const bool vp = q < total && !(a.surrogate && a.verdict[q] == DP_V_SURROGATE_REJECT);
     int32_t i = 0, j = 0, st0 = 0, L1 = 0;
     double ox = 0.0, oy = 0.0;
     if (vp) {
@@ -358,6 +359,18 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
     return DP_OK;
   });
   if (rc) return rc;
+  if (p->use_surrogate) {
+    if (!c->mlp.w || (p->use_surrogate == 1 && !c->mlp.ready)) {
+      dp_set_error("use_surrogate needs a trained and calibrated surrogate");
+      return DP_ERR_STATE;
+    }
+    if (c->desc.system == DP_SYS_QUADROTOR) {
+      dp_set_error("the surrogate is defined for planar systems");
+      return DP_ERR_ARG;
+    }
+    rc = dp_launch_surrogate(c, p->use_surrogate == 1, s);
+    if (rc) return rc;
+  }
   DP_HIP_RET(hipEventRecord(c->ev[1], s));
 
   dp_check_args a{};
@@ -398,6 +411,7 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   a.goal_bound = goal_bound;
   a.r_dup = r_dup;
   a.cdf = p->cost_delta_factor;
+  a.surrogate = p->use_surrogate ? 1 : 0;
   a.counters = c->d_counters;
 
   if (c->desc.system == DP_SYS_QUADROTOR) {

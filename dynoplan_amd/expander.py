@@ -123,6 +123,29 @@ class GpuExpander:
     def stream(self):
         return self.lib.dp_stream(self.ctx)
 
+    # ---- learned surrogate -------------------------------------------------------
+    def surrogate_init(self, seed=1):
+        N.check(self.lib.dp_surrogate_init(self.ctx, seed))
+
+    def surrogate_train(self, nodes, params, epochs=3, batch=4096, lr=1e-3, seed=7):
+        nodes = np.ascontiguousarray(nodes, np.float64).reshape(-1, self.nx)
+        losses = np.zeros(epochs, np.float32)
+        N.check(self.lib.dp_surrogate_train(self.ctx, _dptr(nodes), nodes.shape[0], C.byref(params), epochs, batch,
+                                            lr, seed, losses.ctypes.data_as(C.POINTER(C.c_float))))
+        return losses
+
+    def surrogate_calibrate(self, nodes, params, margin=0.5):
+        nodes = np.ascontiguousarray(nodes, np.float64).reshape(-1, self.nx)
+        st = N.SurrogateStats()
+        N.check(self.lib.dp_surrogate_calibrate(self.ctx, _dptr(nodes), nodes.shape[0], C.byref(params), margin,
+                                                C.byref(st)))
+        return st.as_dict()
+
+    def surrogate_scores(self, n):
+        out = np.zeros(n, np.float32)
+        N.check(self.lib.dp_surrogate_scores(self.ctx, out.ctypes.data_as(C.POINTER(C.c_float)), n))
+        return out
+
     def fetch(self):
         n = self.counts()["pairs"]
         out = {

@@ -247,6 +247,40 @@ void dp_free_primitives(dp_prim_set* set);
  * main_primitives' bintoyaml / yamltobin modes, main_primitives.cpp:4-20). */
 int dp_save_primitives(const char* path, int32_t format, const dp_prim_set* set);
 
+/* ---- learned collision-feasibility surrogate --------------------------------
+ * No counterpart in the reference.  An MLP over a 32x32 occupancy patch
+ * (0.1 m cells) around the node and a learned 64-d primitive embedding
+ * (3 hidden layers of 256, BF16 MFMA with FP32 accumulation) scores each
+ * pair; with dp_expand_params.use_surrogate = 1, pairs scoring below the
+ * calibrated threshold get DP_V_SURROGATE_REJECT and skip the exact check,
+ * every other pair is checked exactly.  Planar systems. */
+typedef struct dp_surrogate_stats {
+  float threshold;
+  float reserved;
+  int64_t pairs, feasible, infeasible;
+  int64_t rejectable;            /* infeasible pairs scoring below the threshold */
+  int64_t colliding, rejectable_colliding;
+} dp_surrogate_stats;
+
+int dp_surrogate_init(dp_ctx* ctx, uint64_t seed);
+/* Exact-checker labels (FREE = 1) of the pairs of `nodes`, then `epochs`
+ * passes of minibatch Adam; loss_out[epochs] (optional) gets the epoch losses. */
+int dp_surrogate_train(dp_ctx* ctx, const double* nodes, int32_t N, const dp_expand_params* params, int32_t epochs,
+                       int32_t batch, float lr, uint64_t seed, float* loss_out);
+/* Threshold = lowest score of a FREE pair of held-out `nodes` minus margin. */
+int dp_surrogate_calibrate(dp_ctx* ctx, const double* nodes, int32_t N, const dp_expand_params* params,
+                           float margin, dp_surrogate_stats* out);
+int dp_surrogate_set_threshold(dp_ctx* ctx, float threshold);
+/* Scores of the last expansion run with use_surrogate != 0. */
+int dp_surrogate_scores(dp_ctx* ctx, float* out, int64_t n);
+int64_t dp_surrogate_param_count(void);
+/* Forward + backward on explicit inputs X [B][1088] (patch | embedding),
+ * labels y [B], parameters `params`: mean BCE loss and its gradient. */
+int dp_surrogate_grad(dp_ctx* ctx, const float* params, const float* X, const float* y, int32_t B, float* loss,
+                      float* grad);
+/* Numerics check of the training GEMM: C = A B, inputs rounded to BF16. */
+int dp_debug_gemm_bf16(dp_ctx* ctx, const float* A, const float* B, int32_t M, int32_t N, int32_t K, float* C);
+
 /* Quadrotor FP32 rollout error report: the FP32 final position (absolute,
  * [3][n] row per axis) of every pair the FP32 pass found FREE in the last
  * dp_expand (other entries unspecified).  No counterpart in the reference;
