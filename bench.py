@@ -37,6 +37,7 @@ def workloads():
         "unicycle1_small": synth.unicycle1_small,
         "car_trailer": synth.car_trailer,
         "quadrotor": synth.quadrotor,
+        "multi": synth.multi_instance,
     }
 
 
@@ -51,21 +52,45 @@ def algorithmic_bytes(wl, prim):
     return float(np.sum(lens * wl.desc.nx * 8 + 8))
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(wl, n_nodes):
+    """The FP64 oracle on the host cores: single-threaded (as the reference's
+    loop) and with OpenMP over frontier nodes; ``value`` is the threaded one."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    nodes = wl.nodes[:n_nodes]
-    O.expand(wl, nodes=nodes[:4], g_parent=wl.g_parent[:4])  # warm-up (load, page in)
-    times, pairs = [], 0
-    for _ in range(3):
-        t0 = time.perf_counter()
-        _, c = O.expand(wl, nodes=nodes, g_parent=wl.g_parent[:n_nodes])
-        times.append(time.perf_counter() - t0)
-        pairs = c["pairs"]
-    t = float(np.median(times))
-    return {"value": pairs / t, "unit": "pairs/s", "cores": 1, "kind": "port",
-            "sample": f"first {n_nodes} of {wl.nodes.shape[0]} frontier nodes ({pairs} pairs), "
-                      f"single-threaded FP64 oracle (grid NN, swept-box cull), median of 3"}
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    nodes = np.ascontiguousarray(wl.nodes[:n_nodes])
+    gp = np.ascontiguousarray(wl.g_parent[:n_nodes])
+    O.expand(wl, nodes=nodes[:4], g_parent=gp[:4], want_out=False)  # warm-up (load, page in)
+
+    def timed(reps, thr):
+        times, pairs = [], 0
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            _, c = O.expand(wl, nodes=nodes, g_parent=gp, threads=thr, want_out=False)
+            times.append(time.perf_counter() - t0)
+            pairs = c["pairs"]
+        return pairs, float(np.median(times))
+
+    pairs, t1 = timed(3, 0)
+    progress(f"cpu oracle single-threaded: {pairs} pairs in {t1:.3f} s")
+    _, tn = timed(5, threads)
+    progress(f"cpu oracle {threads} threads: {pairs} pairs in {tn:.3f} s")
+    return {"value": pairs / tn, "unit": "pairs/s", "cores": threads, "kind": "port",
+            "single_thread_value": pairs / t1, "cpu_model": cpu_model(),
+            "sample": f"first {n_nodes} of {wl.nodes.shape[0]} frontier nodes ({pairs} pairs); FP64 oracle "
+                      f"(grid NN, swept-box cull) with OpenMP over nodes on {threads} threads, median of 5; "
+                      f"single-threaded median of 3 in single_thread_value"}
 
 
 def plain_gpu(ex, wl, n_nodes):
@@ -90,14 +115,168 @@ def plain_gpu(ex, wl, n_nodes):
             "ms_per_node": 1e3 * t / n_nodes}
 
 
+METRIC = "checked primitive expansions/s, batched+surrogate vs plain GPU at 100% recall"
+
+
+def train_surrogate(ex, wl, label_pairs):
+    """Untimed: exact-checker labels of fresh frontier nodes (same
+    distribution, other seed), minibatch Adam on the GPU, then the
+    conservative threshold from held-out nodes."""
+    import copy
+    from dynoplan_amd import synth
+    rng = np.random.default_rng(wl.seed + 100)
+    lo, hi = wl.lo + 0.5, wl.hi - 0.5
+
+    def extra(st):
+        st[:, 3] = synth.wrap_angle(st[:, 2] + rng.uniform(-0.5, 0.5, size=st.shape[0]))
+
+    probe = synth.free_planar_nodes(rng, wl.desc, wl.boxes, lo, hi, 1024, extra)
+    p0 = copy.copy(wl.params)
+    p0.use_surrogate = 0
+    per_node = ex.expand(probe, np.zeros(probe.shape[0]), p0)["pairs"] / probe.shape[0]
+    n_train = int(np.ceil(label_pairs / max(per_node, 1e-9)))
+    train_nodes = synth.free_planar_nodes(rng, wl.desc, wl.boxes, lo, hi, n_train, extra)
+    cal_nodes = synth.free_planar_nodes(rng, wl.desc, wl.boxes, lo, hi, 4096, extra)
+    n_labels = ex.expand(train_nodes, np.zeros(n_train), p0)["pairs"]
+    progress(f"surrogate: {n_labels} exact-checker labels from {n_train} nodes")
+    ex.surrogate_init(seed=wl.seed)
+    t0 = time.perf_counter()
+    losses = ex.surrogate_train(train_nodes, p0, epochs=4, batch=4096, lr=2e-3, seed=wl.seed)
+    t_train = time.perf_counter() - t0
+    stats = ex.surrogate_calibrate(cal_nodes, p0, margin=0.5)
+    progress(f"surrogate: trained in {t_train:.1f} s, epoch losses {[round(float(x), 4) for x in losses]}, "
+             f"calibration {stats}")
+    return {"labels": int(n_labels), "train_nodes": n_train, "epochs": 4, "train_s": t_train,
+            "epoch_losses": [float(x) for x in losses], "calibration": stats}
+
+
+def time_steps(ex, params, steps):
+    import torch
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ex.expand_async(params)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def bench_multi(args):
+    """128 independent instances (unicycle1, unicycle2, quadrotor) expanded as
+    one hipGraph launch; throughput, per-instance completion latency p50/p99,
+    and the same instances launched one stream each and one after another."""
+    import copy
+    import torch
+    from dynoplan_amd import _native as N
+    from dynoplan_amd.expander import GpuBatch, expander_for
+
+    t0 = time.perf_counter()
+    wls = workloads()["multi"]()
+    progress(f"generated {len(wls)} instances in {time.perf_counter() - t0:.1f} s")
+    torch.cuda.init()
+    exs = []
+    for w in wls:
+        e = expander_for(w, device=0)
+        e.upload_nodes(w.nodes, w.g_parent)
+        exs.append(e)
+    params = []
+    for w in wls:
+        p = copy.copy(w.params)
+        p.fp32_rollout = 1 if w.desc.system == N.SYS_QUADROTOR else 0
+        params.append(p)
+    progress(f"uploaded {len(exs)} contexts")
+
+    def run(mode, steps, warmup):
+        b = GpuBatch(exs, mode)
+        for _ in range(warmup):
+            b.expand_async(params)
+            torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            b.expand_async(params)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        lat = []
+        for _ in range(steps):
+            b.expand_async(params)
+            lat.append(b.latency_ms())
+        st = b.stats()
+        b.close()
+        return el, np.array(lat), st
+
+    el_g, lat_g, st_g = run(N.BATCH_GRAPH, args.steps, args.warmup)
+    counts = [e.counts() for e in exs]
+    pairs = sum(c["pairs"] for c in counts)
+    value = pairs * args.steps / el_g
+    progress(f"graph: {args.steps} steps in {el_g:.3f} s, {pairs} pairs per step, stats {st_g}")
+    el_s, lat_s, _ = run(N.BATCH_STREAMS, args.steps, 1)
+    progress(f"streams: {args.steps} steps in {el_s:.3f} s")
+    # one instance after another on one stream (no batching across instances)
+    one = torch.cuda.current_stream().cuda_stream
+    for e, p in zip(exs, params):
+        e.expand_async(p, one)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        for e, p in zip(exs, params):
+            e.expand_async(p, one)
+    torch.cuda.synchronize()
+    el_q = time.perf_counter() - t
+    progress(f"sequential: {args.steps} steps in {el_q:.3f} s")
+    abytes = 0.0
+    for e, w in zip(exs, wls):
+        abytes += algorithmic_bytes(w, e.fetch()["prim"])
+    achieved = abytes / (el_g / args.steps) / 1e9
+    batch_lat = lat_g.max(axis=1)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el_g / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (planar), f32 rollout + f64 re-check (quadrotor)",
+        "data": "synthetic",
+        "config": {"workload": f"{len(wls)} mixed instances (unicycle1, unicycle2, quadrotor), O U{{10..300}}, "
+                               f"P U{{1000..10000}}, N U{{64..4096}}, V 10000, one hipGraph launch",
+                   "instances": len(wls), "nodes": int(sum(w.nodes.shape[0] for w in wls)),
+                   "pairs_per_step": int(pairs), "surrogate": False},
+        "latency_ms": {"instance_p50": float(np.percentile(lat_g, 50)), "instance_p99": float(np.percentile(lat_g, 99)),
+                       "batch_p50": float(np.percentile(batch_lat, 50)), "batch_p99": float(np.percentile(batch_lat, 99))},
+        "streams_value": pairs * args.steps / el_s,
+        "streams_latency_ms": {"instance_p50": float(np.percentile(lat_s, 50)),
+                               "instance_p99": float(np.percentile(lat_s, 99))},
+        "sequential_value": pairs * args.steps / el_q,
+        "graph_stats": st_g,
+        "roofline": {"bound": "hbm", "kernel": "whole batch (all kernels)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": abytes},
+    }
+    if not args.no_baselines:
+        sample = [0, 1, 2, 3]
+        cbs = [cpu_baseline(wls[i], 128) for i in sample]
+        line["cpu_baseline"] = {"value": float(np.mean([cb["value"] for cb in cbs])), "unit": "pairs/s",
+                                "cores": cbs[0]["cores"], "kind": "port",
+                                "sample": f"first 128 nodes of instances {sample}, mean of their rates",
+                                "per_instance": [cb["value"] for cb in cbs]}
+    for e in exs:
+        e.close()
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="unicycle1_scale", choices=sorted(workloads()))
-    ap.add_argument("--cpu-nodes", type=int, default=4096, help="frontier nodes in the CPU baseline sample")
+    ap.add_argument("--cpu-nodes", type=int, default=0,
+                    help="frontier nodes in the CPU baseline sample (0: 4096 planar, 512 quadrotor)")
     ap.add_argument("--plain-nodes", type=int, default=256, help="nodes expanded one per call (plain path)")
+    ap.add_argument("--label-pairs", type=int, default=2_000_000, help="surrogate training labels (car_trailer)")
+    ap.add_argument("--no-surrogate", action="store_true", help="car_trailer without the learned pre-filter")
     ap.add_argument("--no-baselines", action="store_true", help="skip CPU and plain-GPU legs (profiling runs)")
     args = ap.parse_args()
 
@@ -106,47 +285,53 @@ def main():
               "(multi-GPU sharding is a design note, not a measured configuration).", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "multi":
+        bench_multi(args)
+        return
+
+    import copy
     import torch
+    from dynoplan_amd import _native as N
     from dynoplan_amd.expander import expander_for
 
     wl = workloads()[args.workload]()
-    from dynoplan_amd import _native as N
     is_quad = wl.desc.system == N.SYS_QUADROTOR
+    use_sur = args.workload == "car_trailer" and not args.no_surrogate
     torch.cuda.init()
     ex = expander_for(wl, device=0)
+    progress(f"workload {wl.name}: {wl.nodes.shape[0]} nodes, {wl.offsets.size - 1} primitives")
+    surrogate = train_surrogate(ex, wl, args.label_pairs) if use_sur else None
+    params = copy.copy(wl.params)
+    params.use_surrogate = 1 if use_sur else 0
+
     ex.upload_nodes(wl.nodes, wl.g_parent)
     ex.reserve(wl.nodes.shape[0], wl.nodes.shape[0] * (wl.offsets.size - 1))
-
-    progress(f"workload {wl.name}: {wl.nodes.shape[0]} nodes, {wl.offsets.size - 1} primitives")
     for _ in range(args.warmup):
-        ex.expand_async(wl.params)
+        ex.expand_async(params)
         torch.cuda.synchronize()
         progress(f"warm-up step done: {ex.timing()}")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ex.expand_async(wl.params)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    elapsed = time_steps(ex, params, args.steps)
     counts = ex.counts()
-    progress(f"timed {args.steps} steps in {elapsed:.3f} s: {counts['pairs']} pairs per step")
     pairs = counts["pairs"]
     value = pairs * args.steps / elapsed
+    progress(f"timed {args.steps} steps in {elapsed:.3f} s: {pairs} pairs per step")
 
     # Per-stage device time from HIP events recorded on the expansion stream.
-    stage = {"applicability": [], "check": [], "resolve": [], "total": []}
+    stage = {"applicability": [], "surrogate": [], "check": [], "resolve": [], "total": []}
     for _ in range(args.steps):
-        ex.expand_async(wl.params)
+        ex.expand_async(params)
         t = ex.timing()
         for k in stage:
             stage[k].append(t[k])
     stage_ms = {k: float(np.mean(v)) for k, v in stage.items()}
+    progress(f"stage ms {stage_ms}")
     out = ex.fetch()
-    abytes = algorithmic_bytes(wl, out["prim"])
-    check_s = stage_ms["check"] * 1e-3
-    achieved = abytes / check_s / 1e9
+    checked = out["verdict"] != N.V_SURROGATE_REJECT
+    abytes = algorithmic_bytes(wl, out["prim"][checked])
+    achieved = abytes / (stage_ms["check"] * 1e-3) / 1e9
 
     line = {
-        "metric": "checked primitive expansions/s (batched GPU path; plain GPU path and CPU oracle alongside)",
+        "metric": METRIC,
         "value": value,
         "unit": "pairs/s",
         "n_gpus": 1,
@@ -156,27 +341,47 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 rollout + f64 re-check" if is_quad else "f64",
+        "dtype": "f32 rollout + f64 re-check" if is_quad else ("f64 + bf16 surrogate" if use_sur else "f64"),
         "data": "synthetic",
         "config": {"workload": wl.name, "nodes": int(wl.nodes.shape[0]), "primitives": int(wl.offsets.size - 1),
                    "obstacles": int(wl.boxes.shape[0] + wl.spheres.shape[0]), "visited": int(wl.visited.shape[0]),
                    "delta": float(wl.params.delta), "alpha": float(wl.params.alpha),
                    "limit_branching_factor": int(wl.params.limit_branching_factor), "seed": wl.seed,
-                   "pairs_per_step": int(pairs)},
+                   "surrogate": use_sur, "pairs_per_step": int(pairs)},
         "counts": counts,
         "stage_ms": stage_ms,
-        "roofline": {"bound": "hbm", "kernel": "k_quad32+k_quad64" if is_quad else "k_check_planar", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes_per_launch": abytes},
+        "roofline": {"bound": "hbm", "kernel": "k_quad32+k_quad64" if is_quad else "k_check_planar",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes_per_launch": abytes},
     }
-    progress(f"stage ms {stage_ms}")
+    if use_sur:
+        # Untimed validation: the exact path on the same batch decides every
+        # pair; the surrogate run must keep every feasible edge and give the
+        # same edge set.
+        exact = copy.copy(wl.params)
+        exact.use_surrogate = 0
+        ex.expand_async(exact)
+        ref = ex.fetch()
+        free = ref["verdict"] == N.V_FREE
+        kept = out["verdict"][free] == N.V_FREE
+        rej = out["verdict"] == N.V_SURROGATE_REJECT
+        recall = float(kept.mean()) if free.any() else 1.0
+        same_edges = bool(np.array_equal(out["resolution"], ref["resolution"]))
+        t_exact = time_steps(ex, exact, args.steps)
+        surrogate.update({"recall": recall, "edge_set_identical": same_edges,
+                          "rejected_pairs": int(rej.sum()), "infeasible_pairs": int((~free).sum()),
+                          "rejected_share_of_infeasible": float(rej.sum() / max(1, (~free).sum())),
+                          "batched_exact_value": pairs * args.steps / t_exact,
+                          "speedup_vs_batched_exact": t_exact / elapsed})
+        progress(f"surrogate validation: {surrogate}")
+        line["surrogate"] = surrogate
     if not args.no_baselines:
         pg = plain_gpu(ex, wl, args.plain_nodes)
         progress(f"plain GPU path: {pg}")
         line["plain_gpu"] = pg
         line["speedup_vs_plain_gpu"] = value / pg["value"]
-        cb = cpu_baseline(wl, args.cpu_nodes)
-        progress(f"cpu baseline: {cb}")
+        cpu_nodes = args.cpu_nodes or (512 if is_quad else 4096)
+        cb = cpu_baseline(wl, cpu_nodes)
         line["cpu_baseline"] = cb
         line["speedup_vs_cpu"] = value / cb["value"]
     ex.close()

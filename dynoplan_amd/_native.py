@@ -29,6 +29,7 @@ SYS_QUADROTOR = 2
 SYS_CAR_TRAILER = 3
 
 V_FREE, V_BOUNDS, V_COLLIDE, V_SURROGATE_REJECT = 0, 1, 2, 3
+BATCH_GRAPH, BATCH_STREAMS = 0, 1
 R_NONE, R_NOVEL, R_DUP_VISITED, R_DUP_BATCH, R_CAPPED = 0, 1, 2, 3, 4
 
 
@@ -140,6 +141,11 @@ SIGNATURES = {
     "dp_fetch_pairs": (C.c_int, [C.c_void_p, C.POINTER(PairsOut)]),
     "dp_stream": (C.c_void_p, [C.c_void_p]),
     "dp_last_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "dp_batch_create": (C.c_void_p, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32]),
+    "dp_batch_destroy": (None, [C.c_void_p]),
+    "dp_batch_expand": (C.c_int, [C.c_void_p, C.POINTER(ExpandParams), C.c_void_p]),
+    "dp_batch_latency": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "dp_batch_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "dp_debug_fp32_positions": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int64]),
     "dp_load_primitives": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_uint64,
                                      C.POINTER(PrimSet)]),

@@ -37,6 +37,7 @@ void dp_set_error(const std::string& msg) { g_err = msg; }
 
 template <typename T>
 static int dev_alloc(T** p, int64_t n) {
+  dp_bump_epoch();
   if (*p) (void)hipFree(*p);
   *p = nullptr;
   if (n <= 0) n = 1;
@@ -379,6 +380,7 @@ int dp_upload_primitives(dp_ctx* c, const dp_prims* pr, int32_t max_motions) {
           reach = std::max(reach, std::fabs(st[(int64_t)s * nx + k] - st[(int64_t)off[j] * nx + k]));
     c->prim_reach = reach;
   }
+  dp_bump_epoch();
   c->P = P;
   c->S = S;
   c->maxlen = maxlen;
@@ -421,6 +423,7 @@ int dp_upload_world(dp_ctx* c, const dp_world* w) {
   }
   c->have_world = true;
   c->grid_dirty = true;
+  dp_bump_epoch();
   return upload_sys(c);
 }
 
@@ -429,6 +432,7 @@ static int ensure_visited_cap(dp_ctx* c, int64_t need) {
   int64_t cap = std::max<int64_t>(need, std::max<int64_t>(1024, c->nv_cap * 2));
   const int nx = c->desc.nx;
   double* nv_ = nullptr;
+  dp_bump_epoch();
   HIP_CHECK(hipMalloc(&nv_, sizeof(double) * nx * cap));
   if (c->d_vis && c->nv > 0) {
     HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -450,6 +454,7 @@ int dp_visited_reset(dp_ctx* c) {
   ARG_CHECK(c, "null context");
   c->nv = 0;
   c->grid_dirty = true;
+  dp_bump_epoch();
   return DP_OK;
 }
 
@@ -471,6 +476,7 @@ int dp_visited_append(dp_ctx* c, const double* states, int64_t n) {
   }
   c->nv += n;
   c->grid_dirty = true;
+  dp_bump_epoch();
   return DP_OK;
 }
 
@@ -526,9 +532,54 @@ int dp_upload_nodes(dp_ctx* c, const double* nodes, const double* g_parent, int3
   HIP_CHECK(hipDeviceSynchronize());  // kernels of a previous batch may run on a caller stream
   HIP_CHECK(hipMemcpy(c->d_nodes, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(c->d_gpar, g.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+  if (c->N != N) dp_bump_epoch();
   c->N = N;
   return DP_OK;
 }
+
+}  // extern "C"
+
+// check_goal state indices per primitive (dbastar.cpp:499-500, float arithmetic)
+int dp_prepare_goal_index(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
+  if (c->goal_num == p->num_check_goal) return DP_OK;
+  std::vector<int32_t> gi((size_t)8 * c->P, 0);
+  for (int32_t j = 0; j < c->P; ++j) {
+    const int32_t size = c->h_off[j + 1] - c->h_off[j];
+    for (int nn = 0; nn < p->num_check_goal; ++nn) {
+      float r = (float)(nn + 1) / (float)(p->num_check_goal + 1);
+      gi[(size_t)nn * c->P + j] = (int32_t)(r * (float)size);
+    }
+  }
+  HIP_CHECK(hipStreamSynchronize(s));
+  HIP_CHECK(hipMemcpy(c->d_goalidx, gi.data(), sizeof(int32_t) * gi.size(), hipMemcpyHostToDevice));
+  c->goal_num = p->num_check_goal;
+  return DP_OK;
+}
+
+// Pair buffers.  When every (node, primitive) pair fits the reserved
+// capacity nothing is synchronised; otherwise the applicable pairs are counted
+// exactly first (one small device-to-host read) and the buffers grow to that
+// count plus 1/8 headroom.  Small batches reserve N * P outright.
+int dp_size_pairs(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
+  const int64_t worst = (int64_t)c->N * c->P;
+  const int64_t kOutright = 1ll << 22;
+  int rc = dp_reserve(c, c->N, std::min(worst, kOutright));
+  if (rc) return rc;
+  if (worst <= c->pair_cap) return DP_OK;
+  if ((rc = dp_launch_count(c, p, s))) return rc;
+  long long total = 0;
+  HIP_CHECK(hipMemcpyAsync(c->h_pinned, c->d_counters + DPC_PAIRS, sizeof(long long), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  std::memcpy(&total, c->h_pinned, sizeof(long long));
+  if (total > c->pair_cap) {
+    ARG_CHECK(total < (long long)INT32_MAX / 2, "more than 2^30 applicable pairs in one batch");
+    rc = dp_reserve(c, c->N, std::min<int64_t>(worst, total + total / 8 + 1024));
+    if (rc) return rc;
+  }
+  return DP_OK;
+}
+
+extern "C" {
 
 int dp_expand(dp_ctx* c, const dp_expand_params* p, void* stream) {
   ARG_CHECK(c && p, "null argument");
@@ -541,29 +592,10 @@ int dp_expand(dp_ctx* c, const dp_expand_params* p, void* stream) {
   }
   HIP_CHECK(hipSetDevice(c->device));
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-  // check_goal indices per primitive (dbastar.cpp:499-500, float arithmetic)
-  if (c->goal_num != p->num_check_goal) {
-    std::vector<int32_t> gi((size_t)8 * c->P, 0);
-    for (int32_t j = 0; j < c->P; ++j) {
-      const int32_t size = c->h_off[j + 1] - c->h_off[j];
-      for (int nn = 0; nn < p->num_check_goal; ++nn) {
-        float r = (float)(nn + 1) / (float)(p->num_check_goal + 1);
-        gi[(size_t)nn * c->P + j] = (int32_t)(r * (float)size);
-      }
-    }
-    HIP_CHECK(hipStreamSynchronize(s));
-    HIP_CHECK(hipMemcpy(c->d_goalidx, gi.data(), sizeof(int32_t) * gi.size(), hipMemcpyHostToDevice));
-    c->goal_num = p->num_check_goal;
-  }
-  // Pair buffers: every (node, primitive) pair may apply.
-  const int64_t worst = (int64_t)c->N * c->P;
-  const int64_t kAutoCap = 1ll << 26;
-  int rc = dp_reserve(c, c->N, std::min(worst, kAutoCap));
+  int rc = dp_prepare_goal_index(c, p, s);
   if (rc) return rc;
-  if (worst > c->pair_cap) {
-    dp_set_error("N * P exceeds the pair capacity; call dp_reserve with a larger max_pairs");
-    return DP_ERR_CAPACITY;
-  }
+  rc = dp_size_pairs(c, p, s);
+  if (rc) return rc;
   rc = dp_launch_expand(c, p, s);
   c->last_pairs = -1;
   return rc;
@@ -760,17 +792,18 @@ int dp_debug_fp32_positions(dp_ctx* c, float* out, int64_t n) {
   return DP_OK;
 }
 
-int dp_last_timing(dp_ctx* c, float* ms4) {
-  ARG_CHECK(c && ms4, "null argument");
+int dp_last_timing(dp_ctx* c, float* ms5) {
+  ARG_CHECK(c && ms5, "null argument");
   if (!c->timing_valid) {
     dp_set_error("no expansion has been timed");
     return DP_ERR_STATE;
   }
   HIP_CHECK(hipEventSynchronize(c->ev[3]));
-  HIP_CHECK(hipEventElapsedTime(&ms4[0], c->ev[0], c->ev[1]));
-  HIP_CHECK(hipEventElapsedTime(&ms4[1], c->ev[1], c->ev[2]));
-  HIP_CHECK(hipEventElapsedTime(&ms4[2], c->ev[2], c->ev[3]));
-  HIP_CHECK(hipEventElapsedTime(&ms4[3], c->ev[0], c->ev[3]));
+  HIP_CHECK(hipEventElapsedTime(&ms5[0], c->ev[0], c->ev[4]));
+  HIP_CHECK(hipEventElapsedTime(&ms5[1], c->ev[1], c->ev[2]));
+  HIP_CHECK(hipEventElapsedTime(&ms5[2], c->ev[2], c->ev[3]));
+  HIP_CHECK(hipEventElapsedTime(&ms5[3], c->ev[0], c->ev[3]));
+  HIP_CHECK(hipEventElapsedTime(&ms5[4], c->ev[4], c->ev[1]));
   return DP_OK;
 }
 

@@ -203,6 +203,7 @@ struct dp_ctx {
   // timing
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   bool timing_valid = false;
+  bool no_events = false;           // set while a multi-instance graph is captured
   int64_t last_pairs = -1;
 };
 
@@ -260,3 +261,11 @@ void dp_mlp_free(dp_ctx* c);
 
 // ---- k_expand.hip -------------------------------------------------------
 int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s);
+int dp_launch_count(dp_ctx* c, const dp_expand_params* p, hipStream_t s);
+// Bumped by every device (re)allocation and every change of state a captured
+// multi-instance graph bakes in; a graph captured at an older epoch is
+// captured again before it is replayed.
+void dp_bump_epoch();
+uint64_t dp_epoch();
+int dp_prepare_goal_index(dp_ctx* c, const dp_expand_params* p, hipStream_t s);
+int dp_size_pairs(dp_ctx* c, const dp_expand_params* p, hipStream_t s);

@@ -335,6 +335,27 @@ static int dispatch_nx(int nx, F&& f) {
   return DP_ERR_ARG;
 }
 
+// Applicability count only: the exact number of pairs of the uploaded nodes
+// lands in counters[DPC_PAIRS] (buffers are sized from it).
+int dp_launch_count(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
+  const double r_app = (double)(p->alpha * p->delta);  // dbastar.cpp:585-586 (float product)
+  const int32_t N = c->N;
+  return dispatch_nx(c->desc.nx, [&](auto NXc) -> int {
+    constexpr int NX = decltype(NXc)::value;
+    k_app<NX, false><<<N, 64, 0, s>>>(c->d_sys, c->d_nodes, c->N_cap, N, c->d_start, c->P, r_app, c->d_app,
+                                     nullptr, nullptr, 0);
+    dp_scan_small(c->d_app, N, c->d_total, c->pair_cap, c->d_counters, DPC_PAIRS, s);
+    DP_HIP_RET(hipGetLastError());
+    return DP_OK;
+  });
+}
+
+// Stage events are skipped while a multi-instance graph is being captured.
+#define DP_EV(ev_, s_) \
+  do { \
+    if (!c->no_events) DP_HIP_RET(hipEventRecord(ev_, s_)); \
+  } while (0)
+
 int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   const int nx = c->desc.nx;
   const int32_t N = c->N;
@@ -347,7 +368,7 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
     if (rc) return rc;
   }
 
-  DP_HIP_RET(hipEventRecord(c->ev[0], s));
+  DP_EV(c->ev[0], s);
   dp_zero_counters(c->d_counters, s);
   int rc = dispatch_nx(nx, [&](auto NXc) -> int {
     constexpr int NX = decltype(NXc)::value;
@@ -359,6 +380,7 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
     return DP_OK;
   });
   if (rc) return rc;
+  DP_EV(c->ev[4], s);
   if (p->use_surrogate) {
     if (!c->mlp.w || (p->use_surrogate == 1 && !c->mlp.ready)) {
       dp_set_error("use_surrogate needs a trained and calibrated surrogate");
@@ -371,7 +393,7 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
     rc = dp_launch_surrogate(c, p->use_surrogate == 1, s);
     if (rc) return rc;
   }
-  DP_HIP_RET(hipEventRecord(c->ev[1], s));
+  DP_EV(c->ev[1], s);
 
   dp_check_args a{};
   a.sys = c->d_sys;
@@ -417,11 +439,11 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   if (c->desc.system == DP_SYS_QUADROTOR) {
     rc = dp_launch_quad(c, p, goal_bound, r_dup, s);
     if (rc) return rc;
-    DP_HIP_RET(hipEventRecord(c->ev[2], s));
+    DP_EV(c->ev[2], s);
     rc = dp_launch_resolve(c, p, r_dup, s);
     if (rc) return rc;
-    DP_HIP_RET(hipEventRecord(c->ev[3], s));
-    c->timing_valid = true;
+    DP_EV(c->ev[3], s);
+    c->timing_valid = !c->no_events;
     return DP_OK;
   }
   const int lpp = c->maxlen <= 16 ? 16 : (c->maxlen <= 32 ? 32 : 64);
@@ -443,11 +465,11 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   });
   if (rc) return rc;
   DP_HIP_RET(hipGetLastError());
-  DP_HIP_RET(hipEventRecord(c->ev[2], s));
+  DP_EV(c->ev[2], s);
 
   rc = dp_launch_resolve(c, p, r_dup, s);
   if (rc) return rc;
-  DP_HIP_RET(hipEventRecord(c->ev[3], s));
-  c->timing_valid = true;
+  DP_EV(c->ev[3], s);
+  c->timing_valid = !c->no_events;
   return DP_OK;
 }

@@ -466,6 +466,7 @@ int dp_launch_visited_rebuild(dp_ctx* c, double radius, hipStream_t s) {
   }
   DP_HIP_RET(hipGetLastError());
   c->grid_dirty = false;
+  dp_bump_epoch();
   return DP_OK;
 }
 
@@ -535,5 +536,6 @@ int dp_launch_visited_append_novel(dp_ctx* c, int64_t novel_total, hipStream_t s
   DP_HIP_RET(hipGetLastError());
   c->nv += novel_total;
   c->grid_dirty = true;
+  dp_bump_epoch();
   return DP_OK;
 }

@@ -368,6 +368,7 @@ static void gemm(hipStream_t s, const uint16_t* A, int64_t sam, int64_t sak, con
 
 static int ensure_u16(uint16_t** p, int64_t* cap, int64_t need) {
   if (*cap >= need && *p) return DP_OK;
+  dp_bump_epoch();
   DP_HIP_RET(hipDeviceSynchronize());
   if (*p) (void)hipFree(*p);
   *p = nullptr;
@@ -378,6 +379,7 @@ static int ensure_u16(uint16_t** p, int64_t* cap, int64_t need) {
 
 static int ensure_f32(float** p, int64_t* cap, int64_t need) {
   if (*cap >= need && *p) return DP_OK;
+  dp_bump_epoch();
   DP_HIP_RET(hipDeviceSynchronize());
   if (*p) (void)hipFree(*p);
   *p = nullptr;
@@ -433,6 +435,7 @@ static int refresh_bf16(dp_ctx* c, hipStream_t s) {
   k_to_bf16<<<dp_grid_blocks(L.total, 256, 2048), 256, 0, s>>>(m.w, m.wb, L.total);
   k_to_bf16<<<dp_grid_blocks((int64_t)m.P * 64, 256, 2048), 256, 0, s>>>(m.emb, m.embb, (int64_t)m.P * 64);
   m.cache_valid = false;
+  dp_bump_epoch();
   DP_HIP_RET(hipGetLastError());
   return DP_OK;
 }
@@ -634,7 +637,10 @@ int dp_surrogate_train(dp_ctx* c, const double* nodes, int32_t N, const dp_expan
   k_patch<<<c->N, 256, 0, s>>>(c->d_nodes, c->N_cap, c->N, c->d_obs, c->nb, 0.1, m.patch);
   if ((rc = ensure_u16(&m.xb, &m.xb_cap, (int64_t)batch * 1088))) return rc;
   if ((rc = dp_ensure_i32(&m.sid, &m.sid_cap, batch))) return rc;
-  if (!m.loss) DP_HIP_RET(hipMalloc(&m.loss, sizeof(float) * 2));
+  if (!m.loss) {
+    dp_bump_epoch();
+    DP_HIP_RET(hipMalloc(&m.loss, sizeof(float) * 2));
+  }
   const int64_t steps_per_epoch = (npairs + batch - 1) / batch;
   for (int32_t e = 0; e < epochs; ++e) {
     DP_HIP_RET(hipMemsetAsync(m.loss + 1, 0, sizeof(float), s));
@@ -707,6 +713,7 @@ int dp_surrogate_calibrate(dp_ctx* c, const double* nodes, int32_t N, const dp_e
   }
   m.threshold = tau;
   m.ready = true;
+  dp_bump_epoch();
   if (out) {
     out->threshold = tau;
     out->pairs = np;
@@ -726,6 +733,7 @@ int dp_surrogate_set_threshold(dp_ctx* c, float tau) {
   }
   c->mlp.threshold = tau;
   c->mlp.ready = true;
+  dp_bump_epoch();
   return DP_OK;
 }
 

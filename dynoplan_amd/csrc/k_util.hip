@@ -147,6 +147,7 @@ int dp_scan_large(dp_ctx* c, int32_t* data, const int32_t* n_dev, int64_t n_max,
 // device is drained first (growth happens on the first batches only).
 int dp_ensure_i32(int32_t** p, int64_t* cap, int64_t need) {
   if (*cap >= need && *p) return DP_OK;
+  dp_bump_epoch();
   DP_HIP_RET(hipDeviceSynchronize());
   if (*p) (void)hipFree(*p);
   *p = nullptr;
@@ -158,6 +159,7 @@ int dp_ensure_i32(int32_t** p, int64_t* cap, int64_t need) {
 
 int dp_ensure_f64(double** p, int64_t* cap, int64_t need) {
   if (*cap >= need && *p) return DP_OK;
+  dp_bump_epoch();
   DP_HIP_RET(hipDeviceSynchronize());
   if (*p) (void)hipFree(*p);
   *p = nullptr;

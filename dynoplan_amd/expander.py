@@ -116,9 +116,9 @@ class GpuExpander:
         return c.as_dict()
 
     def timing(self):
-        ms = (C.c_float * 4)()
+        ms = (C.c_float * 5)()
         N.check(self.lib.dp_last_timing(self.ctx, ms))
-        return {"applicability": ms[0], "check": ms[1], "resolve": ms[2], "total": ms[3]}
+        return {"applicability": ms[0], "check": ms[1], "resolve": ms[2], "total": ms[3], "surrogate": ms[4]}
 
     def stream(self):
         return self.lib.dp_stream(self.ctx)
@@ -157,6 +157,46 @@ class GpuExpander:
         po = pairs_out(out, n)
         N.check(self.lib.dp_fetch_pairs(self.ctx, C.byref(po)))
         return out
+
+
+class GpuBatch:
+    """Several GpuExpander contexts (independent planning instances) expanded
+    together: one hipGraph launch (mode N.BATCH_GRAPH) or one stream per
+    instance (N.BATCH_STREAMS)."""
+
+    def __init__(self, expanders, mode=N.BATCH_GRAPH):
+        self.lib = N.load()
+        self.expanders = list(expanders)
+        arr = (C.c_void_p * len(self.expanders))(*[e.ctx.value for e in self.expanders])
+        h = self.lib.dp_batch_create(arr, len(self.expanders), mode)
+        if not h:
+            raise N.DynoplanError(N.DP_ERR_ARG, self.lib.dp_last_error().decode())
+        self.h = C.c_void_p(h)
+
+    def expand_async(self, params, stream=None):
+        arr = (N.ExpandParams * len(self.expanders))(*params)
+        N.check(self.lib.dp_batch_expand(self.h, arr, C.c_void_p(stream) if stream else None))
+
+    def latency_ms(self):
+        ms = np.zeros(len(self.expanders), np.float32)
+        N.check(self.lib.dp_batch_latency(self.h, ms.ctypes.data_as(C.POINTER(C.c_float))))
+        return ms
+
+    def stats(self):
+        a, b = C.c_int64(), C.c_int64()
+        N.check(self.lib.dp_batch_stats(self.h, C.byref(a), C.byref(b)))
+        return {"captures": a.value, "launches": b.value}
+
+    def close(self):
+        if self.h:
+            self.lib.dp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def pairs_out(out, n):

@@ -417,3 +417,51 @@ def quadrotor(seed=3, P=10000, lmin=10, lmax=50, n_boxes=100, n_spheres=50, N_no
     return Workload(name=f"quadrotor_v0 N={N_nodes} P={P} O={n_boxes}+{n_spheres} V={n_visited}", desc=desc,
                     offsets=offsets, states=states, actions=actions, lo=lo, hi=hi, boxes=boxes, spheres=spheres,
                     nodes=nodes, g_parent=rng.uniform(0, 10, N_nodes), visited=visited, params=params, seed=seed)
+
+
+# ---- many instances --------------------------------------------------------------
+def _prefix(offsets, states, actions, P):
+    """The first P primitives of a library (the reference's max_motions)."""
+    off = np.ascontiguousarray(offsets[: P + 1])
+    return off, np.ascontiguousarray(states[: off[-1]]), np.ascontiguousarray(actions[: off[-1] - P])
+
+
+def multi_instance(seed=6, n_instances=128, p_range=(1000, 10000), o_range=(10, 300), n_range=(64, 4096),
+                   n_visited=10_000, systems=(N.SYS_UNICYCLE1, N.SYS_UNICYCLE2, N.SYS_QUADROTOR)):
+    """Independent planning problems for one batched launch: system drawn
+    uniformly from `systems`, obstacles O ~ U{o_range}, primitives P ~
+    U{p_range} (the first P of one 10k library per system), frontier N ~
+    U{n_range}.  Planar maps are 20x20 m with O boxes; quadrotor worlds are
+    10x10x5 m with 2/3 of O boxes and 1/3 spheres."""
+    rng = np.random.default_rng(seed)
+    pmax = p_range[1]
+    libs = {}
+    for s in systems:
+        lr = np.random.default_rng(seed * 1000 + s)
+        if s == N.SYS_UNICYCLE1:
+            libs[s] = unicycle1_primitives(lr, pmax, 5, 30)
+        elif s == N.SYS_UNICYCLE2:
+            libs[s] = unicycle2_primitives(lr, pmax, 5, 20)
+        elif s == N.SYS_QUADROTOR:
+            libs[s] = quadrotor_primitives(lr, pmax, 10, 50)
+        else:
+            libs[s] = car_trailer_primitives(lr, pmax, 10, 40)
+    out = []
+    for i in range(n_instances):
+        s = systems[int(rng.integers(len(systems)))]
+        P = int(rng.integers(p_range[0], p_range[1] + 1))
+        O = int(rng.integers(o_range[0], o_range[1] + 1))
+        Nn = int(rng.integers(n_range[0], n_range[1] + 1))
+        sub_seed = int(rng.integers(1 << 31))
+        off, st, ac = _prefix(*libs[s], P)
+        if s == N.SYS_QUADROTOR:
+            nb = (2 * O + 2) // 3
+            w = quadrotor(seed=sub_seed, P=1, n_boxes=nb, n_spheres=O - nb, N_nodes=Nn, n_visited=n_visited)
+        else:
+            prim_fn = {N.SYS_UNICYCLE1: unicycle1_primitives, N.SYS_UNICYCLE2: unicycle2_primitives,
+                       N.SYS_CAR_TRAILER: car_trailer_primitives}[s]
+            w = _planar_workload("", s, sub_seed, 1, 5, 6, 20.0, O, Nn, n_visited, prim_fn=prim_fn)
+        w["offsets"], w["states"], w["actions"] = off, st, ac
+        w["name"] = f"#{i} system {s} N={Nn} P={P} O={O} V={n_visited}"
+        out.append(w)
+    return out

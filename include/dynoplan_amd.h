@@ -214,9 +214,9 @@ int dp_fetch_pairs(dp_ctx* ctx, dp_pairs_out* out);
 void* dp_stream(dp_ctx* ctx);
 
 /* Device time (ms) of the kernels of the last dp_expand, split by stage
- * (HIP events on the expansion stream). stages: 0 applicability, 1 check,
- * 2 duplicate resolution, 3 total. */
-int dp_last_timing(dp_ctx* ctx, float* ms4);
+ * (HIP events on the expansion stream), ms5[5]: 0 applicability, 1 check,
+ * 2 duplicate resolution, 3 total, 4 learned pre-filter (0 without it). */
+int dp_last_timing(dp_ctx* ctx, float* ms5);
 
 /* ---- primitive files (load_motion_primitives_new, src/ompl/robots.cpp:3061-3174) ---- */
 enum { DP_FMT_AUTO = 0, DP_FMT_YAML = 1, DP_FMT_JSON = 2, DP_FMT_MSGPACK = 3, DP_FMT_BOOST = 4 };
@@ -286,6 +286,30 @@ int dp_debug_gemm_bf16(dp_ctx* ctx, const float* A, const float* B, int32_t M, i
  * dp_expand (other entries unspecified).  No counterpart in the reference;
  * used to report the FP32-vs-FP64 rollout error per system. */
 int dp_debug_fp32_positions(dp_ctx* ctx, float* out, int64_t n);
+
+/* ---- many planning instances in one launch ---------------------------------
+ * No counterpart in the reference, which plans one instance per process.  A
+ * batch groups n contexts (independent instances, any mix of systems, one
+ * device); dp_batch_expand expands the uploaded nodes of all of them.
+ * DP_BATCH_GRAPH: one hipGraph launch whose branches are the instances'
+ * pipelines, captured on the first call and again whenever a context's node
+ * count, primitives, world, visited set, buffer capacities or surrogate, or
+ * the params, changed since (replayed otherwise; a capture call also runs the
+ * batch once eagerly).  DP_BATCH_STREAMS: the same pipelines launched
+ * directly, one stream per instance.  Results are read per context
+ * (dp_get_counts, dp_fetch_pairs); per-stage dp_last_timing is not kept for
+ * graph launches. */
+enum { DP_BATCH_GRAPH = 0, DP_BATCH_STREAMS = 1 };
+typedef struct dp_batch dp_batch;
+dp_batch* dp_batch_create(dp_ctx* const* ctxs, int32_t n, int32_t mode);
+void dp_batch_destroy(dp_batch* b);
+/* params[n], one per context; stream NULL = the batch's own stream. */
+int dp_batch_expand(dp_batch* b, const dp_expand_params* params, void* stream);
+/* Completion time of each instance of the last dp_batch_expand, ms[n] after
+ * the batch started on the GPU (constant-rate GPU clock).  Waits for it. */
+int dp_batch_latency(dp_batch* b, float* ms);
+/* Graph captures and launches so far. */
+int dp_batch_stats(const dp_batch* b, int64_t* captures, int64_t* launches);
 
 #ifdef __cplusplus
 }

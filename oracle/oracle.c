@@ -673,3 +673,197 @@ int64_t oracle_expand(const dp_system_desc* sys, const dp_prims* prims, int32_t 
   nn_free(&acc);
   return overflow ? DP_ERR_CAPACITY : np;
 }
+
+/* ------------------------------------------------------------------ */
+/* the same batch on several host threads (CPU baseline timing)          */
+/* ------------------------------------------------------------------ */
+
+/* Identical results to oracle_expand.  Phase 1 runs the per-pair work of
+ * every node in parallel (applicability, translation, bounds, collision,
+ * check_goal, g/h and the test against the visited set, which is read-only);
+ * phase 2 replays the sequential part of dbastar.cpp:850-918 (branching cap,
+ * in-batch duplicates, insertion) in canonical order on one thread. */
+int64_t oracle_expand_mt(const dp_system_desc* sys, const dp_prims* prims, int32_t max_motions,
+                         const dp_world* world, const double* visited, int64_t nv,
+                         const double* nodes, const double* g_parent, int32_t N,
+                         const dp_expand_params* par, int32_t nn_mode, int32_t col_mode,
+                         dp_pairs_out* out, dp_counts* counts, int32_t nthreads) {
+  if (!sys || !prims || !world || !nodes || !par || !counts || N < 0) return DP_ERR_ARG;
+  if (!(par->alpha > 0.0f && par->alpha < 1.0f)) return DP_ERR_ARG;
+  if (sys->dist_w[0] <= 0.0) return DP_ERR_ARG;
+  series_init();
+  const int nx = sys->nx;
+  int32_t P = prims->P;
+  if (max_motions > 0 && max_motions < P) P = max_motions;
+  if (nthreads < 1) nthreads = 1;
+  const double r_app = (double)(par->alpha * par->delta);
+  const double r_dup = (1. - (double)par->alpha) * (double)par->delta;
+  const double goal_bound = par->delta_factor_goal * (double)par->delta;
+  int32_t maxlen = 0;
+  for (int32_t j = 0; j < P; ++j) {
+    int32_t l = prims->offsets[j + 1] - prims->offsets[j];
+    if (l > maxlen) maxlen = l;
+  }
+  const double cell = (r_dup / sys->dist_w[0]) * (1.0 + 1e-9);
+  nnset vis, acc;
+  nn_init(&vis, nx, sys->pos_dim, nn_mode == ORACLE_NN_GRID, cell, nv + 16);
+  for (int64_t i = 0; i < nv; ++i) nn_add(&vis, visited + i * nx);
+  nn_init(&acc, nx, sys->pos_dim, nn_mode == ORACLE_NN_GRID, cell, 1024);
+
+  /* applicable primitives per node */
+  int64_t* first = (int64_t*)calloc((size_t)N + 1, sizeof(int64_t));
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 16)
+  for (int32_t i = 0; i < N; ++i) {
+    double canon[DP_MAX_NX];
+    canonical_state(sys, nodes + (int64_t)i * nx, canon);
+    int64_t n = 0;
+    for (int32_t j = 0; j < P; ++j)
+      if (oracle_distance(sys, canon, prims->states + (int64_t)prims->offsets[j] * nx) <= r_app) ++n;
+    first[i + 1] = n;
+  }
+  for (int32_t i = 0; i < N; ++i) first[i + 1] += first[i];
+  const int64_t np = first[N];
+  int32_t* pj = (int32_t*)malloc(sizeof(int32_t) * (size_t)(np + 1));
+  int8_t* vd = (int8_t*)malloc((size_t)(np + 1));
+  int8_t* dv = (int8_t*)malloc((size_t)(np + 1));
+  int32_t* fk = (int32_t*)malloc(sizeof(int32_t) * (size_t)(np + 1));
+  int32_t* ch = (int32_t*)malloc(sizeof(int32_t) * (size_t)(np + 1));
+  double* xf = (double*)malloc(sizeof(double) * (size_t)(np + 1) * nx);
+  double* gg = (double*)malloc(sizeof(double) * (size_t)(np + 1));
+  double* hh = (double*)malloc(sizeof(double) * (size_t)(np + 1));
+
+#pragma omp parallel num_threads(nthreads)
+  {
+    double* traj = (double*)malloc(sizeof(double) * (size_t)(maxlen > 0 ? maxlen : 1) * nx);
+    int32_t* cb = (int32_t*)malloc(sizeof(int32_t) * (world->n_boxes + 1));
+    int32_t* cs = (int32_t*)malloc(sizeof(int32_t) * (world->n_spheres + 1));
+#pragma omp for schedule(dynamic, 4)
+    for (int32_t i = 0; i < N; ++i) {
+      const double* node = nodes + (int64_t)i * nx;
+      double canon[DP_MAX_NX];
+      canonical_state(sys, node, canon);
+      int64_t q = first[i];
+      for (int32_t j = 0; j < P; ++j) {
+        if (!(oracle_distance(sys, canon, prims->states + (int64_t)prims->offsets[j] * nx) <= r_app)) continue;
+        int32_t L1 = oracle_transform_primitive(sys, prims, j, node, traj);
+        int8_t verdict = DP_V_FREE;
+        int32_t first_k = -1;
+        for (int32_t k = 0; k < L1; ++k)
+          if (!oracle_state_valid(sys, world, traj + k * nx)) {
+            first_k = k;
+            verdict = DP_V_BOUNDS;
+            break;
+          }
+        if (verdict == DP_V_FREE) {
+          int32_t ncb = world->n_boxes, ncs = world->n_spheres;
+          const int32_t* pcb = NULL;
+          const int32_t* pcs = NULL;
+          if (col_mode == ORACLE_COL_SWEPT) {
+            swept_candidates(sys, world, traj, L1, cb, &ncb, cs, &ncs);
+            pcb = cb;
+            pcs = cs;
+          }
+          for (int32_t k = 0; k < L1; ++k)
+            if (collides_subset(sys, world, traj + k * nx, pcb, ncb, pcs, ncs)) {
+              first_k = k;
+              verdict = DP_V_COLLIDE;
+              break;
+            }
+        }
+        int32_t chosen = -1;
+        const double* x = traj + (int64_t)(L1 - 1) * nx;
+        double gq = NAN, hq = NAN;
+        int8_t dupv = 0;
+        if (verdict == DP_V_FREE) {
+          if (par->check_intermediate_goal)
+            for (int32_t nn = 0; nn < par->num_check_goal; ++nn) {
+              int32_t idx = oracle_goal_index(nn, par->num_check_goal, L1);
+              if (oracle_distance(sys, traj + (int64_t)idx * nx, par->goal) <= goal_bound) {
+                x = traj + (int64_t)idx * nx;
+                chosen = idx;
+                break;
+              }
+            }
+          hq = oracle_lower_bound_time(sys, x, par->goal);
+          double cost_motion = chosen != -1 ? chosen * sys->dt : (L1 - 1) * sys->dt;
+          gq = g_parent[i] + cost_motion + par->cost_delta_factor * oracle_lower_bound_time(sys, node, traj);
+          dupv = (int8_t)nn_any_within(sys, &vis, x, r_dup);
+        }
+        pj[q] = j;
+        vd[q] = verdict;
+        fk[q] = first_k;
+        ch[q] = chosen;
+        dv[q] = dupv;
+        gg[q] = gq;
+        hh[q] = hq;
+        for (int c = 0; c < nx; ++c) xf[q * nx + c] = verdict == DP_V_FREE ? x[c] : NAN;
+        ++q;
+      }
+    }
+    free(traj);
+    free(cb);
+    free(cs);
+  }
+
+  memset(counts, 0, sizeof(*counts));
+  counts->nodes = N;
+  counts->pairs = np;
+  int overflow = 0;
+  for (int32_t i = 0; i < N; ++i) {
+    int32_t novel_in_node = 0;
+    for (int64_t q = first[i]; q < first[i + 1]; ++q) {
+      int8_t res = DP_R_NONE;
+      if (vd[q] == DP_V_BOUNDS) counts->bounds++;
+      if (vd[q] != DP_V_BOUNDS) counts->checked++;
+      if (vd[q] == DP_V_COLLIDE) counts->collide++;
+      if (vd[q] == DP_V_FREE) {
+        const double* x = xf + q * nx;
+        counts->free_++;
+        if (ch[q] != -1) counts->goal_hits++;
+        if (par->limit_branching_factor > 0 && novel_in_node >= par->limit_branching_factor) {
+          res = DP_R_CAPPED;
+          counts->capped++;
+        } else {
+          int db = dv[q] ? 0 : nn_any_within(sys, &acc, x, r_dup);
+          if ((!dv[q] && !db) || ch[q] != -1) {
+            res = DP_R_NOVEL;
+            counts->novel++;
+            novel_in_node++;
+            nn_add(&acc, x);
+          } else if (dv[q]) {
+            res = DP_R_DUP_VISITED;
+            counts->dup_visited++;
+          } else {
+            res = DP_R_DUP_BATCH;
+            counts->dup_batch++;
+          }
+        }
+      }
+      if (out && q >= out->capacity) overflow = 1;
+      if (out && q < out->capacity) {
+        if (out->node) out->node[q] = i;
+        if (out->prim) out->prim[q] = pj[q];
+        if (out->verdict) out->verdict[q] = vd[q];
+        if (out->first_k) out->first_k[q] = fk[q];
+        if (out->goal_index) out->goal_index[q] = ch[q];
+        if (out->resolution) out->resolution[q] = res;
+        if (out->final_state)
+          for (int c = 0; c < nx; ++c) out->final_state[q * nx + c] = xf[q * nx + c];
+        if (out->g) out->g[q] = gg[q];
+        if (out->h) out->h[q] = hh[q];
+      }
+    }
+  }
+  free(first);
+  free(pj);
+  free(vd);
+  free(dv);
+  free(fk);
+  free(ch);
+  free(xf);
+  free(gg);
+  free(hh);
+  nn_free(&vis);
+  nn_free(&acc);
+  return overflow ? DP_ERR_CAPACITY : np;
+}

@@ -44,6 +44,14 @@ int64_t oracle_expand(const dp_system_desc* sys, const dp_prims* prims, int32_t 
                       const dp_expand_params* par, int32_t nn_mode, int32_t col_mode,
                       dp_pairs_out* out, dp_counts* counts);
 
+/* The same batch with the per-pair work spread over `nthreads` OpenMP
+ * threads (identical results; used to time the multi-core CPU baseline). */
+int64_t oracle_expand_mt(const dp_system_desc* sys, const dp_prims* prims, int32_t max_motions,
+                         const dp_world* world, const double* visited, int64_t nv,
+                         const double* nodes, const double* g_parent, int32_t N,
+                         const dp_expand_params* par, int32_t nn_mode, int32_t col_mode,
+                         dp_pairs_out* out, dp_counts* counts, int32_t nthreads);
+
 /* Building blocks, exported for the pinning tests. */
 double oracle_distance(const dp_system_desc* sys, const double* a, const double* b);
 double oracle_lower_bound_time(const dp_system_desc* sys, const double* a, const double* b);

@@ -33,6 +33,8 @@ def load():
     lib.oracle_expand.argtypes = [P(N.SystemDesc), P(N.Prims), C.c_int32, P(N.World), P(C.c_double), C.c_int64,
                                   P(C.c_double), P(C.c_double), C.c_int32, P(N.ExpandParams), C.c_int32,
                                   C.c_int32, P(N.PairsOut), P(N.Counts)]
+    lib.oracle_expand_mt.restype = C.c_int64
+    lib.oracle_expand_mt.argtypes = lib.oracle_expand.argtypes + [C.c_int32]
     lib.oracle_distance.restype = C.c_double
     lib.oracle_distance.argtypes = [P(N.SystemDesc), P(C.c_double), P(C.c_double)]
     lib.oracle_lower_bound_time.restype = C.c_double
@@ -88,8 +90,10 @@ def world_struct(lo, hi, boxes, spheres):
 
 
 def expand(wl, nodes=None, g_parent=None, visited=None, params=None, nn_mode=NN_GRID, col_mode=COL_SWEPT,
-           max_motions=0):
-    """Run the oracle on a synth.Workload (optionally overriding parts)."""
+           max_motions=0, threads=0, want_out=True):
+    """Run the oracle on a synth.Workload (optionally overriding parts).
+    threads > 0 runs oracle_expand_mt on that many OpenMP threads; with
+    want_out False only the counts are produced (timing runs)."""
     lib = load()
     nodes = np.ascontiguousarray(wl.nodes if nodes is None else nodes, np.float64)
     g_parent = np.ascontiguousarray(wl.g_parent if g_parent is None else g_parent, np.float64)
@@ -105,7 +109,7 @@ def expand(wl, nodes=None, g_parent=None, visited=None, params=None, nn_mode=NN_
     nx = wl.desc.nx
     # capacity: at most N * P pairs
     P = offsets.size - 1 if max_motions <= 0 else min(max_motions, offsets.size - 1)
-    cap = max(1, nodes.shape[0] * P)
+    cap = max(1, nodes.shape[0] * P) if want_out else 1
     out = {
         "node": np.zeros(cap, np.int32), "prim": np.zeros(cap, np.int32), "verdict": np.zeros(cap, np.int8),
         "first_k": np.zeros(cap, np.int32), "goal_index": np.zeros(cap, np.int32),
@@ -115,9 +119,10 @@ def expand(wl, nodes=None, g_parent=None, visited=None, params=None, nn_mode=NN_
     from dynoplan_amd.expander import pairs_out
     po = pairs_out(out, cap)
     cnt = N.Counts()
-    n = lib.oracle_expand(C.byref(wl.desc), C.byref(pr), max_motions, C.byref(w), dptr(visited), visited.shape[0],
-                          dptr(nodes), dptr(g_parent), nodes.shape[0], C.byref(params), nn_mode, col_mode,
-                          C.byref(po), C.byref(cnt))
+    args = (C.byref(wl.desc), C.byref(pr), max_motions, C.byref(w), dptr(visited), visited.shape[0],
+            dptr(nodes), dptr(g_parent), nodes.shape[0], C.byref(params), nn_mode, col_mode,
+            C.byref(po) if want_out else None, C.byref(cnt))
+    n = lib.oracle_expand_mt(*args, threads) if threads > 0 else lib.oracle_expand(*args)
     if n < 0:
         raise RuntimeError(f"oracle_expand failed: {n}")
     return {k: v[:n] for k, v in out.items()}, cnt.as_dict()

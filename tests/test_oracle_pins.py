@@ -328,6 +328,20 @@ def test_oracle_modes_agree(system):
             assert np.array_equal(ref[k], out[k], equal_nan=ref[k].dtype == np.float64), k
 
 
+@pytest.mark.parametrize("system", [N.SYS_UNICYCLE1, N.SYS_CAR_TRAILER, N.SYS_QUADROTOR])
+def test_oracle_threaded_equals_sequential(system):
+    """The multi-threaded baseline computes exactly the sequential batch."""
+    if system == N.SYS_QUADROTOR:
+        wl = synth.quadrotor(seed=11, P=200, N_nodes=24, n_visited=500)
+    else:
+        wl = small(system, seed=9)
+    ref, cr = O.expand(wl)
+    out, c = O.expand(wl, threads=4)
+    assert c == cr and cr["free_"] > 0
+    for k in ref:
+        assert np.array_equal(ref[k], out[k], equal_nan=ref[k].dtype == np.float64), k
+
+
 def test_oracle_verdicts_consistent_with_state_checks(lib):
     wl = small(N.SYS_CAR_TRAILER, seed=8)
     out, c = O.expand(wl)
