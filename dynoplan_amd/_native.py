@@ -118,6 +118,26 @@ class SurrogateStats(C.Structure):
         return {n: (float(getattr(self, n)) if n == "threshold" else int(getattr(self, n)))
                 for n, _ in self._fields_ if n != "reserved"}
 
+SEARCH_SOLVED, SEARCH_MAX_EXPANDS, SEARCH_EMPTY_QUEUE, SEARCH_MAX_TIME = 0, 1, 2, 3
+
+
+class SearchOptions(C.Structure):
+    _fields_ = [("delta", C.c_float), ("alpha", C.c_float), ("delta_factor_goal", C.c_double),
+                ("cost_delta_factor", C.c_double), ("max_expands", C.c_int64), ("search_timelimit_ms", C.c_double),
+                ("limit_branching_factor", C.c_int32), ("heuristic", C.c_int32), ("batch_nodes", C.c_int32),
+                ("use_surrogate", C.c_int32), ("fp32_rollout", C.c_int32), ("reserved", C.c_int32)]
+
+
+class SearchInfo(C.Structure):
+    _fields_ = [("solved", C.c_int32), ("status", C.c_int32), ("cost", C.c_double), ("distance_to_goal", C.c_double)] + [
+        (n, C.c_int64) for n in ("expands", "all_nodes", "closed", "rewires", "gpu_batches", "pairs", "checked",
+                                 "traj_states", "traj_actions")] + [
+        ("time_search_ms", C.c_double), ("time_gpu_ms", C.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 # name -> (restype, argtypes); every symbol include/dynoplan_amd.h declares.
 SIGNATURES = {
     "dp_last_error": (C.c_char_p, []),
@@ -141,6 +161,11 @@ SIGNATURES = {
     "dp_fetch_pairs": (C.c_int, [C.c_void_p, C.POINTER(PairsOut)]),
     "dp_stream": (C.c_void_p, [C.c_void_p]),
     "dp_last_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "dp_surrogate_get": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "dp_search_options_default": (None, [C.POINTER(SearchOptions)]),
+    "dp_search": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(SearchOptions),
+                            C.POINTER(SearchInfo)]),
+    "dp_search_trajectory": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "dp_batch_create": (C.c_void_p, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32]),
     "dp_batch_destroy": (None, [C.c_void_p]),
     "dp_batch_expand": (C.c_int, [C.c_void_p, C.POINTER(ExpandParams), C.c_void_p]),

@@ -381,6 +381,18 @@ int dp_upload_primitives(dp_ctx* c, const dp_prims* pr, int32_t max_motions) {
     c->prim_reach = reach;
   }
   dp_bump_epoch();
+  c->h_st.assign(st, st + (size_t)S * nx);
+  c->h_nu = pr->actions && pr->nu > 0 ? pr->nu : 0;
+  if (c->h_nu) {
+    // action rows of primitive j start at offsets[j] - j
+    c->h_ac.assign((size_t)(S - P) * c->h_nu, 0.0);
+    for (int32_t j = 0; j < P; ++j)
+      for (int32_t k = 0; k < off[j + 1] - off[j] - 1; ++k)
+        for (int u = 0; u < c->h_nu; ++u)
+          c->h_ac[(size_t)(off[j] - j + k) * c->h_nu + u] = pr->actions[(int64_t)(pr->offsets[j] - j + k) * pr->nu + u];
+  } else {
+    c->h_ac.clear();
+  }
   c->P = P;
   c->S = S;
   c->maxlen = maxlen;
@@ -730,37 +742,14 @@ int dp_visited_radius(dp_ctx* c, const double* queries, int64_t n, double radius
   for (int k = 0; k < nx && c->nv > 0; ++k)
     HIP_CHECK(hipMemcpy(vis.data() + (size_t)k * c->nv, c->d_vis + (size_t)k * c->nv_cap, sizeof(double) * c->nv,
                         hipMemcpyDeviceToHost));
-  // Host evaluation of the same metric for the (rare) rewiring queries.
-  dp_dev_sys& S = c->hsys;
+  // The device metric, evaluated on the host (same expressions, same rounding).
+  const dp_dev_sys& S = c->hsys;
   auto dist = [&](const double* a, int64_t v) {
     double b[DP_MAX_NX];
     for (int k = 0; k < nx; ++k) b[k] = vis[(size_t)k * c->nv + v];
-    auto so2 = [](double x, double y) { double d = std::fabs(x - y); return d > DP_PI ? DP_TWO_PI - d : d; };
-    auto l2 = [](const double* x, const double* y, int m) {
-      double s = 0.0;
-      for (int i = 0; i < m; ++i) { double d = x[i] - y[i]; s = s + d * d; }
-      return std::sqrt(s);
-    };
-    switch (S.system) {
-      case DP_SYS_UNICYCLE1: return S.w[0] * l2(a, b, 2) + S.w[1] * so2(a[2], b[2]);
-      case DP_SYS_UNICYCLE2: {
-        double d = S.w[0] * l2(a, b, 2) + S.w[1] * so2(a[2], b[2]);
-        d = d + S.w[2] * std::fabs(a[3] - b[3]);
-        return d + S.w[3] * std::fabs(a[4] - b[4]);
-      }
-      case DP_SYS_CAR_TRAILER: {
-        double d = S.w[0] * l2(a, b, 2) + S.w[1] * so2(a[2], b[2]);
-        return d + S.w[2] * so2(a[3], b[3]);
-      }
-      default: {
-        double dot = std::fabs(a[3] * b[3] + a[4] * b[4] + a[5] * b[5] + a[6] * b[6]);
-        double ang = std::acos(std::min(1.0, dot));
-        double d = S.w[0] * l2(a, b, 3) + S.w[1] * ang;
-        d = d + S.w[2] * l2(a + 7, b + 7, 3);
-        return d + S.w[3] * l2(a + 10, b + 10, 3);
-      }
-    }
+    return dp_distance(S, a, b);
   };
+
   int64_t w = 0;
   out_offsets[0] = 0;
   for (int64_t q = 0; q < n; ++q) {

@@ -15,6 +15,10 @@
 #define DP_TWO_PI 6.283185307179586
 #define DP_HALF_PI 1.5707963267948966
 
+// Metric and geometry helpers are compiled for the host too: the host search
+// loop evaluates the same expressions, rounding for rounding.
+#define DP_HD __host__ __device__ __forceinline__
+
 #define DP_ASIN_TERMS 30
 #define DP_TRIG_TERMS 10
 
@@ -34,19 +38,19 @@ struct dp_dev_sys {
   float near_eps;
 };
 
-__device__ __forceinline__ double dp_so2(double a, double b) {
+DP_HD double dp_so2(double a, double b) {
   double d = fabs(a - b);
   return d > DP_PI ? DP_TWO_PI - d : d;
 }
 
-__device__ __forceinline__ double dp_l2_2(double ax, double ay, double bx, double by) {
+DP_HD double dp_l2_2(double ax, double ay, double bx, double by) {
   double dx = ax - bx, dy = ay - by;
   double s = dx * dx;
   s = s + dy * dy;
   return sqrt(s);
 }
 
-__device__ __forceinline__ double dp_l2_3(const double* a, const double* b) {
+DP_HD double dp_l2_3(const double* a, const double* b) {
   double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
   double s = d0 * d0;
   s = s + d1 * d1;
@@ -54,7 +58,7 @@ __device__ __forceinline__ double dp_l2_3(const double* a, const double* b) {
   return sqrt(s);
 }
 
-__device__ __forceinline__ double dp_asin_series(const dp_dev_sys& S, double t) {
+DP_HD double dp_asin_series(const dp_dev_sys& S, double t) {
   double z = t * t;
   double p = S.asin_c[DP_ASIN_TERMS - 1];
 #pragma unroll
@@ -63,7 +67,7 @@ __device__ __forceinline__ double dp_asin_series(const dp_dev_sys& S, double t) 
 }
 
 // acos on [0,1] through asin series whose argument stays <= 1/2.
-__device__ __forceinline__ double dp_acos01(const dp_dev_sys& S, double x) {
+DP_HD double dp_acos01(const dp_dev_sys& S, double x) {
   if (x >= 1.0) return 0.0;
   if (x >= 0.5) {
     double t = sqrt((1.0 - x) * 0.5);
@@ -72,7 +76,7 @@ __device__ __forceinline__ double dp_acos01(const dp_dev_sys& S, double x) {
   return DP_HALF_PI - dp_asin_series(S, x);
 }
 
-__device__ __forceinline__ double dp_quat_angle(const dp_dev_sys& S, const double* a, const double* b) {
+DP_HD double dp_quat_angle(const dp_dev_sys& S, const double* a, const double* b) {
   double dot = a[0] * b[0];
   dot = dot + a[1] * b[1];
   dot = dot + a[2] * b[2];
@@ -83,7 +87,7 @@ __device__ __forceinline__ double dp_quat_angle(const dp_dev_sys& S, const doubl
 }
 
 // robot->distance (nigh_custom_spaces.hpp:391-573 key spaces).
-__device__ __forceinline__ double dp_distance(const dp_dev_sys& S, const double* a, const double* b) {
+DP_HD double dp_distance(const dp_dev_sys& S, const double* a, const double* b) {
   switch (S.system) {
     case DP_SYS_UNICYCLE1:
       return S.w[0] * dp_l2_2(a[0], a[1], b[0], b[1]) + S.w[1] * dp_so2(a[2], b[2]);
@@ -113,7 +117,7 @@ __device__ __forceinline__ double dp_distance(const dp_dev_sys& S, const double*
 // proves the full sum is above r.  When no exit applies the full sum is
 // formed in dp_distance's exact operation order, so the decision is always
 // identical to evaluating dp_distance.
-__device__ __forceinline__ bool dp_within(const dp_dev_sys& S, const double* a, const double* b, double r) {
+DP_HD bool dp_within(const dp_dev_sys& S, const double* a, const double* b, double r) {
   if (S.system != DP_SYS_QUADROTOR) return dp_distance(S, a, b) <= r;
   const double A = S.w[0] * dp_l2_3(a, b);
   if (A > r) return false;
@@ -126,10 +130,10 @@ __device__ __forceinline__ bool dp_within(const dp_dev_sys& S, const double* a, 
   return d <= r;
 }
 
-__device__ __forceinline__ double dp_max(double a, double b) { return b > a ? b : a; }
+DP_HD double dp_max(double a, double b) { return b > a ? b : a; }
 
 // robot->lower_bound_time.
-__device__ __forceinline__ double dp_lower_bound_time(const dp_dev_sys& S, const double* a, const double* b) {
+DP_HD double dp_lower_bound_time(const dp_dev_sys& S, const double* a, const double* b) {
   switch (S.system) {
     case DP_SYS_UNICYCLE1:
     case DP_SYS_CAR_TRAILER:
@@ -145,13 +149,13 @@ __device__ __forceinline__ double dp_lower_bound_time(const dp_dev_sys& S, const
   }
 }
 
-__device__ __forceinline__ bool dp_is_offset_dim(const dp_dev_sys& S, int c) {
+DP_HD bool dp_is_offset_dim(const dp_dev_sys& S, int c) {
   if (S.system == DP_SYS_QUADROTOR) return c <= 2 || (c >= 7 && c <= 9);
   return c <= 1;
 }
 
 // Separating-axis test, oriented box vs axis-aligned box; touching collides.
-__device__ __forceinline__ bool dp_obb_aabb(double cx, double cy, double c, double s, double hl, double hw,
+DP_HD bool dp_obb_aabb(double cx, double cy, double c, double s, double hl, double hw,
                                             double ox, double oy, double hx, double hy) {
   double tx = ox - cx;
   double ty = oy - cy;
@@ -166,7 +170,7 @@ __device__ __forceinline__ bool dp_obb_aabb(double cx, double cy, double c, doub
   return true;
 }
 
-__device__ __forceinline__ bool dp_sphere_box(double px, double py, double pz, double r, const double* b) {
+DP_HD bool dp_sphere_box(double px, double py, double pz, double r, const double* b) {
   double dx = fabs(px - b[0]) - b[3];
   double dy = fabs(py - b[1]) - b[4];
   double dz = fabs(pz - b[2]) - b[5];
@@ -178,7 +182,7 @@ __device__ __forceinline__ bool dp_sphere_box(double px, double py, double pz, d
   return d2 <= r * r;
 }
 
-__device__ __forceinline__ bool dp_sphere_sphere(double px, double py, double pz, double r, const double* o) {
+DP_HD bool dp_sphere_sphere(double px, double py, double pz, double r, const double* o) {
   double dx = px - o[0], dy = py - o[1], dz = pz - o[2];
   double d2 = dx * dx + dy * dy;
   d2 = d2 + dz * dz;
@@ -187,5 +191,74 @@ __device__ __forceinline__ bool dp_sphere_sphere(double px, double py, double pz
 }
 
 // Wave-level helpers (wave64).
+// Quadrotor step (explicit Euler for position and velocity, body-frame
+// exp-map quaternion update, X-config torques; u = 1 is per-rotor hover) in
+// FP64, one fixed operation order.
+DP_HD void dp_quad_cos_sinc(const dp_dev_sys& S, double z, double* c, double* sinc) {
+  double pc = S.cos_c[DP_TRIG_TERMS - 1];
+  double ps = S.sinc_c[DP_TRIG_TERMS - 1];
+#pragma unroll
+  for (int n = DP_TRIG_TERMS - 2; n >= 0; --n) {
+    pc = pc * z + S.cos_c[n];
+    ps = ps * z + S.sinc_c[n];
+  }
+  *c = pc;
+  *sinc = ps;
+}
+
+DP_HD void dp_quad_step64(const dp_dev_sys& S, const double* s, const double* u, double* o) {
+  const double m = S.quad[0], g = S.quad[1], arm = S.quad[2], t2t = S.quad[3];
+  const double Jx = S.quad[4], Jy = S.quad[5], Jz = S.quad[6];
+  const double dt = S.dt;
+  const double qx = s[3], qy = s[4], qz = s[5], qw = s[6];
+  const double vx = s[7], vy = s[8], vz = s[9];
+  const double wx = s[10], wy = s[11], wz = s[12];
+  double g4 = g * 0.25;
+  double usum = ((u[0] + u[1]) + u[2]) + u[3];
+  double f = g4 * usum;
+  double F = m * g4;
+  double tx = (arm * F) * (((u[2] + u[3]) - u[0]) - u[1]);
+  double ty = (arm * F) * (((u[1] + u[2]) - u[0]) - u[3]);
+  double tz = (t2t * F) * (((u[1] + u[3]) - u[0]) - u[2]);
+  double rzx = 2.0 * (qx * qz + qw * qy);
+  double rzy = 2.0 * (qy * qz - qw * qx);
+  double rzz = 1.0 - 2.0 * (qx * qx + qy * qy);
+  double ax = f * rzx;
+  double ay = f * rzy;
+  double az = f * rzz - g;
+  double jwx = Jx * wx, jwy = Jy * wy, jwz = Jz * wz;
+  double cx = wy * jwz - wz * jwy;
+  double cy = wz * jwx - wx * jwz;
+  double cz = wx * jwy - wy * jwx;
+  double dwx = (tx - cx) / Jx;
+  double dwy = (ty - cy) / Jy;
+  double dwz = (tz - cz) / Jz;
+  o[0] = s[0] + dt * vx;
+  o[1] = s[1] + dt * vy;
+  o[2] = s[2] + dt * vz;
+  o[7] = vx + dt * ax;
+  o[8] = vy + dt * ay;
+  o[9] = vz + dt * az;
+  o[10] = wx + dt * dwx;
+  o[11] = wy + dt * dwy;
+  o[12] = wz + dt * dwz;
+  double h = 0.5 * dt;
+  double z = (h * h) * ((wx * wx + wy * wy) + wz * wz);
+  double ec, esinc;
+  dp_quad_cos_sinc(S, z, &ec, &esinc);
+  double k = h * esinc;
+  double ex = k * wx, ey = k * wy, ez = k * wz, ew = ec;
+  double nw = ((qw * ew - qx * ex) - qy * ey) - qz * ez;
+  double nx = ((qw * ex + qx * ew) + qy * ez) - qz * ey;
+  double ny = ((qw * ey - qx * ez) + qy * ew) + qz * ex;
+  double nz = ((qw * ez + qx * ey) - qy * ex) + qz * ew;
+  double n2 = ((nx * nx + ny * ny) + nz * nz) + nw * nw;
+  double inv = 1.0 / sqrt(n2);
+  o[3] = nx * inv;
+  o[4] = ny * inv;
+  o[5] = nz * inv;
+  o[6] = nw * inv;
+}
+
 __device__ __forceinline__ uint64_t dp_ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ int dp_lane() { return __lane_id(); }

@@ -18,6 +18,11 @@
 // Bound on back-off iterations of one wave in the ticketed resolution;
 // reaching it leaves the node to the host-driven rounds instead of waiting.
 #define DP_SPIN_LIMIT (1 << 20)
+// Persistent resolution: polls (s_sleep 1 each) one lane may spend waiting
+// for an undecided contender of an earlier node, and the retries of a node
+// whose waits all ran out (bounds the kernel's run time if a wait never ends).
+#define DP_WAIT_POLLS (1 << 14)
+#define DP_SPIN_LIMIT_WAITING (1 << 10)
 // Earlier-conflict slots stored per pair (more: the resolution scans the grid).
 #define DP_CONF_SLOTS 16
 
@@ -127,6 +132,11 @@ struct dp_ctx {
   // primitives
   int32_t P = 0, S = 0, maxlen = 0;
   std::vector<int32_t> h_off;
+  std::vector<double> h_st;   // [S][nx] primitive states as uploaded (host search: trajectories)
+  std::vector<double> h_ac;   // [S - P][nu] actions, or empty
+  int32_t h_nu = 0;
+  std::vector<double> sol_states, sol_actions;  // trajectory of the last dp_search
+  int32_t sol_nu = 0;
   int32_t* d_off = nullptr;      // P+1
   double* d_st = nullptr;        // [nx][S]
   double* d_start = nullptr;     // [nx][P]

@@ -229,6 +229,7 @@ struct round_args {
   int32_t* count;
   int32_t* node_novel;
   int32_t limit;
+  int32_t wait;  // polls a lane may spend on one undecided earlier contender
   long long* counters;
 };
 
@@ -259,13 +260,22 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
     const bool dupv = (fl & 1) != 0;
     bool ext_dup = false, ext_block = false;
     uint64_t same = 0;
-    if (fr && !forced && !dupv) {
+    // Earlier contenders of pair p: those in this chunk go to `same` (the
+    // walk below decides them in order); an earlier-chunk contender already
+    // NOVEL makes p a duplicate, one still undecided blocks p.  With
+    // polls > 0 an undecided contender is waited for (it belongs to an
+    // earlier node, whose wave is resident in persistent mode).
+    auto eval = [&](int32_t p, int32_t polls) {
       auto visit = [&](int32_t o) {
         if (o >= cb) {
           same |= 1ull << (o - cb);
           return true;
         }
-        const int r = __hip_atomic_load(&a.res[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int r = __hip_atomic_load(&a.res[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int32_t w = 0; r == DP_R_UNDECIDED && w < polls; ++w) {
+          __builtin_amdgcn_s_sleep(1);
+          r = __hip_atomic_load(&a.res[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         if (r == DP_R_NOVEL) ext_dup = true;
         else if (r == DP_R_UNDECIDED) ext_block = true;
         return !ext_dup;
@@ -279,7 +289,8 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
         for (int k = 0; k < NX; ++k) x[k] = a.fin[(int64_t)k * a.pcap + p];
         scan_earlier<NX>(*a.sys, a.grid, a.cstart, a.cidx, a.cx, a.pcap, x, p, a.r_dup, visit);
       }
-    }
+    };
+    if (fr && !forced && !dupv) eval(p, 0);
     const uint64_t free_m = __ballot(fr);
     const uint64_t forced_m = __ballot(fr && forced);
     const uint64_t dupv_m = __ballot(fr && dupv && !forced);
@@ -297,8 +308,23 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
       else if (dupv_m & bit) dupvr_m |= bit;
       else if (dup_m & bit) dupb_m |= bit;
       else if (block_m & bit) {
-        stop_lane = b;
-        break;
+        // Only this lane's verdict is needed next: evaluate it again,
+        // waiting for its undecided contenders.
+        bool d = false, blk = false;
+        if (a.wait > 0 && lane == b) {
+          ext_dup = false;
+          ext_block = false;
+          eval(p, a.wait);
+          d = ext_dup;
+          blk = ext_block && !ext_dup;
+        }
+        const bool dd = __ballot(d) != 0;
+        if (a.wait <= 0 || __ballot(blk) != 0) {
+          stop_lane = b;
+          break;
+        }
+        if (dd || (readlane64(same, b) & novel_m)) dupb_m |= bit;
+        else novel_m |= bit;
       } else if (readlane64(same, b) & novel_m) dupb_m |= bit;
       else novel_m |= bit;
     }
@@ -363,8 +389,9 @@ __global__ __launch_bounds__(64) void k_resolve_ticketed(round_args a) {
   if (node >= a.N) return;
   int32_t spins = 0;
   int32_t cur = a.cursor[node], count = a.count[node];
+  const int32_t limit = a.wait > 0 ? DP_SPIN_LIMIT_WAITING : DP_SPIN_LIMIT;
   while (!advance_node<NX>(a, node, cur, count)) {
-    if (++spins > DP_SPIN_LIMIT) {
+    if (++spins > limit) {
       if (threadIdx.x == 0) atomicOr((unsigned long long*)&a.counters[DPC_ERRFLAG], 2ull);
       break;
     }
@@ -437,6 +464,7 @@ static round_args make_round_args(dp_ctx* c) {
   a.count = c->d_node_count;
   a.node_novel = c->d_node_novel;
   a.limit = c->round_limit;
+  a.wait = 0;
   a.counters = c->d_counters;
   return a;
 }
@@ -501,6 +529,7 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
     k_round_init<<<dp_grid_blocks(N, 256, 1024), 256, 0, s>>>(N, c->d_app, c->d_total, c->d_node_cursor,
                                                               c->d_node_count, c->d_node_novel, c->d_counters);
     round_args ra = make_round_args(c);
+    ra.wait = DP_WAIT_POLLS;
     k_resolve_ticketed<NX><<<N, 64, 0, s>>>(ra);
     return DP_OK;
   });

@@ -642,6 +642,8 @@ int dp_surrogate_train(dp_ctx* c, const double* nodes, int32_t N, const dp_expan
     DP_HIP_RET(hipMalloc(&m.loss, sizeof(float) * 2));
   }
   const int64_t steps_per_epoch = (npairs + batch - 1) / batch;
+  const double total_steps = (double)steps_per_epoch * epochs;
+  int64_t step = 0;
   for (int32_t e = 0; e < epochs; ++e) {
     DP_HIP_RET(hipMemsetAsync(m.loss + 1, 0, sizeof(float), s));
     for (int64_t t = 0; t < steps_per_epoch; ++t) {
@@ -652,7 +654,10 @@ int dp_surrogate_train(dp_ctx* c, const double* nodes, int32_t N, const dp_expan
       k_gather_labels<<<dp_grid_blocks(batch, 256, 256), 256, 0, s>>>(m.sid, m.slabel, batch, m.zb + 2 * batch);
       rc = dp_mlp_fwd_bwd(c, m.xb, m.zb + 2 * batch, batch, m.sid, m.sprim, m.loss + 1, s);
       if (rc) return rc;
-      if ((rc = adam_step(c, lr, s))) return rc;
+      // learning rate decays linearly to 10% over the run
+      const float lr_t = (float)(lr * (1.0 - 0.9 * (double)step / total_steps));
+      ++step;
+      if ((rc = adam_step(c, lr_t, s))) return rc;
     }
     if (loss_out) {
       float h[2];
@@ -794,6 +799,18 @@ int dp_surrogate_grad(dp_ctx* c, const float* params, const float* X, const floa
   (void)hipFree(xb);
   DP_HIP_RET(hipMemset(m.g, 0, sizeof(float) * L.total));
   return rc;
+}
+
+int dp_surrogate_get(dp_ctx* c, float* params, float* emb) {
+  if (!c || !c->mlp.w) {
+    dp_set_error("no surrogate");
+    return DP_ERR_STATE;
+  }
+  DP_HIP_RET(hipSetDevice(c->device));
+  DP_HIP_RET(hipDeviceSynchronize());
+  if (params) DP_HIP_RET(hipMemcpy(params, c->mlp.w, sizeof(float) * layout_of().total, hipMemcpyDeviceToHost));
+  if (emb) DP_HIP_RET(hipMemcpy(emb, c->mlp.emb, sizeof(float) * (size_t)c->mlp.P * 64, hipMemcpyDeviceToHost));
+  return DP_OK;
 }
 
 // C = A B with A [M][K], B [K][N] FP32 inputs rounded to BF16 (numerics test

@@ -73,73 +73,6 @@ namespace {
 
 constexpr int kQWaves = 4;
 
-// ---- FP64 step: the oracle's operation order --------------------------------
-__device__ __forceinline__ void quad_cos_sinc(const dp_dev_sys& S, double z, double* c, double* sinc) {
-  double pc = S.cos_c[DP_TRIG_TERMS - 1];
-  double ps = S.sinc_c[DP_TRIG_TERMS - 1];
-#pragma unroll
-  for (int n = DP_TRIG_TERMS - 2; n >= 0; --n) {
-    pc = pc * z + S.cos_c[n];
-    ps = ps * z + S.sinc_c[n];
-  }
-  *c = pc;
-  *sinc = ps;
-}
-
-__device__ __forceinline__ void quad_step64(const dp_dev_sys& S, const double* s, const double* u, double* o) {
-  const double m = S.quad[0], g = S.quad[1], arm = S.quad[2], t2t = S.quad[3];
-  const double Jx = S.quad[4], Jy = S.quad[5], Jz = S.quad[6];
-  const double dt = S.dt;
-  const double qx = s[3], qy = s[4], qz = s[5], qw = s[6];
-  const double vx = s[7], vy = s[8], vz = s[9];
-  const double wx = s[10], wy = s[11], wz = s[12];
-  double g4 = g * 0.25;
-  double usum = ((u[0] + u[1]) + u[2]) + u[3];
-  double f = g4 * usum;
-  double F = m * g4;
-  double tx = (arm * F) * (((u[2] + u[3]) - u[0]) - u[1]);
-  double ty = (arm * F) * (((u[1] + u[2]) - u[0]) - u[3]);
-  double tz = (t2t * F) * (((u[1] + u[3]) - u[0]) - u[2]);
-  double rzx = 2.0 * (qx * qz + qw * qy);
-  double rzy = 2.0 * (qy * qz - qw * qx);
-  double rzz = 1.0 - 2.0 * (qx * qx + qy * qy);
-  double ax = f * rzx;
-  double ay = f * rzy;
-  double az = f * rzz - g;
-  double jwx = Jx * wx, jwy = Jy * wy, jwz = Jz * wz;
-  double cx = wy * jwz - wz * jwy;
-  double cy = wz * jwx - wx * jwz;
-  double cz = wx * jwy - wy * jwx;
-  double dwx = (tx - cx) / Jx;
-  double dwy = (ty - cy) / Jy;
-  double dwz = (tz - cz) / Jz;
-  o[0] = s[0] + dt * vx;
-  o[1] = s[1] + dt * vy;
-  o[2] = s[2] + dt * vz;
-  o[7] = vx + dt * ax;
-  o[8] = vy + dt * ay;
-  o[9] = vz + dt * az;
-  o[10] = wx + dt * dwx;
-  o[11] = wy + dt * dwy;
-  o[12] = wz + dt * dwz;
-  double h = 0.5 * dt;
-  double z = (h * h) * ((wx * wx + wy * wy) + wz * wz);
-  double ec, esinc;
-  quad_cos_sinc(S, z, &ec, &esinc);
-  double k = h * esinc;
-  double ex = k * wx, ey = k * wy, ez = k * wz, ew = ec;
-  double nw = ((qw * ew - qx * ex) - qy * ey) - qz * ez;
-  double nx = ((qw * ex + qx * ew) + qy * ez) - qz * ey;
-  double ny = ((qw * ey - qx * ez) + qy * ew) + qz * ex;
-  double nz = ((qw * ez + qx * ey) - qy * ex) + qz * ew;
-  double n2 = ((nx * nx + ny * ny) + nz * nz) + nw * nw;
-  double inv = 1.0 / sqrt(n2);
-  o[3] = nx * inv;
-  o[4] = ny * inv;
-  o[5] = nz * inv;
-  o[6] = nw * inv;
-}
-
 __device__ __forceinline__ bool state_valid64(const dp_dev_sys& S, const double* s) {
   bool ok = true;
 #pragma unroll
@@ -461,7 +394,7 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
         double u[4], o[13];
 #pragma unroll
         for (int c = 0; c < 4; ++c) u[c] = a.act[(int64_t)c * a.A + a0 + k - 1];
-        quad_step64(S, s, u, o);
+        dp_quad_step64(S, s, u, o);
 #pragma unroll
         for (int c = 0; c < 13; ++c) s[c] = o[c];
       }

@@ -146,6 +146,26 @@ class GpuExpander:
         N.check(self.lib.dp_surrogate_scores(self.ctx, out.ctypes.data_as(C.POINTER(C.c_float)), n))
         return out
 
+    # ---- db-A* search (dbastar.cpp:523-1010) ------------------------------------
+    def search(self, start, goal, **options):
+        """Run the host search loop; returns (info dict, states [n][nx],
+        actions [m][nu]).  Options are dp_search_options fields."""
+        opt = N.SearchOptions()
+        self.lib.dp_search_options_default(C.byref(opt))
+        for k, v in options.items():
+            setattr(opt, k, v)
+        start = np.ascontiguousarray(start, np.float64)
+        goal = np.ascontiguousarray(goal, np.float64)
+        info = N.SearchInfo()
+        N.check(self.lib.dp_search(self.ctx, _dptr(start), _dptr(goal), C.byref(opt), C.byref(info)))
+        d = info.as_dict()
+        states = np.zeros((d["traj_states"], self.nx))
+        nu = self.desc.nu
+        actions = np.zeros((d["traj_actions"], nu))
+        N.check(self.lib.dp_search_trajectory(self.ctx, _dptr(states) if states.size else None,
+                                              _dptr(actions) if actions.size else None))
+        return d, states, actions
+
     def fetch(self):
         n = self.counts()["pairs"]
         out = {

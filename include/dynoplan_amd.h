@@ -264,7 +264,8 @@ typedef struct dp_surrogate_stats {
 
 int dp_surrogate_init(dp_ctx* ctx, uint64_t seed);
 /* Exact-checker labels (FREE = 1) of the pairs of `nodes`, then `epochs`
- * passes of minibatch Adam; loss_out[epochs] (optional) gets the epoch losses. */
+ * passes of minibatch Adam (learning rate `lr` decaying linearly to lr/10);
+ * loss_out[epochs] (optional) gets the epoch losses. */
 int dp_surrogate_train(dp_ctx* ctx, const double* nodes, int32_t N, const dp_expand_params* params, int32_t epochs,
                        int32_t batch, float lr, uint64_t seed, float* loss_out);
 /* Threshold = lowest score of a FREE pair of held-out `nodes` minus margin. */
@@ -278,6 +279,10 @@ int64_t dp_surrogate_param_count(void);
  * labels y [B], parameters `params`: mean BCE loss and its gradient. */
 int dp_surrogate_grad(dp_ctx* ctx, const float* params, const float* X, const float* y, int32_t B, float* loss,
                       float* grad);
+/* Current FP32 master parameters (dp_surrogate_param_count() floats, the
+ * layout W1[256][1088] b1 W2[256][256] b2 W3[256][256] b3 w4[256] b4) and
+ * primitive embeddings [P][64]; either pointer may be NULL. */
+int dp_surrogate_get(dp_ctx* ctx, float* params, float* emb);
 /* Numerics check of the training GEMM: C = A B, inputs rounded to BF16. */
 int dp_debug_gemm_bf16(dp_ctx* ctx, const float* A, const float* B, int32_t M, int32_t N, int32_t K, float* C);
 
@@ -286,6 +291,46 @@ int dp_debug_gemm_bf16(dp_ctx* ctx, const float* A, const float* B, int32_t M, i
  * dp_expand (other entries unspecified).  No counterpart in the reference;
  * used to report the FP32-vs-FP64 rollout error per system. */
 int dp_debug_fp32_positions(dp_ctx* ctx, float* out, int64_t n);
+
+/* ---- db-A* search on the expansion path (dbastar.cpp:523-1010) -------------
+ * The host loop of dbastar(): open list ordered by float fScore then higher
+ * gScore, goal test on every popped node, expansion of the popped nodes with
+ * dp_expand_batch, a node for every NOVEL pair, rewiring of the neighbours
+ * of duplicate pairs, the solution path.  Uses the context's primitives and
+ * world, and replaces its visited set with the search's nodes. */
+enum { DP_SEARCH_SOLVED = 0, DP_SEARCH_MAX_EXPANDS = 1, DP_SEARCH_EMPTY_QUEUE = 2, DP_SEARCH_MAX_TIME = 3 };
+
+typedef struct dp_search_options {  /* Options_dbastar (options.hpp:105-156) */
+  float delta;                      /* .3 */
+  float alpha;                      /* .5 */
+  double delta_factor_goal;         /* 1 */
+  double cost_delta_factor;         /* 1 */
+  int64_t max_expands;              /* 1e6 */
+  double search_timelimit_ms;       /* 1e4 */
+  int32_t limit_branching_factor;   /* 20 */
+  int32_t heuristic;                /* 0: lower_bound_time to the goal, -1: blind */
+  int32_t batch_nodes;              /* nodes popped per expansion; 1 = the reference's order */
+  int32_t use_surrogate;
+  int32_t fp32_rollout;
+  int32_t reserved;
+} dp_search_options;
+
+typedef struct dp_search_info {     /* Out_info_db and Time_benchmark fields */
+  int32_t solved, status;           /* DP_SEARCH_* */
+  double cost;                      /* gScore of the solution (or closest) node */
+  double distance_to_goal;
+  int64_t expands, all_nodes, closed, rewires, gpu_batches;
+  int64_t pairs, checked;           /* checked = num_col_motions */
+  int64_t traj_states, traj_actions;
+  double time_search_ms, time_gpu_ms;
+} dp_search_info;
+
+void dp_search_options_default(dp_search_options* opt);
+int dp_search(dp_ctx* ctx, const double* start, const double* goal, const dp_search_options* opt,
+              dp_search_info* info);
+/* The last search's trajectory: states [traj_states][nx], actions
+ * [traj_actions][nu] (either pointer may be NULL). */
+int dp_search_trajectory(dp_ctx* ctx, double* states, double* actions);
 
 /* ---- many planning instances in one launch ---------------------------------
  * No counterpart in the reference, which plans one instance per process.  A

@@ -90,7 +90,7 @@ def world_struct(lo, hi, boxes, spheres):
 
 
 def expand(wl, nodes=None, g_parent=None, visited=None, params=None, nn_mode=NN_GRID, col_mode=COL_SWEPT,
-           max_motions=0, threads=0, want_out=True):
+           max_motions=0, threads=0, want_out=True, cap=None):
     """Run the oracle on a synth.Workload (optionally overriding parts).
     threads > 0 runs oracle_expand_mt on that many OpenMP threads; with
     want_out False only the counts are produced (timing runs)."""
@@ -109,7 +109,8 @@ def expand(wl, nodes=None, g_parent=None, visited=None, params=None, nn_mode=NN_
     nx = wl.desc.nx
     # capacity: at most N * P pairs
     P = offsets.size - 1 if max_motions <= 0 else min(max_motions, offsets.size - 1)
-    cap = max(1, nodes.shape[0] * P) if want_out else 1
+    if cap is None:
+        cap = max(1, nodes.shape[0] * P) if want_out else 1
     out = {
         "node": np.zeros(cap, np.int32), "prim": np.zeros(cap, np.int32), "verdict": np.zeros(cap, np.int8),
         "first_k": np.zeros(cap, np.int32), "goal_index": np.zeros(cap, np.int32),

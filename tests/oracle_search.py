@@ -1,0 +1,205 @@
+"""CPU restatement of the db-A* search loop over the FP64 oracle (test
+infrastructure: only tests/ use it).
+
+Follows src/dbastar/dbastar.cpp:523-1010 with one popped node per
+iteration: open list ordered by compareAStarNode (dbastar.cpp:56-68) on float
+f/g scores, goal test on the popped node (:787-802), expansion of every
+applicable primitive in canonical order through oracle_expand, a node for
+every NOVEL pair (:856-876), rewiring of every neighbour within
+(1 - alpha) delta for duplicate pairs (:883-910), the closest node when not
+solved (:944-952), and the trajectory of from_solution_to_yaml_and_traj
+(:129-310).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+import oracle_lib as O
+from dynoplan_amd import _native as N
+
+f32 = np.float32
+
+
+class Heap:
+    """Binary max-heap of entries under compareAStarNode; a node re-pushed
+    after it was closed owns a second entry (the reference re-pushes without
+    marking it open)."""
+
+    def __init__(self, nodes):
+        self.nodes = nodes
+        self.h = []
+        self.en = []
+        self.pos = []
+
+    def after(self, a, b):
+        x, y = self.nodes[self.en[a]], self.nodes[self.en[b]]
+        if x["f"] != y["f"]:
+            return x["f"] > y["f"]
+        return x["g"] < y["g"]
+
+    def _put(self, i, e):
+        self.h[i] = e
+        self.pos[e] = i
+
+    def up(self, i):
+        while i > 0:
+            p = (i - 1) // 2
+            if not self.after(self.h[p], self.h[i]):
+                break
+            e = self.h[i]
+            self._put(i, self.h[p])
+            self._put(p, e)
+            i = p
+
+    def down(self, i):
+        n = len(self.h)
+        while True:
+            c = 2 * i + 1
+            if c >= n:
+                break
+            top = c + 1 if c + 1 < n and self.after(self.h[c], self.h[c + 1]) else c
+            if self.after(self.h[top], self.h[i]):
+                break
+            e = self.h[i]
+            self._put(i, self.h[top])
+            self._put(top, e)
+            i = top
+
+    def push(self, node):
+        e = len(self.en)
+        self.en.append(node)
+        self.pos.append(len(self.h))
+        self.h.append(e)
+        self.up(len(self.h) - 1)
+        return e
+
+    def pop(self):
+        e = self.h[0]
+        self._put(0, self.h[-1])
+        self.h.pop()
+        if self.h:
+            self.down(0)
+        self.pos[e] = -1
+        return self.en[e]
+
+
+def search(wl, start, goal, delta=0.3, alpha=0.5, max_expands=100000, limit_branching_factor=20,
+           delta_factor_goal=1.0, cost_delta_factor=1.0):
+    lib = O.load()
+    d = wl.desc
+    nx = d.nx
+    start = np.ascontiguousarray(start, np.float64)
+    goal = np.ascontiguousarray(goal, np.float64)
+    par = N.ExpandParams()
+    par.delta = delta
+    par.alpha = alpha
+    par.delta_factor_goal = delta_factor_goal
+    par.cost_delta_factor = cost_delta_factor
+    par.limit_branching_factor = limit_branching_factor
+    par.num_check_goal = 4
+    par.check_intermediate_goal = 1
+    for k in range(nx):
+        par.goal[k] = goal[k]
+    r_dup = (1.0 - float(par.alpha)) * float(par.delta)
+    tol = delta_factor_goal * float(par.delta)
+
+    def dist(a, b):
+        return lib.oracle_distance(C.byref(d), O.dptr(np.ascontiguousarray(a)), O.dptr(np.ascontiguousarray(b)))
+
+    def lbt(a, b):
+        return lib.oracle_lower_bound_time(C.byref(d), O.dptr(np.ascontiguousarray(a)),
+                                           O.dptr(np.ascontiguousarray(b)))
+
+    h0 = f32(lbt(start, goal))
+    nodes = [dict(g=f32(0.0), h=h0, f=f32(f32(0.0) + h0), parent=-1, motion=-1, inter=-1, open=True)]
+    X = [start.copy()]
+    XP = np.zeros((1024, d.pos_dim))  # positions, for a conservative pre-filter of radius queries
+    XP[0] = start[:d.pos_dim]
+    w0 = float(d.dist_w[0])
+    heap = Heap(nodes)
+    nodes[0]["handle"] = heap.push(0)
+    info = dict(expands=0, rewires=0, status=None)
+    sol = -1
+    while True:
+        if info["expands"] >= max_expands:
+            info["status"] = N.SEARCH_MAX_EXPANDS
+            break
+        if not heap.h:
+            info["status"] = N.SEARCH_EMPTY_QUEUE
+            break
+        b = heap.pop()
+        nodes[b]["open"] = False
+        info["expands"] += 1
+        if dist(X[b], goal) < tol:
+            info["status"] = N.SEARCH_SOLVED
+            sol = b
+            break
+        out, _ = O.expand(wl, nodes=X[b][None, :], g_parent=np.array([float(nodes[b]["g"])]),
+                          visited=np.array(X), params=par, want_out=True, cap=int(wl.offsets.size - 1))
+        for q in range(out["resolution"].size):
+            r = int(out["resolution"][q])
+            x = out["final_state"][q]
+            gq = float(out["g"][q])
+            if r == N.R_NOVEL:
+                hq = float(out["h"][q])
+                nodes.append(dict(g=f32(gq), h=f32(hq), f=f32(gq + hq), parent=b, motion=int(out["prim"][q]),
+                                  inter=int(out["goal_index"][q]), open=True))
+                X.append(x.copy())
+                if len(X) > XP.shape[0]:
+                    XP = np.concatenate([XP, np.zeros_like(XP)])
+                XP[len(X) - 1] = x[:d.pos_dim]
+                nodes[-1]["handle"] = heap.push(len(nodes) - 1)
+            elif r in (N.R_DUP_VISITED, N.R_DUP_BATCH):
+                # the metric is at least w0 * |position difference|
+                pd = np.sqrt(((XP[:len(X)] - x[:d.pos_dim]) ** 2).sum(1))
+                for j in np.nonzero(w0 * pd <= r_dup * (1 + 1e-6) + 1e-12)[0]:
+                    j = int(j)
+                    if not dist(x, X[j]) <= r_dup:
+                        continue
+                    t = gq + lbt(x, X[j])
+                    m = nodes[j]
+                    if t < m["g"]:
+                        m["g"] = f32(t)
+                        m["f"] = f32(t + float(m["h"]))
+                        m["parent"] = b
+                        m["motion"] = int(out["prim"][q])
+                        m["inter"] = -1
+                        info["rewires"] += 1
+                        if m["open"]:
+                            heap.up(heap.pos[m["handle"]])
+                        else:
+                            m["handle"] = heap.push(j)
+    if sol < 0:
+        ds = [dist(x, goal) for x in X]
+        sol = int(np.argmin(ds))
+    info["solved"] = int(info["status"] == N.SEARCH_SOLVED)
+    info["cost"] = float(nodes[sol]["g"])
+    info["all_nodes"] = len(nodes)
+    path = []
+    v = sol
+    while v != -1:
+        path.append(v)
+        v = nodes[v]["parent"]
+    path.reverse()
+    states, actions = [], []
+    if len(path) == 1:
+        states.append(X[path[0]])
+    pr = O.prims_struct(np.ascontiguousarray(wl.offsets, np.int32), np.ascontiguousarray(wl.states),
+                        np.ascontiguousarray(wl.actions))
+    for i in range(len(path) - 1):
+        ch = nodes[path[i + 1]]
+        j = ch["motion"]
+        L1 = int(wl.offsets[j + 1] - wl.offsets[j])
+        traj = np.zeros((L1, nx))
+        lib.oracle_transform_primitive(C.byref(d), C.byref(pr), j, O.dptr(X[path[i]]), O.dptr(traj))
+        take = ch["inter"] + 1 if ch["inter"] != -1 else L1
+        for k in range(take):
+            if k < take - 1:
+                states.append(traj[k])
+            elif i == len(path) - 2:
+                states.append(X[path[i + 1]])
+        na = ch["inter"] if ch["inter"] != -1 else L1 - 1
+        a0 = int(wl.offsets[j]) - j
+        actions.extend(wl.actions[a0:a0 + na])
+    return info, np.array(states), np.array(actions).reshape(-1, wl.actions.shape[1])
