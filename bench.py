@@ -141,12 +141,12 @@ def train_surrogate(ex, wl, label_pairs):
     progress(f"surrogate: {n_labels} exact-checker labels from {n_train} nodes")
     ex.surrogate_init(seed=wl.seed)
     t0 = time.perf_counter()
-    losses = ex.surrogate_train(train_nodes, p0, epochs=4, batch=4096, lr=2e-3, seed=wl.seed)
+    losses = ex.surrogate_train(train_nodes, p0, epochs=8, batch=4096, lr=2e-3, seed=wl.seed)
     t_train = time.perf_counter() - t0
     stats = ex.surrogate_calibrate(cal_nodes, p0, margin=0.5)
     progress(f"surrogate: trained in {t_train:.1f} s, epoch losses {[round(float(x), 4) for x in losses]}, "
              f"calibration {stats}")
-    return {"labels": int(n_labels), "train_nodes": n_train, "epochs": 4, "train_s": t_train,
+    return {"labels": int(n_labels), "train_nodes": n_train, "epochs": 8, "train_s": t_train,
             "epoch_losses": [float(x) for x in losses], "calibration": stats}
 
 

@@ -121,6 +121,13 @@ class SurrogateStats(C.Structure):
 SEARCH_SOLVED, SEARCH_MAX_EXPANDS, SEARCH_EMPTY_QUEUE, SEARCH_MAX_TIME = 0, 1, 2, 3
 
 
+class Problem(C.Structure):
+    _fields_ = [("system", C.c_int32), ("nx", C.c_int32), ("robot_type", C.c_char * 64),
+                ("start", C.c_double * DP_MAX_NX), ("goal", C.c_double * DP_MAX_NX),
+                ("lo", C.c_double * 3), ("hi", C.c_double * 3), ("n_boxes", C.c_int32), ("n_spheres", C.c_int32),
+                ("boxes", C.POINTER(C.c_double)), ("spheres", C.POINTER(C.c_double))]
+
+
 class SearchOptions(C.Structure):
     _fields_ = [("delta", C.c_float), ("alpha", C.c_float), ("delta_factor_goal", C.c_double),
                 ("cost_delta_factor", C.c_double), ("max_expands", C.c_int64), ("search_timelimit_ms", C.c_double),
@@ -166,6 +173,9 @@ SIGNATURES = {
     "dp_search": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(SearchOptions),
                             C.POINTER(SearchInfo)]),
     "dp_search_trajectory": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "dp_load_problem": (C.c_int, [C.c_char_p, C.POINTER(Problem)]),
+    "dp_free_problem": (None, [C.POINTER(Problem)]),
+    "dp_save_problem": (C.c_int, [C.c_char_p, C.POINTER(Problem)]),
     "dp_batch_create": (C.c_void_p, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32]),
     "dp_batch_destroy": (None, [C.c_void_p]),
     "dp_batch_expand": (C.c_int, [C.c_void_p, C.POINTER(ExpandParams), C.c_void_p]),

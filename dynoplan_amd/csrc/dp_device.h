@@ -130,6 +130,30 @@ DP_HD bool dp_within(const dp_dev_sys& S, const double* a, const double* b, doub
   return d <= r;
 }
 
+// The first term of dp_distance, the weighted position distance, by the same
+// expression: when it alone exceeds r so does the distance (the other terms
+// are non-negative and IEEE addition is monotone).
+DP_HD double dp_pos_term(const dp_dev_sys& S, const double* a, const double* b) {
+  if (S.system == DP_SYS_QUADROTOR) return S.w[0] * dp_l2_3(a, b);
+  return S.w[0] * dp_l2_2(a[0], a[1], b[0], b[1]);
+}
+
+// dp_within(S, x, y, r) for y stored by component (y_k = col[k * stride + i]):
+// the position is read first, the other components only when the position
+// term does not already exceed r.
+template <int NX>
+__device__ __forceinline__ bool dp_within_col(const dp_dev_sys& S, const double* x, const double* col,
+                                              int64_t stride, int64_t i, double r) {
+  constexpr int PD = NX == 13 ? 3 : 2;
+  double y[NX];
+#pragma unroll
+  for (int k = 0; k < PD; ++k) y[k] = col[(int64_t)k * stride + i];
+  if (dp_pos_term(S, x, y) > r) return false;
+#pragma unroll
+  for (int k = PD; k < NX; ++k) y[k] = col[(int64_t)k * stride + i];
+  return dp_within(S, x, y, r);
+}
+
 DP_HD double dp_max(double a, double b) { return b > a ? b : a; }
 
 // robot->lower_bound_time.

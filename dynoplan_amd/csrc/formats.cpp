@@ -713,4 +713,166 @@ int dp_save_primitives(const char* path, int32_t format, const dp_prim_set* set)
   return DP_OK;
 }
 
+// ---- problem files ------------------------------------------------------------------
+// dynobench's problem YAML (the layout benchmark.py's create_empty_env
+// writes, benchmark/benchmark.py:330-350):
+//   environment: {min: [..], max: [..], obstacles: [{type, center, size}]}
+//   robots: [{type: <model name>, start: [..], goal: [..]}]
+// Box sizes are full side lengths; a sphere's size is [radius].
+
+static int system_of_model(const std::string& t) {
+  auto starts = [&](const char* p) { return t.compare(0, std::strlen(p), p) == 0; };
+  if (starts("unicycle1")) return DP_SYS_UNICYCLE1;
+  if (starts("unicycle2")) return DP_SYS_UNICYCLE2;
+  if (starts("quadrotor") || starts("quad3d")) return DP_SYS_QUADROTOR;
+  if (starts("car_first_order_with_1_trailers") || starts("car1")) return DP_SYS_CAR_TRAILER;
+  return -1;
+}
+
+int dp_load_problem(const char* path, dp_problem* out) {
+  if (!path || !out) {
+    dp_set_error("invalid argument");
+    return DP_ERR_ARG;
+  }
+  std::memset(out, 0, sizeof(*out));
+  std::ifstream f(path, std::ios::binary);
+  if (!f) {
+    dp_set_error(std::string("cannot open ") + path);
+    return DP_ERR_IO;
+  }
+  std::stringstream ss;
+  ss << f.rdbuf();
+  const std::string data = ss.str();
+  std::vector<double> boxes, spheres;
+  try {
+    YamlReader r(data);
+    if (r.L.empty()) throw ParseError("empty problem file");
+    Node doc = r.block(r.L[0].indent);
+    const Node* env = doc.get("environment");
+    const Node* robots = doc.get("robots");
+    if (!env || !robots || robots->kind != Node::ARR || robots->arr.empty())
+      throw ParseError("expected 'environment' and a non-empty 'robots' list");
+    const Node& rb = robots->arr[0];
+    const Node* type = rb.get("type");
+    if (!type || type->kind != Node::STR) throw ParseError("robots[0].type missing");
+    const int sys = system_of_model(type->str);
+    if (sys < 0) throw ParseError("unsupported robot type " + type->str);
+    dp_system_desc d;
+    dp_system_default(sys, &d);
+    out->system = sys;
+    out->nx = d.nx;
+    std::snprintf(out->robot_type, sizeof(out->robot_type), "%s", type->str.c_str());
+    for (const char* key : {"start", "goal"}) {
+      const Node* v = rb.get(key);
+      if (!v) throw ParseError(std::string("robots[0].") + key + " missing");
+      std::vector<double> x = as_vec(*v);
+      if ((int)x.size() != d.nx) throw ParseError(std::string(key) + " has the wrong dimension");
+      std::memcpy(key[0] == 's' ? out->start : out->goal, x.data(), sizeof(double) * x.size());
+    }
+    const Node* mn = env->get("min");
+    const Node* mx = env->get("max");
+    if (!mn || !mx) throw ParseError("environment.min / max missing");
+    std::vector<double> lo = as_vec(*mn), hi = as_vec(*mx);
+    if (lo.size() != hi.size() || lo.size() < 2 || lo.size() > 3) throw ParseError("environment bounds");
+    for (size_t k = 0; k < lo.size(); ++k) {
+      out->lo[k] = lo[k];
+      out->hi[k] = hi[k];
+    }
+    const Node* obs = env->get("obstacles");
+    if (obs && obs->kind == Node::ARR)
+      for (const Node& o : obs->arr) {
+        const Node* ot = o.get("type");
+        const Node* c = o.get("center");
+        const Node* sz = o.get("size");
+        if (!ot || !c || !sz) throw ParseError("obstacle needs type, center, size");
+        std::vector<double> cv = as_vec(*c), sv = as_vec(*sz);
+        cv.resize(3, 0.0);
+        if (ot->str == "box") {
+          sv.resize(3, 0.0);
+          for (int k = 0; k < 3; ++k) boxes.push_back(cv[k]);
+          for (int k = 0; k < 3; ++k) boxes.push_back(0.5 * sv[k]);
+        } else if (ot->str == "sphere") {
+          if (sv.empty()) throw ParseError("sphere size");
+          for (int k = 0; k < 3; ++k) spheres.push_back(cv[k]);
+          spheres.push_back(sv[0]);
+        } else {
+          throw ParseError("unsupported obstacle type " + ot->str);
+        }
+      }
+  } catch (const std::exception& e) {
+    dp_set_error(std::string(path) + ": " + e.what());
+    return DP_ERR_FORMAT;
+  }
+  out->n_boxes = (int32_t)(boxes.size() / 6);
+  out->n_spheres = (int32_t)(spheres.size() / 4);
+  out->boxes = (double*)std::malloc(sizeof(double) * (boxes.size() + 1));
+  out->spheres = (double*)std::malloc(sizeof(double) * (spheres.size() + 1));
+  if (!out->boxes || !out->spheres) {
+    dp_free_problem(out);
+    dp_set_error("out of host memory");
+    return DP_ERR_NOMEM;
+  }
+  if (!boxes.empty()) std::memcpy(out->boxes, boxes.data(), sizeof(double) * boxes.size());
+  if (!spheres.empty()) std::memcpy(out->spheres, spheres.data(), sizeof(double) * spheres.size());
+  return DP_OK;
+}
+
+void dp_free_problem(dp_problem* p) {
+  if (!p) return;
+  std::free(p->boxes);
+  std::free(p->spheres);
+  p->boxes = nullptr;
+  p->spheres = nullptr;
+  p->n_boxes = p->n_spheres = 0;
+}
+
+int dp_save_problem(const char* path, const dp_problem* p) {
+  if (!path || !p || p->nx <= 0 || p->nx > DP_MAX_NX || (p->n_boxes && !p->boxes) || (p->n_spheres && !p->spheres)) {
+    dp_set_error("invalid argument");
+    return DP_ERR_ARG;
+  }
+  const int dims = p->system == DP_SYS_QUADROTOR ? 3 : 2;
+  std::ostringstream o;
+  o.precision(17);
+  auto vec = [&](const double* v, int n) {
+    o << "[";
+    for (int k = 0; k < n; ++k) o << (k ? ", " : "") << v[k];
+    o << "]";
+  };
+  o << "environment:\n  min: ";
+  vec(p->lo, dims);
+  o << "\n  max: ";
+  vec(p->hi, dims);
+  o << "\n  obstacles:\n";
+  for (int32_t i = 0; i < p->n_boxes; ++i) {
+    const double* b = p->boxes + 6 * i;
+    double size[3] = {2.0 * b[3], 2.0 * b[4], 2.0 * b[5]};
+    o << "    - type: box\n      center: ";
+    vec(b, dims);
+    o << "\n      size: ";
+    vec(size, dims);
+    o << "\n";
+  }
+  for (int32_t i = 0; i < p->n_spheres; ++i) {
+    const double* s = p->spheres + 4 * i;
+    o << "    - type: sphere\n      center: ";
+    vec(s, 3);
+    o << "\n      size: ";
+    vec(s + 3, 1);
+    o << "\n";
+  }
+  o << "robots:\n  - type: " << (p->robot_type[0] ? p->robot_type : "unicycle1_v0") << "\n    start: ";
+  vec(p->start, p->nx);
+  o << "\n    goal: ";
+  vec(p->goal, p->nx);
+  o << "\n";
+  std::ofstream f(path, std::ios::binary);
+  const std::string data = o.str();
+  if (!f || !f.write(data.data(), (std::streamsize)data.size())) {
+    dp_set_error(std::string("cannot write ") + path);
+    return DP_ERR_IO;
+  }
+  return DP_OK;
+}
+
 }  // extern "C"

@@ -130,12 +130,8 @@ __device__ __forceinline__ bool grid_cell_any(const dp_dev_sys& S, const dp_grid
   const int64_t cell = dp_cell_neighbor(G, c, cq);
   if (cell < 0) return false;
   const int32_t e = start[cell + 1];
-  for (int32_t p = start[cell]; p < e; ++p) {
-    double y[NX];
-#pragma unroll
-    for (int k = 0; k < NX; ++k) y[k] = pts[(int64_t)k * stride + p];
-    if (dp_within(S, x, y, r)) return true;
-  }
+  for (int32_t p = start[cell]; p < e; ++p)
+    if (dp_within_col<NX>(S, x, pts, stride, p, r)) return true;
   return false;
 }
 

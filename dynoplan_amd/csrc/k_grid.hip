@@ -142,10 +142,7 @@ __device__ __forceinline__ void scan_earlier(const dp_dev_sys& S, const dp_grid_
     for (int32_t t = cstart[cell]; t < e; ++t) {
       const int32_t o = cidx[t];
       if (o >= q) continue;
-      double y[NX];
-#pragma unroll
-      for (int k = 0; k < NX; ++k) y[k] = cx[(int64_t)k * pcap + t];
-      if (!dp_within(S, x, y, r)) continue;
+      if (!dp_within_col<NX>(S, x, cx, pcap, t, r)) continue;
       if (!f(o)) return;
     }
   }

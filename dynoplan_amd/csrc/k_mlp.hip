@@ -642,8 +642,6 @@ int dp_surrogate_train(dp_ctx* c, const double* nodes, int32_t N, const dp_expan
     DP_HIP_RET(hipMalloc(&m.loss, sizeof(float) * 2));
   }
   const int64_t steps_per_epoch = (npairs + batch - 1) / batch;
-  const double total_steps = (double)steps_per_epoch * epochs;
-  int64_t step = 0;
   for (int32_t e = 0; e < epochs; ++e) {
     DP_HIP_RET(hipMemsetAsync(m.loss + 1, 0, sizeof(float), s));
     for (int64_t t = 0; t < steps_per_epoch; ++t) {
@@ -654,10 +652,7 @@ int dp_surrogate_train(dp_ctx* c, const double* nodes, int32_t N, const dp_expan
       k_gather_labels<<<dp_grid_blocks(batch, 256, 256), 256, 0, s>>>(m.sid, m.slabel, batch, m.zb + 2 * batch);
       rc = dp_mlp_fwd_bwd(c, m.xb, m.zb + 2 * batch, batch, m.sid, m.sprim, m.loss + 1, s);
       if (rc) return rc;
-      // learning rate decays linearly to 10% over the run
-      const float lr_t = (float)(lr * (1.0 - 0.9 * (double)step / total_steps));
-      ++step;
-      if ((rc = adam_step(c, lr_t, s))) return rc;
+      if ((rc = adam_step(c, lr, s))) return rc;
     }
     if (loss_out) {
       float h[2];

@@ -448,12 +448,7 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
         const int64_t cell = dp_cell_neighbor(a.grid, cc, cq);
         if (cell < 0) continue;
         const int32_t e = a.vstart[cell + 1];
-        for (int32_t p = a.vstart[cell]; p < e && !dupv; ++p) {
-          double y[13];
-#pragma unroll
-          for (int c = 0; c < 13; ++c) y[c] = a.vis[(int64_t)c * a.vstride + p];
-          dupv = dp_within(S, x, y, a.r_dup);
-        }
+        for (int32_t p = a.vstart[cell]; p < e && !dupv; ++p) dupv = dp_within_col<13>(S, x, a.vis, a.vstride, p, a.r_dup);
       }
     }
 #pragma unroll

@@ -1,5 +1,6 @@
-"""Primitive files through the C-ABI loader (dp_load_primitives /
-dp_save_primitives): returns plain numpy arrays in the dp_prims layout."""
+"""Primitive and problem files through the C-ABI (dp_load_primitives /
+dp_save_primitives, dp_load_problem / dp_save_problem): plain numpy arrays in
+the layouts of include/dynoplan_amd.h."""
 import ctypes as C
 
 import numpy as np
@@ -39,3 +40,44 @@ def save_primitives(path, offsets, states, actions, cost=None, fmt=N.FMT_AUTO):
         cost = np.ascontiguousarray(cost, np.float64)
         ps.cost = cost.ctypes.data_as(C.POINTER(C.c_double))
     N.check(lib.dp_save_primitives(str(path).encode(), fmt, C.byref(ps)))
+
+
+def load_problem(path):
+    """dynobench problem YAML -> dict(system, robot_type, start, goal, lo, hi,
+    boxes [n][6] centre + half sizes, spheres [n][4])."""
+    lib = N.load()
+    p = N.Problem()
+    N.check(lib.dp_load_problem(str(path).encode(), C.byref(p)))
+    try:
+        nx = p.nx
+        out = {
+            "system": p.system, "robot_type": p.robot_type.decode(), "start": np.array(p.start[:nx]),
+            "goal": np.array(p.goal[:nx]), "lo": np.array(p.lo[:]), "hi": np.array(p.hi[:]),
+            "boxes": np.ctypeslib.as_array(p.boxes, (max(p.n_boxes, 1) * 6,))[: p.n_boxes * 6].reshape(-1, 6).copy(),
+            "spheres": np.ctypeslib.as_array(p.spheres, (max(p.n_spheres, 1) * 4,))[: p.n_spheres * 4]
+            .reshape(-1, 4).copy(),
+        }
+    finally:
+        lib.dp_free_problem(C.byref(p))
+    return out
+
+
+def save_problem(path, system, robot_type, start, goal, lo, hi, boxes=None, spheres=None):
+    lib = N.load()
+    p = N.Problem()
+    p.system = system
+    start = np.asarray(start, np.float64)
+    p.nx = start.size
+    p.robot_type = robot_type.encode()
+    for k in range(start.size):
+        p.start[k] = float(start[k])
+        p.goal[k] = float(goal[k])
+    for k in range(3):
+        p.lo[k] = float(lo[k]) if k < len(lo) else 0.0
+        p.hi[k] = float(hi[k]) if k < len(hi) else 0.0
+    boxes = np.ascontiguousarray(boxes if boxes is not None else np.zeros((0, 6)), np.float64).reshape(-1, 6)
+    spheres = np.ascontiguousarray(spheres if spheres is not None else np.zeros((0, 4)), np.float64).reshape(-1, 4)
+    p.n_boxes, p.n_spheres = boxes.shape[0], spheres.shape[0]
+    p.boxes = boxes.ctypes.data_as(C.POINTER(C.c_double))
+    p.spheres = spheres.ctypes.data_as(C.POINTER(C.c_double))
+    N.check(lib.dp_save_problem(str(path).encode(), C.byref(p)))

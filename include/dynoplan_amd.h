@@ -247,6 +247,27 @@ void dp_free_primitives(dp_prim_set* set);
  * main_primitives' bintoyaml / yamltobin modes, main_primitives.cpp:4-20). */
 int dp_save_primitives(const char* path, int32_t format, const dp_prim_set* set);
 
+/* ---- problem files (dynobench problem YAML; benchmark.py:330-350) ------------
+ * environment: {min, max, obstacles: [{type: box, center, size (full side
+ * lengths)} | {type: sphere, center, size: [radius]}]}, robots: [{type,
+ * start, goal}] (the first robot is read).  `world` is ready for
+ * dp_upload_world (boxes as centre + half sizes); dp_free_problem releases
+ * its arrays. */
+typedef struct dp_problem {
+  int32_t system;                  /* DP_SYS_* from the robot type's prefix */
+  int32_t nx;
+  char robot_type[64];             /* e.g. unicycle1_v0 */
+  double start[DP_MAX_NX], goal[DP_MAX_NX];
+  double lo[3], hi[3];             /* environment min / max */
+  int32_t n_boxes, n_spheres;
+  double* boxes;                   /* [n_boxes][6] cx cy cz hx hy hz */
+  double* spheres;                 /* [n_spheres][4] cx cy cz r */
+} dp_problem;
+
+int dp_load_problem(const char* path, dp_problem* out);
+void dp_free_problem(dp_problem* p);
+int dp_save_problem(const char* path, const dp_problem* p);
+
 /* ---- learned collision-feasibility surrogate --------------------------------
  * No counterpart in the reference.  An MLP over a 32x32 occupancy patch
  * (0.1 m cells) around the node and a learned 64-d primitive embedding
@@ -264,8 +285,7 @@ typedef struct dp_surrogate_stats {
 
 int dp_surrogate_init(dp_ctx* ctx, uint64_t seed);
 /* Exact-checker labels (FREE = 1) of the pairs of `nodes`, then `epochs`
- * passes of minibatch Adam (learning rate `lr` decaying linearly to lr/10);
- * loss_out[epochs] (optional) gets the epoch losses. */
+ * passes of minibatch Adam; loss_out[epochs] (optional) gets the epoch losses. */
 int dp_surrogate_train(dp_ctx* ctx, const double* nodes, int32_t N, const dp_expand_params* params, int32_t epochs,
                        int32_t batch, float lr, uint64_t seed, float* loss_out);
 /* Threshold = lowest score of a FREE pair of held-out `nodes` minus margin. */

@@ -155,7 +155,7 @@ def test_train_calibrate_recall_and_gain(need_gpu):
     wl = synth.car_trailer(P=1500, N_nodes=512, n_boxes=150, extent=14.0, n_visited=5000)
     rng = np.random.default_rng(99)
     extra = lambda s: s.__setitem__((slice(None), 3), synth.wrap_angle(s[:, 2] + rng.uniform(-0.5, 0.5, s.shape[0])))
-    train_nodes = synth.free_planar_nodes(rng, wl.desc, wl.boxes, wl.lo + 0.5, wl.hi - 0.5, 4096, extra)
+    train_nodes = synth.free_planar_nodes(rng, wl.desc, wl.boxes, wl.lo + 0.5, wl.hi - 0.5, 16384, extra)
     cal_nodes = synth.free_planar_nodes(rng, wl.desc, wl.boxes, wl.lo + 0.5, wl.hi - 0.5, 2048, extra)
     p0 = synth.make_params(goal=wl.params.goal[:4], limit_branching_factor=20)
     n = int(N.load().dp_surrogate_param_count())
