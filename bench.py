@@ -278,6 +278,8 @@ def main():
     ap.add_argument("--label-pairs", type=int, default=2_000_000, help="surrogate training labels (car_trailer)")
     ap.add_argument("--no-surrogate", action="store_true", help="car_trailer without the learned pre-filter")
     ap.add_argument("--no-baselines", action="store_true", help="skip CPU and plain-GPU legs (profiling runs)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (scale reference for HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -306,6 +308,16 @@ def main():
 
     ex.upload_nodes(wl.nodes, wl.g_parent)
     ex.reserve(wl.nodes.shape[0], wl.nodes.shape[0] * (wl.offsets.size - 1))
+    if args.copy_mib > 0:
+        # a copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel give
+        # the scale of the counters read for the expansion kernels
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        progress(f"copied {args.copy_mib} MiB three times")
+        del src, dst
     for _ in range(args.warmup):
         ex.expand_async(params)
         torch.cuda.synchronize()

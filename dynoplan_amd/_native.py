@@ -145,6 +145,33 @@ class SearchInfo(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+IDB_MAX_IT, IDB_TIME_LIMIT, IDB_MAX_SOLUTION = 0, 1, 2
+
+
+class IdbOptions(C.Structure):
+    _fields_ = [("delta_0", C.c_double), ("num_primitives_0", C.c_int64), ("delta_rate", C.c_double),
+                ("num_primitives_rate", C.c_double), ("timelimit_s", C.c_double), ("max_it", C.c_int32),
+                ("max_num_sol", C.c_int32), ("add_primitives", C.c_int32), ("cuts", C.c_int32), ("seed", C.c_uint64)]
+
+
+class IdbInfo(C.Structure):
+    _fields_ = [("solved", C.c_int32), ("exit_criteria", C.c_int32), ("iterations", C.c_int32),
+                ("solutions", C.c_int32), ("cost", C.c_double), ("last_delta", C.c_double),
+                ("last_primitives", C.c_int64), ("primitives_added", C.c_int64), ("expands", C.c_int64),
+                ("time_ms", C.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class TileConfig(C.Structure):
+    _fields_ = [("lanes_per_pair", C.c_int32), ("check_blocks", C.c_int32), ("quad32_blocks", C.c_int32),
+                ("quad64_threads", C.c_int32), ("quad64_blocks", C.c_int32), ("check_ms", C.c_float)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 # name -> (restype, argtypes); every symbol include/dynoplan_amd.h declares.
 SIGNATURES = {
     "dp_last_error": (C.c_char_p, []),
@@ -173,9 +200,17 @@ SIGNATURES = {
     "dp_search": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(SearchOptions),
                             C.POINTER(SearchInfo)]),
     "dp_search_trajectory": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "dp_search_trajectory_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "dp_load_problem": (C.c_int, [C.c_char_p, C.POINTER(Problem)]),
     "dp_free_problem": (None, [C.POINTER(Problem)]),
     "dp_save_problem": (C.c_int, [C.c_char_p, C.POINTER(Problem)]),
+    "dp_idbastar_options_default": (None, [C.POINTER(IdbOptions)]),
+    "dp_idbastar": (C.c_int, [C.c_void_p, C.POINTER(Prims), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                              C.POINTER(SearchOptions), C.POINTER(IdbOptions), C.POINTER(IdbInfo)]),
+    "dp_set_tiles": (C.c_int, [C.c_void_p, C.POINTER(TileConfig)]),
+    "dp_get_tiles": (C.c_int, [C.c_void_p, C.POINTER(TileConfig)]),
+    "dp_autotune": (C.c_int, [C.c_void_p, C.POINTER(ExpandParams), C.POINTER(TileConfig), C.POINTER(TileConfig),
+                              C.POINTER(C.c_int32)]),
     "dp_batch_create": (C.c_void_p, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32]),
     "dp_batch_destroy": (None, [C.c_void_p]),
     "dp_batch_expand": (C.c_int, [C.c_void_p, C.POINTER(ExpandParams), C.c_void_p]),

@@ -796,4 +796,74 @@ int dp_last_timing(dp_ctx* c, float* ms5) {
   return DP_OK;
 }
 
+int dp_set_tiles(dp_ctx* c, const dp_tile_config* t) {
+  ARG_CHECK(c && t, "null argument");
+  const int l = t->lanes_per_pair, q = t->quad64_threads;
+  ARG_CHECK(l == 0 || l == 16 || l == 32 || l == 64, "lanes_per_pair must be 0, 16, 32 or 64");
+  ARG_CHECK(q == 64 || q == 128 || q == 256, "quad64_threads must be 64, 128 or 256");
+  ARG_CHECK(t->check_blocks >= 1 && t->check_blocks <= 65535 && t->quad32_blocks >= 1 &&
+                t->quad32_blocks <= 65535 && t->quad64_blocks >= 1 && t->quad64_blocks <= 65535,
+            "block counts must be in [1, 65535]");
+  c->tiles = *t;
+  dp_bump_epoch();
+  return DP_OK;
+}
+
+int dp_get_tiles(const dp_ctx* c, dp_tile_config* t) {
+  ARG_CHECK(c && t, "null argument");
+  *t = c->tiles;
+  return DP_OK;
+}
+
+int dp_autotune(dp_ctx* c, const dp_expand_params* p, dp_tile_config* best, dp_tile_config* trials,
+                int32_t* n_trials) {
+  ARG_CHECK(c && p && best, "null argument");
+  std::vector<dp_tile_config> cand;
+  const dp_tile_config base = c->tiles;
+  if (c->desc.system == DP_SYS_QUADROTOR) {
+    for (int q32 : {2048, 4096, 8192})
+      for (int t64 : {64, 128, 256})
+        for (int b64 : {4096, 8192, 16384}) {
+          dp_tile_config t = base;
+          t.quad32_blocks = q32;
+          t.quad64_threads = t64;
+          t.quad64_blocks = b64;
+          cand.push_back(t);
+        }
+  } else {
+    for (int l : {16, 32, 64})
+      for (int b : {1024, 2048, 4096, 8192}) {
+        dp_tile_config t = base;
+        t.lanes_per_pair = l;
+        t.check_blocks = b;
+        cand.push_back(t);
+      }
+  }
+  int best_i = -1;
+  for (size_t i = 0; i < cand.size(); ++i) {
+    int rc = dp_set_tiles(c, &cand[i]);
+    if (rc) return rc;
+    float ms[5], runs[3];
+    for (int r = -1; r < 3; ++r) {
+      if ((rc = dp_expand(c, p, nullptr))) return rc;
+      if ((rc = dp_last_timing(c, ms))) return rc;
+      if (r >= 0) runs[r] = ms[1];
+    }
+    std::sort(runs, runs + 3);
+    cand[i].check_ms = runs[1];
+    if (best_i < 0 || cand[i].check_ms < cand[best_i].check_ms) best_i = (int)i;
+  }
+  c->tiles = cand[best_i];
+  dp_bump_epoch();
+  *best = cand[best_i];
+  if (trials && n_trials) {
+    const int32_t n = (int32_t)std::min<size_t>(cand.size(), 64);
+    for (int32_t i = 0; i < n; ++i) trials[i] = cand[i];
+    *n_trials = n;
+  }
+  // the counters of the last timed run stay valid; finish its resolution
+  dp_counts cnt;
+  return dp_get_counts(c, &cnt);
+}
+
 }  // extern "C"

@@ -214,6 +214,7 @@ struct dp_ctx {
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   bool timing_valid = false;
   bool no_events = false;           // set while a multi-instance graph is captured
+  dp_tile_config tiles{0, 2048, 4096, 256, 8192, 0.0f};  // launch shapes (dp_autotune)
   int64_t last_pairs = -1;
 };
 

@@ -442,12 +442,13 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
     c->timing_valid = !c->no_events;
     return DP_OK;
   }
-  const int lpp = c->maxlen <= 16 ? 16 : (c->maxlen <= 32 ? 32 : 64);
+  const int lpp = c->tiles.lanes_per_pair > 0 ? c->tiles.lanes_per_pair
+                                             : (c->maxlen <= 16 ? 16 : (c->maxlen <= 32 ? 32 : 64));
   const int seg = 64 / lpp;
   const size_t lds = sizeof(double) * 4 * c->nb + sizeof(int16_t) * (size_t)kWaves * seg * (c->nb > 0 ? c->nb : 1);
   // The pair count is on the device; size the grid from the bound N*P.
   const int64_t groups = ((int64_t)N * c->P + seg - 1) / seg;
-  const int blocks = dp_grid_blocks(groups, kWaves, 2048);
+  const int blocks = dp_grid_blocks(groups, kWaves, c->tiles.check_blocks);
   rc = dispatch_nx(nx, [&](auto NXc) -> int {
     constexpr int NX = decltype(NXc)::value;
     if constexpr (NX == 13) {

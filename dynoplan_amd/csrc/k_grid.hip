@@ -257,6 +257,12 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
     const bool dupv = (fl & 1) != 0;
     bool ext_dup = false, ext_block = false;
     uint64_t same = 0;
+    // undecided earlier-chunk contenders seen by the first evaluation (the
+    // first kBlk; `more` when there were others)
+    constexpr int kBlk = 4;
+    int32_t blk_o[kBlk];
+    int nblk = 0;
+    bool more = false;
     // Earlier contenders of pair p: those in this chunk go to `same` (the
     // walk below decides them in order); an earlier-chunk contender already
     // NOVEL makes p a duplicate, one still undecided blocks p.  With
@@ -273,8 +279,13 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
           __builtin_amdgcn_s_sleep(1);
           r = __hip_atomic_load(&a.res[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        if (r == DP_R_NOVEL) ext_dup = true;
-        else if (r == DP_R_UNDECIDED) ext_block = true;
+        if (r == DP_R_NOVEL) {
+          ext_dup = true;
+        } else if (r == DP_R_UNDECIDED) {
+          ext_block = true;
+          if (nblk < kBlk) blk_o[nblk++] = o;
+          else more = true;
+        }
         return !ext_dup;
       };
       const int cnt = a.conf_cnt[p];
@@ -309,11 +320,25 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
         // waiting for its undecided contenders.
         bool d = false, blk = false;
         if (a.wait > 0 && lane == b) {
-          ext_dup = false;
-          ext_block = false;
-          eval(p, a.wait);
-          d = ext_dup;
-          blk = ext_block && !ext_dup;
+          if (!more) {
+            // wait for the recorded contenders only
+            for (int e = 0; e < nblk && !d && !blk; ++e) {
+              int r = __hip_atomic_load(&a.res[blk_o[e]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              for (int32_t w = 0; r == DP_R_UNDECIDED && w < a.wait; ++w) {
+                __builtin_amdgcn_s_sleep(1);
+                r = __hip_atomic_load(&a.res[blk_o[e]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              }
+              if (r == DP_R_NOVEL) d = true;
+              else if (r == DP_R_UNDECIDED) blk = true;
+            }
+          } else {
+            ext_dup = false;
+            ext_block = false;
+            nblk = 0;
+            eval(p, a.wait);
+            d = ext_dup;
+            blk = ext_block && !ext_dup;
+          }
         }
         const bool dd = __ballot(d) != 0;
         if (a.wait <= 0 || __ballot(blk) != 0) {
@@ -503,13 +528,11 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
   if (rc) return rc;
   c->round_limit = p->limit_branching_factor;
   c->round_r_dup = r_dup;
-  // Conflict lists are stored for the first `window` pairs of each node; a
-  // node's cursor passes that point only when the branching cap has not been
-  // reached (positions of the cap on the reference workloads: median 22-87,
-  // 90th percentile 28-260).  Without a cap every pair is covered.
-  const int32_t window = p->limit_branching_factor > 0
-                             ? (c->desc.system == DP_SYS_QUADROTOR ? 64 : 256)
-                             : 0;
+  // Conflict lists are stored for every pair of planar batches, and for the
+  // first 64 pairs of each node for the quadrotor, whose cap is usually
+  // reached earlier (median position 22) and whose FP64 metric makes the
+  // lists dear; later quadrotor pairs scan the grid when reached.
+  const int32_t window = p->limit_branching_factor > 0 && c->desc.system == DP_SYS_QUADROTOR ? 64 : 0;
   const int pb = dp_grid_blocks(c->pair_cap, 256, 8192);
   rc = dispatch_nx(c->desc.nx, [&](auto NXc) -> int {
     constexpr int NX = decltype(NXc)::value;

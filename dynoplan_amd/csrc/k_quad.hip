@@ -538,9 +538,10 @@ int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, doub
     k_quad_chunks<<<dp_grid_blocks(c->N, 256, 1024), 256, 0, s>>>(c->N, c->d_app, c->d_total, c->d_chunk);
     dp_scan_small(c->d_chunk, c->N, nullptr, INT64_MAX, c->d_counters, -1, s);
     const int64_t max_chunks = (int64_t)c->N * ((c->P + 63) / 64);
-    k_quad32<<<dp_grid_blocks(max_chunks, kQWaves, 4096), 256, lds, s>>>(a);
+    k_quad32<<<dp_grid_blocks(max_chunks, kQWaves, c->tiles.quad32_blocks), 256, lds, s>>>(a);
   }
-  k_quad64<<<dp_grid_blocks(c->pair_cap, 256, 8192), 256, 0, s>>>(a);
+  const int t64 = c->tiles.quad64_threads;
+  k_quad64<<<dp_grid_blocks(c->pair_cap, t64, c->tiles.quad64_blocks), t64, 0, s>>>(a);
   DP_HIP_RET(hipGetLastError());
   return DP_OK;
 }

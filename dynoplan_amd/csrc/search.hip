@@ -431,4 +431,163 @@ int dp_search_trajectory(dp_ctx* c, double* states, double* actions) {
   return DP_OK;
 }
 
+int dp_search_trajectory_size(const dp_ctx* c, int64_t* n_states, int64_t* n_actions) {
+  if (!c || !n_states || !n_actions) {
+    dp_set_error("null argument");
+    return DP_ERR_ARG;
+  }
+  *n_states = (int64_t)(c->sol_states.size() / c->desc.nx);
+  *n_actions = c->sol_nu ? (int64_t)(c->sol_actions.size() / c->sol_nu) : 0;
+  return DP_OK;
+}
+
+void dp_idbastar_options_default(dp_idbastar_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->delta_0 = 0.3;  // idbastar.hpp:48-59
+  o->num_primitives_0 = 1000;
+  o->delta_rate = 0.9;
+  o->num_primitives_rate = 1.5;
+  o->timelimit_s = 10.0;
+  o->max_it = 10;
+  o->max_num_sol = 5;
+  o->add_primitives = 1;
+  o->cuts = 5;
+  o->seed = 1;
+}
+
+int dp_idbastar(dp_ctx* c, const dp_prims* lib, const double* start, const double* goal,
+                const dp_search_options* sopt, const dp_idbastar_options* o, dp_idbastar_info* info) {
+  if (!c || !lib || !lib->offsets || !lib->states || !start || !goal || !sopt || !o || !info || lib->P <= 0 ||
+      o->cuts < 1 || o->num_primitives_0 < 1) {
+    dp_set_error("invalid argument");
+    return DP_ERR_ARG;
+  }
+  const int nx = c->desc.nx;
+  const int nu = lib->actions ? lib->nu : 0;
+  if (lib->nx != nx) {
+    dp_set_error("primitive nx does not match the system");
+    return DP_ERR_ARG;
+  }
+  std::memset(info, 0, sizeof(*info));
+  info->cost = 1e8;
+  const double t0 = now_ms();
+  // motions inserted from solutions, newest first; each is states [L1][nx] + actions [L1-1][nu]
+  std::vector<std::vector<double>> add_st, add_ac;
+  std::vector<double> best_states, best_actions;
+  uint64_t rng = o->seed ? o->seed : 1;
+  auto unif = [&]() {  // U[-1, 1] from a splitmix64 stream
+    uint64_t z = (rng += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+  };
+  const dp_dev_sys& S = c->hsys;
+  double delta = o->delta_0;
+  double max_motions = (double)o->num_primitives_0;
+  bool solved_db = false;
+  int rc;
+  for (int32_t it = 0;; ++it) {
+    if (it > 0) {  // idbastar.cpp:128-146 (new_schedule)
+      delta *= solved_db ? o->delta_rate : 0.9999;
+      max_motions *= o->num_primitives_rate;
+    }
+    // primitive set of this iteration: inserted motions, then the library
+    std::vector<int32_t> off{0};
+    std::vector<double> st, ac;
+    for (size_t m = 0; m < add_st.size(); ++m) {
+      st.insert(st.end(), add_st[m].begin(), add_st[m].end());
+      ac.insert(ac.end(), add_ac[m].begin(), add_ac[m].end());
+      off.push_back(off.back() + (int32_t)(add_st[m].size() / nx));
+    }
+    for (int32_t j = 0; j < lib->P; ++j) {
+      const int32_t a = lib->offsets[j], b = lib->offsets[j + 1];
+      st.insert(st.end(), lib->states + (int64_t)a * nx, lib->states + (int64_t)b * nx);
+      if (nu) ac.insert(ac.end(), lib->actions + (int64_t)(a - j) * nu, lib->actions + (int64_t)(b - j - 1) * nu);
+      off.push_back(off.back() + (b - a));
+    }
+    dp_prims pr;
+    std::memset(&pr, 0, sizeof(pr));
+    pr.P = (int32_t)off.size() - 1;
+    pr.nx = nx;
+    pr.nu = nu;
+    pr.offsets = off.data();
+    pr.states = st.data();
+    pr.actions = nu ? ac.data() : nullptr;
+    const int32_t use = (int32_t)std::min<double>(max_motions, (double)pr.P);
+    if ((rc = dp_upload_primitives(c, &pr, use))) return rc;
+
+    dp_search_options so = *sopt;
+    so.delta = (float)delta;
+    dp_search_info si;
+    if ((rc = dp_search(c, start, goal, &so, &si))) return rc;
+    solved_db = si.solved != 0;
+    info->iterations = it + 1;
+    info->last_delta = delta;
+    info->last_primitives = use;
+    info->expands += si.expands;
+    if (solved_db) {
+      ++info->solutions;
+      info->solved = 1;
+      if (si.cost < info->cost) {
+        info->cost = si.cost;
+        best_states = c->sol_states;
+        best_actions = c->sol_actions;
+      }
+      const int64_t ns = (int64_t)(c->sol_states.size() / nx);
+      if (o->add_primitives && nu && ns >= 2) {
+        // cut into pieces of about equal numbers of actions, canonical, noise
+        const int64_t na = ns - 1;
+        const int32_t cuts = (int32_t)std::min<int64_t>(o->cuts, na);
+        std::vector<std::vector<double>> pst, pac;
+        for (int32_t k = 0; k < cuts; ++k) {
+          const int64_t a0 = na * k / cuts, a1 = na * (k + 1) / cuts;
+          std::vector<double> ps(c->sol_states.begin() + a0 * nx, c->sol_states.begin() + (a1 + 1) * nx);
+          std::vector<double> pa(c->sol_actions.begin() + a0 * nu, c->sol_actions.begin() + a1 * nu);
+          double off0[DP_MAX_NX];
+          for (int d = 0; d < nx; ++d) off0[d] = ps[d];
+          for (int64_t r = 0; r <= a1 - a0; ++r)
+            for (int d = 0; d < nx; ++d)
+              if (dp_is_offset_dim(S, d)) ps[(size_t)r * nx + d] -= off0[d];
+          for (int64_t r : {(int64_t)0, a1 - a0}) {
+            for (int d = 0; d < nx; ++d) ps[(size_t)r * nx + d] += 1e-7 * unif();
+            if (S.system == DP_SYS_QUADROTOR) {  // keep the quaternion unit
+              double* q = &ps[(size_t)r * nx + 3];
+              const double n = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+              for (int e = 0; e < 4; ++e) q[e] /= n;
+            }
+          }
+          pst.push_back(std::move(ps));
+          pac.push_back(std::move(pa));
+        }
+        add_st.insert(add_st.begin(), pst.begin(), pst.end());
+        add_ac.insert(add_ac.begin(), pac.begin(), pac.end());
+        info->primitives_added += cuts;
+      }
+    }
+    // exit criteria, idbastar.cpp:300-313 (a later one overrides an earlier)
+    bool finished = false;
+    if (it + 1 >= o->max_it) {
+      finished = true;
+      info->exit_criteria = DP_IDB_MAX_IT;
+    }
+    if (info->solutions >= o->max_num_sol) {
+      finished = true;
+      info->exit_criteria = DP_IDB_MAX_SOLUTION;
+    }
+    if ((now_ms() - t0) / 1000.0 > o->timelimit_s) {
+      finished = true;
+      info->exit_criteria = DP_IDB_TIME_LIMIT;
+    }
+    if (finished) break;
+  }
+  if (info->solved) {
+    c->sol_states = best_states;
+    c->sol_actions = best_actions;
+  }
+  info->time_ms = now_ms() - t0;
+  return DP_OK;
+}
+
 }  // extern "C"

@@ -120,6 +120,27 @@ class GpuExpander:
         N.check(self.lib.dp_last_timing(self.ctx, ms))
         return {"applicability": ms[0], "check": ms[1], "resolve": ms[2], "total": ms[3], "surrogate": ms[4]}
 
+    def tiles(self):
+        t = N.TileConfig()
+        N.check(self.lib.dp_get_tiles(self.ctx, C.byref(t)))
+        return t.as_dict()
+
+    def set_tiles(self, **kw):
+        t = N.TileConfig()
+        N.check(self.lib.dp_get_tiles(self.ctx, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        N.check(self.lib.dp_set_tiles(self.ctx, C.byref(t)))
+
+    def autotune(self, params):
+        """Pick the fastest launch shape for the uploaded nodes; returns
+        (best, all trials) as dicts."""
+        best = N.TileConfig()
+        trials = (N.TileConfig * 64)()
+        n = C.c_int32()
+        N.check(self.lib.dp_autotune(self.ctx, C.byref(params), C.byref(best), trials, C.byref(n)))
+        return best.as_dict(), [trials[i].as_dict() for i in range(n.value)]
+
     def stream(self):
         return self.lib.dp_stream(self.ctx)
 
@@ -165,6 +186,49 @@ class GpuExpander:
         N.check(self.lib.dp_search_trajectory(self.ctx, _dptr(states) if states.size else None,
                                               _dptr(actions) if actions.size else None))
         return d, states, actions
+
+    def idbastar(self, offsets, states, actions, start, goal, search=None, **options):
+        """The iDb-A* schedule over a primitive library; returns (info dict,
+        best states, best actions)."""
+        offsets = np.ascontiguousarray(offsets, np.int32)
+        states = np.ascontiguousarray(states, np.float64)
+        actions = np.ascontiguousarray(actions, np.float64)
+        pr = N.Prims()
+        pr.P = offsets.size - 1
+        pr.nx = states.shape[1]
+        pr.nu = actions.shape[1]
+        pr.offsets = _iptr(offsets)
+        pr.states = _dptr(states)
+        pr.actions = _dptr(actions)
+        so = N.SearchOptions()
+        self.lib.dp_search_options_default(C.byref(so))
+        for k, v in (search or {}).items():
+            setattr(so, k, v)
+        io = N.IdbOptions()
+        self.lib.dp_idbastar_options_default(C.byref(io))
+        for k, v in options.items():
+            setattr(io, k, v)
+        start = np.ascontiguousarray(start, np.float64)
+        goal = np.ascontiguousarray(goal, np.float64)
+        info = N.IdbInfo()
+        N.check(self.lib.dp_idbastar(self.ctx, C.byref(pr), _dptr(start), _dptr(goal), C.byref(so), C.byref(io),
+                                     C.byref(info)))
+        d = info.as_dict()
+        # the best trajectory is the context's last trajectory
+        st = np.zeros((0, self.nx))
+        ac = np.zeros((0, self.desc.nu))
+        if d["solved"]:
+            n_s, n_a = self._traj_sizes()
+            st = np.zeros((n_s, self.nx))
+            ac = np.zeros((n_a, self.desc.nu))
+            N.check(self.lib.dp_search_trajectory(self.ctx, _dptr(st) if st.size else None,
+                                                  _dptr(ac) if ac.size else None))
+        return d, st, ac
+
+    def _traj_sizes(self):
+        a, b = C.c_int64(), C.c_int64()
+        N.check(self.lib.dp_search_trajectory_size(self.ctx, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def fetch(self):
         n = self.counts()["pairs"]

@@ -218,6 +218,25 @@ void* dp_stream(dp_ctx* ctx);
  * 2 duplicate resolution, 3 total, 4 learned pre-filter (0 without it). */
 int dp_last_timing(dp_ctx* ctx, float* ms5);
 
+/* Launch shapes of the check stage.  Every configuration gives identical
+ * results; dp_autotune picks the fastest for the uploaded batch. */
+typedef struct dp_tile_config {
+  int32_t lanes_per_pair;   /* planar check: lanes per pair, 16, 32 or 64 (0: from the longest primitive) */
+  int32_t check_blocks;     /* planar check: workgroups (grid-stride) */
+  int32_t quad32_blocks;    /* quadrotor FP32 pass: workgroups */
+  int32_t quad64_threads;   /* quadrotor FP64 pass: threads per workgroup, 64, 128 or 256 */
+  int32_t quad64_blocks;    /* quadrotor FP64 pass: workgroups */
+  float check_ms;           /* dp_autotune: check-stage time of the chosen shape */
+} dp_tile_config;
+
+int dp_set_tiles(dp_ctx* ctx, const dp_tile_config* tiles);
+int dp_get_tiles(const dp_ctx* ctx, dp_tile_config* tiles);
+/* Times the check stage of the uploaded nodes under each candidate shape
+ * (median of 3 after a warm-up), keeps the fastest and returns it; `trials`
+ * (optional, up to 64) receives every candidate with its time. */
+int dp_autotune(dp_ctx* ctx, const dp_expand_params* params, dp_tile_config* best, dp_tile_config* trials,
+                int32_t* n_trials);
+
 /* ---- primitive files (load_motion_primitives_new, src/ompl/robots.cpp:3061-3174) ---- */
 enum { DP_FMT_AUTO = 0, DP_FMT_YAML = 1, DP_FMT_JSON = 2, DP_FMT_MSGPACK = 3, DP_FMT_BOOST = 4 };
 
@@ -351,6 +370,51 @@ int dp_search(dp_ctx* ctx, const double* start, const double* goal, const dp_sea
 /* The last search's trajectory: states [traj_states][nx], actions
  * [traj_actions][nu] (either pointer may be NULL). */
 int dp_search_trajectory(dp_ctx* ctx, double* states, double* actions);
+/* Sizes of that trajectory (after dp_search or dp_idbastar). */
+int dp_search_trajectory_size(const dp_ctx* ctx, int64_t* n_states, int64_t* n_actions);
+
+/* ---- iDb-A* schedule (src/idbastar/idbastar.cpp:128-312) ---------------------
+ * Repeated dp_search runs with delta and the number of primitives scheduled
+ * as the reference's new_schedule: iteration 0 uses delta_0 and
+ * num_primitives_0; later iterations multiply delta by delta_rate after a
+ * solved run (by 0.9999 otherwise) and the primitive count by
+ * num_primitives_rate.  After a solved run the solution trajectory is cut
+ * into `cuts` pieces, made canonical (offset dims of the first state
+ * subtracted), its end states perturbed by 1e-7 U[-1,1] (seeded), and the
+ * pieces are inserted before the library (idbastar.cpp:234-291), then the
+ * primitives are uploaded again.  Trajectory optimisation between the
+ * iterations (crocoddyl in the reference) is not part of this library: the
+ * raw search solution stands in for the optimised one. */
+enum { DP_IDB_MAX_IT = 0, DP_IDB_TIME_LIMIT = 1, DP_IDB_MAX_SOLUTION = 2 };  /* EXIT_CRITERIA */
+
+typedef struct dp_idbastar_options {  /* Options_idbAStar (idbastar.hpp:45-60) */
+  double delta_0;                     /* .3 */
+  int64_t num_primitives_0;           /* 1000 */
+  double delta_rate;                  /* .9 */
+  double num_primitives_rate;         /* 1.5 */
+  double timelimit_s;                 /* 10 */
+  int32_t max_it;                     /* 10 */
+  int32_t max_num_sol;                /* 5 */
+  int32_t add_primitives;             /* 1 */
+  int32_t cuts;                       /* 5 */
+  uint64_t seed;                      /* noise of inserted primitives */
+} dp_idbastar_options;
+
+typedef struct dp_idbastar_info {
+  int32_t solved, exit_criteria;      /* DP_IDB_* */
+  int32_t iterations, solutions;
+  double cost;                        /* best solution cost (gScore), 1e8 if none */
+  double last_delta;
+  int64_t last_primitives, primitives_added, expands;
+  double time_ms;
+} dp_idbastar_info;
+
+void dp_idbastar_options_default(dp_idbastar_options* opt);
+/* `library`: the primitive set (host arrays, as dp_upload_primitives);
+ * `search`: options of every dp_search run (delta is overridden).  The best
+ * trajectory is available through dp_search_trajectory afterwards. */
+int dp_idbastar(dp_ctx* ctx, const dp_prims* library, const double* start, const double* goal,
+                const dp_search_options* search, const dp_idbastar_options* opt, dp_idbastar_info* info);
 
 /* ---- many planning instances in one launch ---------------------------------
  * No counterpart in the reference, which plans one instance per process.  A

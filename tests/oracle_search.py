@@ -159,7 +159,7 @@ def search(wl, start, goal, delta=0.3, alpha=0.5, max_expands=100000, limit_bran
                         continue
                     t = gq + lbt(x, X[j])
                     m = nodes[j]
-                    if t < m["g"]:
+                    if t < float(m["g"]):  # double against the float score widened (not t narrowed)
                         m["g"] = f32(t)
                         m["f"] = f32(t + float(m["h"]))
                         m["parent"] = b

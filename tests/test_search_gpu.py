@@ -90,3 +90,31 @@ def test_search_batched_frontier(need_gpu):
         info2, st2, _ = ex.search(START, GOAL, delta=0.5, max_expands=5, batch_nodes=1)
         assert info2["status"] == N.SEARCH_MAX_EXPANDS and info2["solved"] == 0 and info2["expands"] == 5
         assert st2.shape[0] >= 1
+
+
+def test_idbastar_schedule(need_gpu):
+    """iDb-A* (idbastar.cpp:128-312): the first iteration is the plain search
+    at delta_0 with the first num_primitives_0 primitives; later iterations
+    follow the schedule and reuse solution pieces as primitives."""
+    wl = bugtrap()
+    with expander_for(wl) as ex:
+        one, st1, ac1 = ex.idbastar(wl.offsets, wl.states, wl.actions, START, GOAL, search={"max_expands": 10000},
+                                    delta_0=0.8, num_primitives_0=100, max_it=1, timelimit_s=120.0)
+        ref, rs, ra = ex.search(START, GOAL, delta=0.8, max_expands=10000)
+        assert one["solved"] == ref["solved"] == 1 and one["iterations"] == 1
+        assert one["cost"] == ref["cost"] and one["exit_criteria"] == N.IDB_MAX_IT
+        assert np.array_equal(st1, rs) and np.array_equal(ac1, ra)
+        info, st, ac = ex.idbastar(wl.offsets, wl.states, wl.actions, START, GOAL, search={"max_expands": 10000},
+                                   delta_0=0.8, num_primitives_0=60, num_primitives_rate=1.5, delta_rate=0.9,
+                                   max_it=4, max_num_sol=3, timelimit_s=120.0)
+    print(f"iDb-A*: {info}")
+    assert info["solved"] == 1 and info["cost"] < 100.0
+    k = info["iterations"]
+    assert 1 <= k <= 4
+    assert info["exit_criteria"] in (N.IDB_MAX_IT, N.IDB_MAX_SOLUTION)
+    if info["exit_criteria"] == N.IDB_MAX_SOLUTION:
+        assert info["solutions"] == 3
+    assert info["primitives_added"] == 5 * info["solutions"]
+    assert info["last_primitives"] <= min(int(60 * 1.5 ** (k - 1)), 100 + info["primitives_added"])
+    assert info["last_delta"] <= 0.8
+    check_trajectory(wl, st, ac, 0.8)  # found at some delta <= delta_0
