@@ -136,14 +136,14 @@ def train_surrogate(ex, wl, label_pairs):
     per_node = ex.expand(probe, np.zeros(probe.shape[0]), p0)["pairs"] / probe.shape[0]
     n_train = int(np.ceil(label_pairs / max(per_node, 1e-9)))
     train_nodes = synth.free_planar_nodes(rng, wl.desc, wl.boxes, lo, hi, n_train, extra)
-    cal_nodes = synth.free_planar_nodes(rng, wl.desc, wl.boxes, lo, hi, 4096, extra)
+    cal_nodes = synth.free_planar_nodes(rng, wl.desc, wl.boxes, lo, hi, 8192, extra)
     n_labels = ex.expand(train_nodes, np.zeros(n_train), p0)["pairs"]
     progress(f"surrogate: {n_labels} exact-checker labels from {n_train} nodes")
     ex.surrogate_init(seed=wl.seed)
     t0 = time.perf_counter()
     losses = ex.surrogate_train(train_nodes, p0, epochs=8, batch=4096, lr=2e-3, seed=wl.seed)
     t_train = time.perf_counter() - t0
-    stats = ex.surrogate_calibrate(cal_nodes, p0, margin=0.5)
+    stats = ex.surrogate_calibrate(cal_nodes, p0, margin=0.5, tail=4.0)
     progress(f"surrogate: trained in {t_train:.1f} s, epoch losses {[round(float(x), 4) for x in losses]}, "
              f"calibration {stats}")
     return {"labels": int(n_labels), "train_nodes": n_train, "epochs": 8, "train_s": t_train,

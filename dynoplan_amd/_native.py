@@ -211,6 +211,7 @@ SIGNATURES = {
     "dp_get_tiles": (C.c_int, [C.c_void_p, C.POINTER(TileConfig)]),
     "dp_autotune": (C.c_int, [C.c_void_p, C.POINTER(ExpandParams), C.POINTER(TileConfig), C.POINTER(TileConfig),
                               C.POINTER(C.c_int32)]),
+    "dp_debug_resolve_trace": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64), C.c_int64]),
     "dp_batch_create": (C.c_void_p, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32]),
     "dp_batch_destroy": (None, [C.c_void_p]),
     "dp_batch_expand": (C.c_int, [C.c_void_p, C.POINTER(ExpandParams), C.c_void_p]),
@@ -225,7 +226,7 @@ SIGNATURES = {
     "dp_surrogate_train": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int32, C.POINTER(ExpandParams),
                                      C.c_int32, C.c_int32, C.c_float, C.c_uint64, C.POINTER(C.c_float)]),
     "dp_surrogate_calibrate": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int32, C.POINTER(ExpandParams),
-                                         C.c_float, C.POINTER(SurrogateStats)]),
+                                         C.c_float, C.c_float, C.POINTER(SurrogateStats)]),
     "dp_surrogate_set_threshold": (C.c_int, [C.c_void_p, C.c_float]),
     "dp_surrogate_scores": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int64]),
     "dp_surrogate_param_count": (C.c_int64, []),

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -204,6 +205,8 @@ dp_ctx* dp_create(const dp_system_desc* sys, int32_t device) {
   ok = ok && hipMemset(c->d_total, 0, sizeof(int32_t)) == hipSuccess;
   ok = ok && upload_sys(c) == DP_OK;
   ok = ok && hipHostMalloc((void**)&c->h_pinned, sizeof(long long) * 4) == hipSuccess;
+  if (const char* e = std::getenv("DP_POLL_CAP")) c->poll_cap = std::max(1, std::min(256, std::atoi(e)));
+  if (const char* e = std::getenv("DP_RESOLVE_ROUNDS")) c->resolve_rounds = std::max(0, std::min(4096, std::atoi(e)));
   if (!ok) {
     if (g_err.empty()) dp_set_error("context creation failed");
     dp_destroy(c);
@@ -218,6 +221,7 @@ void dp_destroy(dp_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   dp_mlp_free(c);
   dev_free(&c->d_sys);
+  dev_free(&c->d_trace);
   dev_free(&c->d_off);
   dev_free(&c->d_st);
   dev_free(&c->d_start);
@@ -251,6 +255,8 @@ void dp_destroy(dp_ctx* c) {
   dev_free(&c->d_cont_x);
   dev_free(&c->d_conf_cnt);
   dev_free(&c->d_conf);
+  dev_free(&c->d_conf_ov);
+  dev_free(&c->d_conf_pool);
   dev_free(&c->d_tiles);
   dev_free(&c->d_node_cursor);
   dev_free(&c->d_node_count);
@@ -525,6 +531,9 @@ int dp_reserve(dp_ctx* c, int32_t max_nodes, int64_t max_pairs) {
     if ((rc = dev_alloc(&c->d_cont_x, (int64_t)nx * max_pairs))) return rc;
     if ((rc = dev_alloc(&c->d_conf_cnt, max_pairs))) return rc;
     if ((rc = dev_alloc(&c->d_conf, (int64_t)DP_CONF_SLOTS * max_pairs))) return rc;
+    if ((rc = dev_alloc(&c->d_conf_ov, max_pairs))) return rc;
+    c->conf_pool_cap = 2 * max_pairs;  // about 1 in 10 lists overflows, by ~15 entries, on dense batches
+    if ((rc = dev_alloc(&c->d_conf_pool, c->conf_pool_cap))) return rc;
     if (c->desc.system == DP_SYS_QUADROTOR && (rc = dev_alloc(&c->d_dbg32, 3 * max_pairs))) return rc;
     c->pair_cap = max_pairs;
   }
@@ -864,6 +873,31 @@ int dp_autotune(dp_ctx* c, const dp_expand_params* p, dp_tile_config* best, dp_t
   // the counters of the last timed run stay valid; finish its resolution
   dp_counts cnt;
   return dp_get_counts(c, &cnt);
+}
+
+int dp_debug_resolve_trace(dp_ctx* c, int32_t enable, uint64_t* out, int64_t n) {
+  ARG_CHECK(c && n >= 0, "invalid argument");
+  HIP_CHECK(hipSetDevice(c->device));
+  HIP_CHECK(hipDeviceSynchronize());
+  if (out && n > 0) {
+    ARG_CHECK(c->d_trace && n <= c->trace_cap / 4, "no trace of that size");
+    HIP_CHECK(hipMemcpy(out, c->d_trace, sizeof(uint64_t) * 4 * n, hipMemcpyDeviceToHost));
+    int khz = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+    out[4 * n] = (uint64_t)khz;
+  }
+  if (enable > 0) {
+    const int64_t need = 4 * (int64_t)std::max(c->N_cap, 1);
+    if (need > c->trace_cap) {
+      int rc = dev_alloc(&c->d_trace, need);
+      if (rc) return rc;
+      c->trace_cap = need;
+    }
+  } else if (enable < 0) {
+    dev_free(&c->d_trace);
+    c->trace_cap = 0;
+  }
+  return DP_OK;
 }
 
 }  // extern "C"

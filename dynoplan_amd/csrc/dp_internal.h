@@ -24,7 +24,9 @@
 #define DP_WAIT_POLLS (1 << 14)
 #define DP_SPIN_LIMIT_WAITING (1 << 10)
 // Earlier-conflict slots stored per pair (more: the resolution scans the grid).
-#define DP_CONF_SLOTS 16
+#define DP_CONF_SLOTS 32
+// longest stored earlier-conflict list (inline slots + overflow pool)
+#define DP_CONF_MAX 4096
 
 void dp_set_error(const std::string& msg);
 
@@ -192,6 +194,9 @@ struct dp_ctx {
   double* d_cont_x = nullptr;       // [nx][pair_cap] contender final states sorted by cell
   int16_t* d_conf_cnt = nullptr;    // [pair_cap] stored earlier conflicts per pair (-1: scan)
   int32_t* d_conf = nullptr;        // [DP_CONF_SLOTS][pair_cap] earlier conflicting contenders
+  int32_t* d_conf_ov = nullptr;     // [pair_cap] offset of entries DP_CONF_SLOTS.. in the overflow pool
+  int32_t* d_conf_pool = nullptr;   // [pool_cap] overflow entries (bump-allocated per batch)
+  int64_t conf_pool_cap = 0;
   double round_r_dup = 0.0;         // duplicate radius of the batch in flight
   int32_t* d_tiles = nullptr;       // [tiles+1] partial sums of the large scan
   int64_t tiles_cap = 0;
@@ -214,6 +219,10 @@ struct dp_ctx {
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   bool timing_valid = false;
   bool no_events = false;           // set while a multi-instance graph is captured
+  int32_t poll_cap = 16;            // resolution: longest pause between polls (64-cycle sleeps)
+  int32_t resolve_rounds = 0;       // > 0: resolve by that many round launches instead
+  unsigned long long* d_trace = nullptr;  // [4 N_cap] resolution per node: start, end, waits, polls
+  int64_t trace_cap = 0;
   dp_tile_config tiles{0, 2048, 4096, 256, 8192, 0.0f};  // launch shapes (dp_autotune)
   int64_t last_pairs = -1;
 };

@@ -159,27 +159,48 @@ __global__ void k_conf(const dp_dev_sys* __restrict__ Sp, dp_grid_geom G, const 
                        const int8_t* __restrict__ verdict, const int8_t* __restrict__ flags,
                        const double* __restrict__ fin, int64_t pcap, const int32_t* __restrict__ cstart,
                        const int32_t* __restrict__ cidx, const double* __restrict__ cx, double r, int32_t window,
-                       int16_t* __restrict__ conf_cnt, int32_t* __restrict__ conf) {
+                       int16_t* __restrict__ conf_cnt, int32_t* __restrict__ conf, int32_t* __restrict__ conf_ov,
+                       int32_t* __restrict__ pool, int64_t pool_cap, long long* __restrict__ counters) {
   const dp_dev_sys& S = *Sp;
-  const int64_t n = *total;
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
-    const int8_t v = verdict[q], f = flags[q];
-    const bool checker = v == DP_V_FREE && !(f & 2) && !(f & 1);
-    if (!checker) continue;
+  // Contenders in cell order: neighbouring threads scan overlapping cell
+  // blocks, so the grid reads are shared through the caches.  Every checker
+  // (a FREE pair neither goal-forced nor a duplicate of the visited set) is a
+  // contender.
+  const int64_t n = cstart[G.ncells];
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t q = cidx[t];
+    if (flags[q] & 2) continue;  // goal-forced: NOVEL without a list
     if (window > 0 && q - app[pnode[q]] >= window) {
       conf_cnt[q] = -1;
       continue;
     }
     double x[NX];
 #pragma unroll
-    for (int k = 0; k < NX; ++k) x[k] = fin[(int64_t)k * pcap + q];
+    for (int k = 0; k < NX; ++k) x[k] = cx[(int64_t)k * pcap + t];
     int cnt = 0;
     scan_earlier<NX>(S, G, cstart, cidx, cx, pcap, x, q, r, [&](int32_t o) {
       if (cnt < DP_CONF_SLOTS) conf[(int64_t)cnt * pcap + q] = o;
       ++cnt;
-      return cnt <= DP_CONF_SLOTS;
+      return cnt < DP_CONF_MAX;
     });
-    conf_cnt[q] = (int16_t)(cnt <= DP_CONF_SLOTS ? cnt : -1);
+    if (cnt > DP_CONF_SLOTS && cnt < DP_CONF_MAX) {
+      // the rest of the list goes to the overflow pool; the scan order is
+      // the same, so a second scan lists the same entries
+      const long long off =
+          (long long)atomicAdd((unsigned long long*)&counters[DPC_CONF], (unsigned long long)(cnt - DP_CONF_SLOTS));
+      if (off + cnt - DP_CONF_SLOTS <= pool_cap) {
+        int k = 0;
+        scan_earlier<NX>(S, G, cstart, cidx, cx, pcap, x, q, r, [&](int32_t o) {
+          if (k >= DP_CONF_SLOTS) pool[off + (k - DP_CONF_SLOTS)] = o;
+          ++k;
+          return k < cnt;
+        });
+        conf_ov[q] = (int32_t)off;
+      } else {
+        cnt = DP_CONF_MAX;  // pool exhausted: scan the grid when reached
+      }
+    }
+    conf_cnt[q] = (int16_t)(cnt < DP_CONF_MAX ? cnt : -1);
   }
 }
 
@@ -213,6 +234,8 @@ struct round_args {
   const int8_t* verdict;
   const int8_t* flags;
   const int16_t* conf_cnt;
+  const int32_t* conf_ov;
+  const int32_t* pool;
   const int32_t* conf;
   const double* fin;
   int64_t pcap;
@@ -226,9 +249,19 @@ struct round_args {
   int32_t* count;
   int32_t* node_novel;
   int32_t limit;
-  int32_t wait;  // polls a lane may spend on one undecided earlier contender
+  int32_t wait;      // polls a lane may spend on one undecided earlier contender
+  int32_t poll_cap;  // longest pause between polls, in 64-cycle sleeps
+  unsigned long long* trace;  // optional [2N]: GPU clock when each node's wave starts and ends
   long long* counters;
 };
+
+// Pause between two polls of a contender's verdict: grows with the number of
+// polls so far, up to `cap` sleeps of 64 cycles, which keeps thousands of
+// waiting waves from flooding the fabric with coherent loads.
+__device__ __forceinline__ void poll_pause(int32_t w, int32_t cap) {
+  const int32_t k = 1 + (w >> 3) < cap ? 1 + (w >> 3) : cap;
+  for (int32_t z = 0; z < k; ++z) __builtin_amdgcn_s_sleep(1);
+}
 
 // Advance node `node`'s cursor through its pairs in canonical order until a
 // pair waits on an undecided earlier contender of another node.  Returns
@@ -263,6 +296,11 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
     int32_t blk_o[kBlk];
     int nblk = 0;
     bool more = false;
+    uint64_t undec = 0;  // entries (< 64) of a stored list whose contender was undecided
+    // entry e of pair q's stored earlier-conflict list
+    auto entry = [&](int32_t q, int e) -> int32_t {
+      return e < DP_CONF_SLOTS ? a.conf[(int64_t)e * a.pcap + q] : a.pool[(int64_t)a.conf_ov[q] + (e - DP_CONF_SLOTS)];
+    };
     // Earlier contenders of pair p: those in this chunk go to `same` (the
     // walk below decides them in order); an earlier-chunk contender already
     // NOVEL makes p a duplicate, one still undecided blocks p.  With
@@ -274,9 +312,9 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
           same |= 1ull << (o - cb);
           return true;
         }
-        int r = __hip_atomic_load(&a.res[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int r = polls == 0 ? a.res[o] : __hip_atomic_load(&a.res[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         for (int32_t w = 0; r == DP_R_UNDECIDED && w < polls; ++w) {
-          __builtin_amdgcn_s_sleep(1);
+          poll_pause(w, a.poll_cap);
           r = __hip_atomic_load(&a.res[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (r == DP_R_NOVEL) {
@@ -289,8 +327,33 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
         return !ext_dup;
       };
       const int cnt = a.conf_cnt[p];
-      if (cnt >= 0) {
-        for (int e = 0; e < cnt && !ext_dup; ++e) visit(a.conf[(int64_t)e * a.pcap + p]);
+      if (cnt >= 0 && polls == 0) {
+        // list entries and their verdicts, eight independent loads at a
+        // time.  Plain (cached) loads suffice: a decided verdict never
+        // changes, and a stale "undecided" is polled coherently later.
+        for (int g = 0; g < cnt; g += 8) {
+          int32_t os[8];
+          int rs[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) os[k] = g + k < cnt ? entry(p, g + k) : INT32_MAX;
+#pragma unroll
+          for (int k = 0; k < 8; ++k) rs[k] = os[k] < cb ? a.res[os[k]] : (int)DP_R_NONE;
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            if (os[k] == INT32_MAX) continue;
+            if (os[k] >= cb) {
+              same |= 1ull << (os[k] - cb);
+            } else if (rs[k] == DP_R_NOVEL) {
+              ext_dup = true;
+            } else if (rs[k] == DP_R_UNDECIDED) {
+              ext_block = true;
+              if (g + k < 64) undec |= 1ull << (g + k);
+              else more = true;
+            }
+          }
+        }
+      } else if (cnt >= 0) {
+        for (int e = 0; e < cnt && !ext_dup; ++e) visit(entry(p, e));
       } else {
         double x[NX];
 #pragma unroll
@@ -299,14 +362,35 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
       }
     };
     if (fr && !forced && !dupv) eval(p, 0);
+    if (a.trace) {
+      const uint64_t lazy = __ballot(fr && !forced && !dupv && a.conf_cnt[p] < 0);
+      if (lane == 0) {
+        a.trace[4 * (int64_t)node + 2] += (unsigned long long)__popcll(lazy) << 32;
+        a.trace[4 * (int64_t)node + 3] += 1ull << 32;
+      }
+    }
     const uint64_t free_m = __ballot(fr);
     const uint64_t forced_m = __ballot(fr && forced);
     const uint64_t dupv_m = __ballot(fr && dupv && !forced);
     const uint64_t dup_m = __ballot(ext_dup);
     const uint64_t block_m = __ballot(ext_block && !ext_dup);
-    uint64_t novel_m = 0, capped_m = 0, dupb_m = 0, dupvr_m = 0;
+    uint64_t novel_m = 0, capped_m = 0, dupb_m = 0, dupvr_m = 0, pub_m = 0;
     int stop_lane = 64;
     uint64_t rem = free_m;
+    // Store the verdicts decided since the last call: other nodes' waves may
+    // be waiting for them, so they go out before this wave waits itself.
+    const uint64_t lane_bit = 1ull << lane;
+    auto publish = [&]() {
+      const uint64_t dec = (novel_m | capped_m | dupb_m | dupvr_m) & ~pub_m;
+      if (fr && (dec & lane_bit)) {
+        const int r = (capped_m & lane_bit) ? DP_R_CAPPED
+                      : (novel_m & lane_bit) ? DP_R_NOVEL
+                      : (dupvr_m & lane_bit) ? DP_R_DUP_VISITED
+                                             : DP_R_DUP_BATCH;
+        __hip_atomic_store(&a.res[p], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      pub_m |= dec;
+    };
     while (rem) {
       const int b = __ffsll((long long)rem) - 1;
       rem &= rem - 1ull;
@@ -318,15 +402,53 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
       else if (block_m & bit) {
         // Only this lane's verdict is needed next: evaluate it again,
         // waiting for its undecided contenders.
+        publish();
         bool d = false, blk = false;
-        if (a.wait > 0 && lane == b) {
+        if (a.wait > 0 && lane == b && undec && !more) {
+          // stored list: poll the undecided contenders, eight at a time,
+          // until one is NOVEL or all are decided
+          uint64_t m = undec;
+          int32_t w = 0;
+          for (;;) {
+            for (int g = 0; g < 64 && !d; g += 8) {
+              if (!((m >> g) & 0xffull)) continue;
+              int32_t oo[8];
+#pragma unroll
+              for (int k = 0; k < 8; ++k) oo[k] = (m >> (g + k)) & 1ull ? entry(p, g + k) : -1;
+#pragma unroll
+              for (int k = 0; k < 8; ++k) {
+                const int r = oo[k] >= 0
+                                  ? __hip_atomic_load(&a.res[oo[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                  : (int)DP_R_UNDECIDED;
+                if (oo[k] < 0) continue;
+                if (r == DP_R_NOVEL) d = true;
+                else if (r != DP_R_UNDECIDED) m &= ~(1ull << (g + k));
+              }
+            }
+            if (d || !m) break;
+            if (++w > a.wait) {
+              blk = true;
+              break;
+            }
+            poll_pause(w, a.poll_cap);
+          }
+          if (a.trace) {
+            a.trace[4 * (int64_t)node + 2] += 1;
+            a.trace[4 * (int64_t)node + 3] += (unsigned long long)w;
+          }
+        } else if (a.wait > 0 && lane == b) {
           if (!more) {
             // wait for the recorded contenders only
             for (int e = 0; e < nblk && !d && !blk; ++e) {
               int r = __hip_atomic_load(&a.res[blk_o[e]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              for (int32_t w = 0; r == DP_R_UNDECIDED && w < a.wait; ++w) {
-                __builtin_amdgcn_s_sleep(1);
+              int32_t w = 0;
+              for (; r == DP_R_UNDECIDED && w < a.wait; ++w) {
+                poll_pause(w, a.poll_cap);
                 r = __hip_atomic_load(&a.res[blk_o[e]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              }
+              if (a.trace) {
+                a.trace[4 * (int64_t)node + 2] += 1;
+                a.trace[4 * (int64_t)node + 3] += (unsigned long long)w;
               }
               if (r == DP_R_NOVEL) d = true;
               else if (r == DP_R_UNDECIDED) blk = true;
@@ -350,14 +472,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
       } else if (readlane64(same, b) & novel_m) dupb_m |= bit;
       else novel_m |= bit;
     }
-    const uint64_t bit = 1ull << lane;
-    if (fr && ((novel_m | capped_m | dupb_m | dupvr_m) & bit)) {
-      const int r = (capped_m & bit) ? DP_R_CAPPED
-                    : (novel_m & bit) ? DP_R_NOVEL
-                    : (dupvr_m & bit) ? DP_R_DUP_VISITED
-                                      : DP_R_DUP_BATCH;
-      __hip_atomic_store(&a.res[p], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    publish();
     count += __popcll(novel_m);
     n_nov += __popcll(novel_m);
     n_cap += __popcll(capped_m);
@@ -409,6 +524,12 @@ __global__ __launch_bounds__(64) void k_resolve_ticketed(round_args a) {
   __syncthreads();
   const int32_t node = s_node;
   if (node >= a.N) return;
+  if (a.trace && threadIdx.x == 0) {
+    a.trace[4 * (int64_t)node] = (unsigned long long)wall_clock64();
+    a.trace[4 * (int64_t)node + 2] = 0;
+    a.trace[4 * (int64_t)node + 3] = 0;
+  }
+  __syncthreads();
   int32_t spins = 0;
   int32_t cur = a.cursor[node], count = a.count[node];
   const int32_t limit = a.wait > 0 ? DP_SPIN_LIMIT_WAITING : DP_SPIN_LIMIT;
@@ -420,6 +541,7 @@ __global__ __launch_bounds__(64) void k_resolve_ticketed(round_args a) {
     __builtin_amdgcn_s_sleep(2);
   }
   if (threadIdx.x == 0 && spins) atomicAdd((unsigned long long*)&a.counters[DPC_ROUNDS], (unsigned long long)spins);
+  if (a.trace && threadIdx.x == 0) a.trace[4 * (int64_t)node + 1] = (unsigned long long)wall_clock64();
 }
 
 // Append the NOVEL final states of the batch to the visited set in canonical
@@ -473,6 +595,8 @@ static round_args make_round_args(dp_ctx* c) {
   a.verdict = c->d_verdict;
   a.flags = c->d_flags;
   a.conf_cnt = c->d_conf_cnt;
+  a.conf_ov = c->d_conf_ov;
+  a.pool = c->d_conf_pool;
   a.conf = c->d_conf;
   a.fin = c->d_final;
   a.pcap = c->pair_cap;
@@ -487,6 +611,8 @@ static round_args make_round_args(dp_ctx* c) {
   a.node_novel = c->d_node_novel;
   a.limit = c->round_limit;
   a.wait = 0;
+  a.poll_cap = c->poll_cap;
+  a.trace = c->d_trace;
   a.counters = c->d_counters;
   return a;
 }
@@ -545,12 +671,17 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
                                           c->d_cont_start, c->d_cont_cursor, c->d_cont_idx, c->d_cont_x);
     k_conf<NX><<<pb, 256, 0, s>>>(c->d_sys, c->grid, c->d_total, c->d_pnode, c->d_app, c->d_verdict, c->d_flags,
                                   c->d_final, c->pair_cap, c->d_cont_start, c->d_cont_idx, c->d_cont_x, r_dup,
-                                  window, c->d_conf_cnt, c->d_conf);
+                                  window, c->d_conf_cnt, c->d_conf, c->d_conf_ov, c->d_conf_pool, c->conf_pool_cap,
+                                  c->d_counters);
     k_round_init<<<dp_grid_blocks(N, 256, 1024), 256, 0, s>>>(N, c->d_app, c->d_total, c->d_node_cursor,
                                                               c->d_node_count, c->d_node_novel, c->d_counters);
     round_args ra = make_round_args(c);
-    ra.wait = DP_WAIT_POLLS;
-    k_resolve_ticketed<NX><<<N, 64, 0, s>>>(ra);
+    if (c->resolve_rounds > 0) {
+      for (int r = 0; r < c->resolve_rounds; ++r) k_round<NX><<<N, 64, 0, s>>>(ra);
+    } else {
+      ra.wait = DP_WAIT_POLLS;
+      k_resolve_ticketed<NX><<<N, 64, 0, s>>>(ra);
+    }
     return DP_OK;
   });
   if (rc) return rc;

@@ -623,8 +623,10 @@ int dp_surrogate_train(dp_ctx* c, const double* nodes, int32_t N, const dp_expan
     int64_t cap3 = 0;
     if ((rc = ensure_f32(&m.slabel, &cap3, npairs))) return rc;
   }
-  DP_HIP_RET(hipMemcpy(m.snode, c->d_pnode, sizeof(int32_t) * npairs, hipMemcpyDeviceToDevice));
-  DP_HIP_RET(hipMemcpy(m.sprim, c->d_pprim, sizeof(int32_t) * npairs, hipMemcpyDeviceToDevice));
+  // on the context stream: device-to-device copies do not block the host,
+  // and the null stream does not order with the context stream
+  DP_HIP_RET(hipMemcpyAsync(m.snode, c->d_pnode, sizeof(int32_t) * npairs, hipMemcpyDeviceToDevice, s));
+  DP_HIP_RET(hipMemcpyAsync(m.sprim, c->d_pprim, sizeof(int32_t) * npairs, hipMemcpyDeviceToDevice, s));
   {
     std::vector<int8_t> v((size_t)npairs);
     std::vector<float> y((size_t)npairs);
@@ -656,6 +658,7 @@ int dp_surrogate_train(dp_ctx* c, const double* nodes, int32_t N, const dp_expan
     }
     if (loss_out) {
       float h[2];
+      DP_HIP_RET(hipStreamSynchronize(s));  // the context stream does not order with the null stream
       DP_HIP_RET(hipMemcpy(h, m.loss, sizeof(float) * 2, hipMemcpyDeviceToHost));
       loss_out[e] = h[1] / (float)steps_per_epoch;
     }
@@ -673,6 +676,7 @@ extern "C" {
 // verdicts, and set the threshold to (lowest score of a FREE pair) - margin:
 // on these held-out nodes every feasible edge is kept (recall 1).
 int dp_surrogate_calibrate(dp_ctx* c, const double* nodes, int32_t N, const dp_expand_params* params, float margin,
+                           float tail,
                            dp_surrogate_stats* out) {
   if (!c || !nodes || !params || N <= 0) {
     dp_set_error("invalid argument");
@@ -695,14 +699,25 @@ int dp_surrogate_calibrate(dp_ctx* c, const double* nodes, int32_t N, const dp_e
     DP_HIP_RET(hipMemcpy(sc.data(), m.score, sizeof(float) * np, hipMemcpyDeviceToHost));
     DP_HIP_RET(hipMemcpy(v.data(), c->d_verdict, np, hipMemcpyDeviceToHost));
   }
-  float min_free = INFINITY;
+  // the ten lowest scores of feasible pairs
+  std::vector<float> low;
   int64_t n_free = 0;
   for (long long q = 0; q < np; ++q)
     if (v[q] == DP_V_FREE) {
       ++n_free;
-      min_free = std::min(min_free, sc[q]);
+      low.push_back(sc[q]);
+      std::push_heap(low.begin(), low.end());
+      if (low.size() > 10) {
+        std::pop_heap(low.begin(), low.end());
+        low.pop_back();
+      }
     }
-  const float tau = n_free ? min_free - margin : -INFINITY;
+  std::sort(low.begin(), low.end());
+  float tau = -INFINITY;
+  if (n_free) {
+    const float spread = low.back() - low.front();
+    tau = low.front() - std::max(margin, tail * spread);
+  }
   int64_t rej = 0, rej_col = 0, n_col = 0;
   for (long long q = 0; q < np; ++q) {
     if (v[q] == DP_V_COLLIDE) ++n_col;

@@ -155,11 +155,11 @@ class GpuExpander:
                                             lr, seed, losses.ctypes.data_as(C.POINTER(C.c_float))))
         return losses
 
-    def surrogate_calibrate(self, nodes, params, margin=0.5):
+    def surrogate_calibrate(self, nodes, params, margin=0.5, tail=3.0):
         nodes = np.ascontiguousarray(nodes, np.float64).reshape(-1, self.nx)
         st = N.SurrogateStats()
         N.check(self.lib.dp_surrogate_calibrate(self.ctx, _dptr(nodes), nodes.shape[0], C.byref(params), margin,
-                                                C.byref(st)))
+                                                tail, C.byref(st)))
         return st.as_dict()
 
     def surrogate_scores(self, n):

@@ -307,9 +307,11 @@ int dp_surrogate_init(dp_ctx* ctx, uint64_t seed);
  * passes of minibatch Adam; loss_out[epochs] (optional) gets the epoch losses. */
 int dp_surrogate_train(dp_ctx* ctx, const double* nodes, int32_t N, const dp_expand_params* params, int32_t epochs,
                        int32_t batch, float lr, uint64_t seed, float* loss_out);
-/* Threshold = lowest score of a FREE pair of held-out `nodes` minus margin. */
+/* Threshold from the scores of the FREE pairs of held-out `nodes`: their
+ * lowest score s1 minus max(margin, tail * (s10 - s1)), s10 the tenth lowest
+ * (the spread of the lower tail widens the margin where the tail is long). */
 int dp_surrogate_calibrate(dp_ctx* ctx, const double* nodes, int32_t N, const dp_expand_params* params,
-                           float margin, dp_surrogate_stats* out);
+                           float margin, float tail, dp_surrogate_stats* out);
 int dp_surrogate_set_threshold(dp_ctx* ctx, float threshold);
 /* Scores of the last expansion run with use_surrogate != 0. */
 int dp_surrogate_scores(dp_ctx* ctx, float* out, int64_t n);
@@ -330,6 +332,14 @@ int dp_debug_gemm_bf16(dp_ctx* ctx, const float* A, const float* B, int32_t M, i
  * dp_expand (other entries unspecified).  No counterpart in the reference;
  * used to report the FP32-vs-FP64 rollout error per system. */
 int dp_debug_fp32_positions(dp_ctx* ctx, float* out, int64_t n);
+
+/* Diagnostics of the duplicate resolution (no counterpart in the reference):
+ * enable > 0 records, for each node of later batches, the GPU clock when its
+ * wave starts and ends, the number of waits on other nodes' verdicts and the
+ * polls they took; enable < 0 stops.  With out != NULL, copies the record of
+ * the last batch, out[4n + 1]: (start, end, waits, polls) per node, then the
+ * clock rate in kHz. */
+int dp_debug_resolve_trace(dp_ctx* ctx, int32_t enable, uint64_t* out, int64_t n);
 
 /* ---- db-A* search on the expansion path (dbastar.cpp:523-1010) -------------
  * The host loop of dbastar(): open list ordered by float fScore then higher
