@@ -205,8 +205,6 @@ dp_ctx* dp_create(const dp_system_desc* sys, int32_t device) {
   ok = ok && hipMemset(c->d_total, 0, sizeof(int32_t)) == hipSuccess;
   ok = ok && upload_sys(c) == DP_OK;
   ok = ok && hipHostMalloc((void**)&c->h_pinned, sizeof(long long) * 4) == hipSuccess;
-  if (const char* e = std::getenv("DP_POLL_CAP")) c->poll_cap = std::max(1, std::min(256, std::atoi(e)));
-  if (const char* e = std::getenv("DP_RESOLVE_ROUNDS")) c->resolve_rounds = std::max(0, std::min(4096, std::atoi(e)));
   if (!ok) {
     if (g_err.empty()) dp_set_error("context creation failed");
     dp_destroy(c);

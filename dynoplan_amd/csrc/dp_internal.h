@@ -220,7 +220,6 @@ struct dp_ctx {
   bool timing_valid = false;
   bool no_events = false;           // set while a multi-instance graph is captured
   int32_t poll_cap = 16;            // resolution: longest pause between polls (64-cycle sleeps)
-  int32_t resolve_rounds = 0;       // > 0: resolve by that many round launches instead
   unsigned long long* d_trace = nullptr;  // [4 N_cap] resolution per node: start, end, waits, polls
   int64_t trace_cap = 0;
   dp_tile_config tiles{0, 2048, 4096, 256, 8192, 0.0f};  // launch shapes (dp_autotune)

@@ -361,13 +361,11 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
         scan_earlier<NX>(*a.sys, a.grid, a.cstart, a.cidx, a.cx, a.pcap, x, p, a.r_dup, visit);
       }
     };
+    const unsigned long long t_eval = a.trace ? (unsigned long long)wall_clock64() : 0ull;
     if (fr && !forced && !dupv) eval(p, 0);
     if (a.trace) {
-      const uint64_t lazy = __ballot(fr && !forced && !dupv && a.conf_cnt[p] < 0);
-      if (lane == 0) {
-        a.trace[4 * (int64_t)node + 2] += (unsigned long long)__popcll(lazy) << 32;
-        a.trace[4 * (int64_t)node + 3] += 1ull << 32;
-      }
+      __builtin_amdgcn_wave_barrier();
+      if (lane == 0) a.trace[4 * (int64_t)node + 3] += (unsigned long long)wall_clock64() - t_eval;
     }
     const uint64_t free_m = __ballot(fr);
     const uint64_t forced_m = __ballot(fr && forced);
@@ -409,6 +407,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
           // until one is NOVEL or all are decided
           uint64_t m = undec;
           int32_t w = 0;
+          const unsigned long long t_wait = a.trace ? (unsigned long long)wall_clock64() : 0ull;
           for (;;) {
             for (int g = 0; g < 64 && !d; g += 8) {
               if (!((m >> g) & 0xffull)) continue;
@@ -432,10 +431,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
             }
             poll_pause(w, a.poll_cap);
           }
-          if (a.trace) {
-            a.trace[4 * (int64_t)node + 2] += 1;
-            a.trace[4 * (int64_t)node + 3] += (unsigned long long)w;
-          }
+          if (a.trace) a.trace[4 * (int64_t)node + 2] += (unsigned long long)wall_clock64() - t_wait;
         } else if (a.wait > 0 && lane == b) {
           if (!more) {
             // wait for the recorded contenders only
@@ -446,10 +442,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
                 poll_pause(w, a.poll_cap);
                 r = __hip_atomic_load(&a.res[blk_o[e]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               }
-              if (a.trace) {
-                a.trace[4 * (int64_t)node + 2] += 1;
-                a.trace[4 * (int64_t)node + 3] += (unsigned long long)w;
-              }
+
               if (r == DP_R_NOVEL) d = true;
               else if (r == DP_R_UNDECIDED) blk = true;
             }
@@ -676,12 +669,8 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
     k_round_init<<<dp_grid_blocks(N, 256, 1024), 256, 0, s>>>(N, c->d_app, c->d_total, c->d_node_cursor,
                                                               c->d_node_count, c->d_node_novel, c->d_counters);
     round_args ra = make_round_args(c);
-    if (c->resolve_rounds > 0) {
-      for (int r = 0; r < c->resolve_rounds; ++r) k_round<NX><<<N, 64, 0, s>>>(ra);
-    } else {
-      ra.wait = DP_WAIT_POLLS;
-      k_resolve_ticketed<NX><<<N, 64, 0, s>>>(ra);
-    }
+    ra.wait = DP_WAIT_POLLS;
+    k_resolve_ticketed<NX><<<N, 64, 0, s>>>(ra);
     return DP_OK;
   });
   if (rc) return rc;
