@@ -24,6 +24,11 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
+# HBM bytes per launch of the check kernel, measured with rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy (profiles/ holds the run).
+MEASURED_TRAFFIC = {
+    "unicycle1_scale": (1.998e9, "profiles/k_check_planar_unicycle1_scale_hbm_traffic.txt"),
+}
 
 
 def progress(msg):
@@ -366,6 +371,8 @@ def main():
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "algorithmic_bytes_per_launch": abytes},
     }
+    if args.workload in MEASURED_TRAFFIC and not use_sur:
+        line["roofline"]["traffic"], line["roofline"]["traffic_source"] = MEASURED_TRAFFIC[args.workload]
     if use_sur:
         # Untimed validation: the exact path on the same batch decides every
         # pair; the surrogate run must keep every feasible edge and give the

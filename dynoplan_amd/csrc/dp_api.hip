@@ -369,8 +369,8 @@ int dp_upload_primitives(dp_ctx* c, const dp_prims* pr, int32_t max_motions) {
       for (int32_t k = 0; k < L; ++k)
         for (int u = 0; u < nu; ++u) {
           double v = ac[(src0 + k) * nu + u];
-          asoa[(size_t)u * A + dst0 + k] = v;
-          asoa32[(size_t)u * A + dst0 + k] = (float)v;
+          asoa[(size_t)(dst0 + k) * nu + u] = v;  // one row of 4 per step: a single vector load
+          asoa32[(size_t)(dst0 + k) * nu + u] = (float)v;
         }
     }
     if ((rc = dev_alloc(&c->d_act, (int64_t)asoa.size()))) return rc;

@@ -144,8 +144,8 @@ struct dp_ctx {
   double* d_start = nullptr;     // [nx][P]
   double* d_geo = nullptr;       // [4*nbox][S] : relx rely c s (per box)
   double* d_sweep = nullptr;     // [4][P] canonical swept footprint box (planar) / centre box (quad)
-  double* d_act = nullptr;       // [nu][A]
-  float* d_act32 = nullptr;      // [nu][A]
+  double* d_act = nullptr;       // [A][nu] (quadrotor: nu = 4, 32-byte rows)
+  float* d_act32 = nullptr;      // [A][nu] (16-byte rows)
   int32_t* d_goalidx = nullptr;  // [8][P]
   int32_t goal_num = -1;
 

@@ -34,8 +34,8 @@ struct dp_quad_args {
   const int32_t* off;
   const double* st;       // [13][S]
   int32_t S;
-  const double* act;      // [4][A]
-  const float* act32;     // [4][A]
+  const double* act;      // [A][4]
+  const float* act32;     // [A][4]
   int32_t A;
   int32_t P;
   const int32_t* goalidx;
@@ -267,9 +267,8 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
       bool near_bound_any = false, near_col_before = false, near = false;
       for (int k = 0; k < L1; ++k) {
         if (k > 0) {
-          float u[4];
-#pragma unroll
-          for (int c = 0; c < 4; ++c) u[c] = a.act32[(int64_t)c * a.A + a0 + k - 1];
+          const float4 uv = reinterpret_cast<const float4*>(a.act32)[a0 + k - 1];
+          const float u[4] = {uv.x, uv.y, uv.z, uv.w};
           quad_step32(K, s, u);
         }
         // bounds with margin
@@ -391,9 +390,9 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
     const int nng = a.check_goal ? a.num_goal : 0;
     for (int k = 0; k < L1; ++k) {
       if (k > 0) {
-        double u[4], o[13];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) u[c] = a.act[(int64_t)c * a.A + a0 + k - 1];
+        const double4 uv = reinterpret_cast<const double4*>(a.act)[a0 + k - 1];
+        const double u[4] = {uv.x, uv.y, uv.z, uv.w};
+        double o[13];
         dp_quad_step64(S, s, u, o);
 #pragma unroll
         for (int c = 0; c < 13; ++c) s[c] = o[c];
