@@ -97,16 +97,32 @@ __global__ __launch_bounds__(256) void k_gemm(gemm_args g) {
 __global__ __launch_bounds__(256) void k_patch(const double* __restrict__ nodes, int32_t Ncap, int32_t N,
                                                const double* __restrict__ obs, int32_t nobs, double cell,
                                                uint16_t* __restrict__ patch) {
+  // Obstacles that can contain a cell centre of this patch go to LDS first
+  // (a superset: the exact per-cell test below decides).
+  __shared__ double cand[4 * DP_MAX_OBS];
+  __shared__ int ncand;
   const int32_t i = blockIdx.x;
   if (i >= N) return;
   const double x = nodes[i], y = nodes[(int64_t)Ncap + i];
+  if (threadIdx.x == 0) ncand = 0;
+  __syncthreads();
+  const double ext = 15.5 * cell + 1e-9;
+  for (int o = threadIdx.x; o < nobs; o += blockDim.x) {
+    const double* b = obs + 4 * o;
+    if (fabs(x - b[0]) <= b[2] + ext && fabs(y - b[1]) <= b[3] + ext) {
+      const int k = atomicAdd(&ncand, 1);
+      for (int c = 0; c < 4; ++c) cand[4 * k + c] = b[c];
+    }
+  }
+  __syncthreads();
+  const int nc = ncand;
   for (int t = threadIdx.x; t < 1024; t += blockDim.x) {
     const int r = t / 32, c = t % 32;
     const double px = x + ((double)c - 15.5) * cell;
     const double py = y + ((double)r - 15.5) * cell;
     bool occ = false;
-    for (int o = 0; o < nobs && !occ; ++o) {
-      const double* b = obs + 4 * o;
+    for (int o = 0; o < nc && !occ; ++o) {
+      const double* b = cand + 4 * o;
       occ = fabs(px - b[0]) <= b[2] && fabs(py - b[1]) <= b[3];
     }
     patch[(int64_t)i * 1024 + t] = occ ? 0x3f80 : 0;  // 1.0 / 0.0 in BF16
@@ -136,37 +152,60 @@ struct infer_args {
   long long* counters;
 };
 
-constexpr int kInferWaves = 2;
+// Inference: 8 waves per workgroup, 32 pairs per wave.  The weights of a
+// layer are staged through LDS in K-slices of 32 (256 rows x 64 B, rows
+// padded to 72 B against bank conflicts) and shared by the 8 waves, so each
+// slice read from L2 serves 256 pairs; activations stay in one 32x256 BF16
+// LDS tile per wave.  146 KB of LDS: one workgroup per CU.
+constexpr int kInferWaves = 8;
+constexpr int kSlice = 32;
+constexpr int kSliceStride = kSlice + 4;  // bf16 per staged row
 
+__device__ __forceinline__ void stage_slice(const uint16_t* __restrict__ W, int ks, uint16_t* sw) {
+  // row n = t / 2, bytes [32 (t % 2), +32) of W[n][ks .. ks + 32)
+  const int t = threadIdx.x;
+  const int n = t >> 1, half = t & 1;
+  const uint2* src = reinterpret_cast<const uint2*>(W + (int64_t)n * 256 + ks + half * 16);
+  uint2* dst = reinterpret_cast<uint2*>(sw + n * kSliceStride + half * 16);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) dst[k] = src[k];
+}
+
+// out[m][n] = relu(sum_k in[m][k] W[n][k] + bias[n]) for this wave's 32 rows
+// x 256 columns; `first`: the input is relu(An + Bp) formed on the fly.
 __device__ __forceinline__ void mlp_layer(const uint16_t* __restrict__ W, const float* __restrict__ bias,
-                                          const uint16_t* lds_in, uint16_t* lds_out, int lane,
-                                          const float* An_row, const float* Bp_row, bool first) {
-  // out[m][n] = relu(sum_k in[m][k] W[n][k] + bias[n]) for 32 rows x 256 cols.
+                                          uint16_t* act, uint16_t* sw, int lane, const float* An_row,
+                                          const float* Bp_row, bool first) {
   f32x16 acc[8];
 #pragma unroll
   for (int t = 0; t < 8; ++t) acc[t] = f32x16{};
   const int kq = 4 * (lane / 32);
-  for (int k0 = 0; k0 < 256; k0 += 8) {
-    bf16x4 a;
-    if (first) {
+  for (int ks = 0; ks < 256; ks += kSlice) {
+    __syncthreads();  // the previous slice has been used by every wave
+    stage_slice(W, ks, sw);
+    __syncthreads();
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        float v = An_row[k0 + kq + i] + Bp_row[k0 + kq + i];
-        a[i] = (short)f2bf(v > 0.f ? v : 0.f);
+    for (int kk = 0; kk < kSlice; kk += 8) {
+      const int k0 = ks + kk;
+      bf16x4 a;
+      if (first) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          float v = An_row[k0 + kq + i] + Bp_row[k0 + kq + i];
+          a[i] = (short)f2bf(v > 0.f ? v : 0.f);
+        }
+      } else {
+        a = *reinterpret_cast<const bf16x4*>(act + (lane % 32) * 256 + k0 + kq);
       }
-    } else {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = (short)lds_in[(lane % 32) * 256 + k0 + kq + i];
-    }
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      bf16x4 b;
-      const uint16_t* wr = W + (int64_t)(t * 32 + lane % 32) * 256 + k0 + kq;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) b[i] = (short)wr[i];
-      acc[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[t], 0, 0, 0);
+      for (int t = 0; t < 8; ++t) {
+        const bf16x4 b = *reinterpret_cast<const bf16x4*>(sw + (t * 32 + lane % 32) * kSliceStride + kk + kq);
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[t], 0, 0, 0);
+      }
     }
   }
+  // every lane has read its inputs (the MFMAs above consumed them): the
+  // tile is overwritten in place
   __builtin_amdgcn_wave_barrier();
 #pragma unroll
   for (int t = 0; t < 8; ++t)
@@ -175,7 +214,7 @@ __device__ __forceinline__ void mlp_layer(const uint16_t* __restrict__ W, const 
       const int m = 8 * (j / 4) + 4 * (lane / 32) + (j % 4);
       const int n = t * 32 + lane % 32;
       float v = acc[t][j] + bias[n];
-      lds_out[m * 256 + n] = f2bf(v > 0.f ? v : 0.f);
+      act[m * 256 + n] = f2bf(v > 0.f ? v : 0.f);
     }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -183,23 +222,26 @@ __device__ __forceinline__ void mlp_layer(const uint16_t* __restrict__ W, const 
 }
 
 __global__ __launch_bounds__(64 * kInferWaves) void k_infer(infer_args a) {
-  // one 32x256 BF16 activation tile per wave, overwritten in place: every
-  // lane has finished reading a layer's input before its epilogue writes
-  __shared__ uint16_t lds[kInferWaves][32 * 256];
+  __shared__ uint16_t act[kInferWaves][32 * 256];
+  __shared__ uint16_t sw[256 * kSliceStride];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int32_t tot = *a.total;
   long long n_rej = 0;
-  for (int64_t q0 = ((int64_t)blockIdx.x * kInferWaves + wave) * 32; q0 < tot;
-       q0 += (int64_t)gridDim.x * kInferWaves * 32) {
+  constexpr int kTile = kInferWaves * 32;
+  // every wave of the workgroup runs the same number of tiles (the layer
+  // loops synchronise the workgroup)
+  for (int64_t t0 = (int64_t)blockIdx.x * kTile; t0 < tot; t0 += (int64_t)gridDim.x * kTile) {
+    const int64_t q0 = t0 + wave * 32;
     const int64_t qm = q0 + lane % 32;
     const bool vm = qm < tot;
-    const int32_t i = vm ? a.pnode[qm] : a.pnode[q0];
-    const int32_t j = vm ? a.pprim[qm] : a.pprim[q0];
-    mlp_layer(a.W2, a.b2, nullptr, lds[wave], lane, a.An + (int64_t)i * 256, a.Bp + (int64_t)j * 256, true);
-    mlp_layer(a.W3, a.b3, lds[wave], lds[wave], lane, nullptr, nullptr, false);
+    const int64_t qs = vm ? qm : (tot - 1);  // rows past the end repeat the last pair, unused
+    const int32_t i = a.pnode[qs];
+    const int32_t j = a.pprim[qs];
+    mlp_layer(a.W2, a.b2, act[wave], sw, lane, a.An + (int64_t)i * 256, a.Bp + (int64_t)j * 256, true);
+    mlp_layer(a.W3, a.b3, act[wave], sw, lane, nullptr, nullptr, false);
     if (lane < 32 && vm) {
       float z = a.b4[0];
-      const uint16_t* h = lds[wave] + lane * 256;
+      const uint16_t* h = act[wave] + lane * 256;
       for (int n = 0; n < 256; ++n) z += bf2f(h[n]) * a.w4[n];
       if (a.score) a.score[qm] = z;
       if (a.reject && z < a.threshold) {
@@ -547,8 +589,9 @@ int dp_launch_surrogate(dp_ctx* c, bool reject, hipStream_t s) {
   a.goal_out = c->d_goal;
   a.res = c->d_res;
   a.counters = c->d_counters;
-  const int64_t groups = ((int64_t)c->N * c->P + 31) / 32;
-  k_infer<<<dp_grid_blocks(groups, kInferWaves, 4096), 64 * kInferWaves, 0, s>>>(a);
+  // one workgroup per CU (LDS-bound); tiles of 256 pairs, grid-stride
+  const int64_t tiles = ((int64_t)c->N * c->P + 255) / 256;
+  k_infer<<<dp_grid_blocks(tiles, 1, 256), 64 * kInferWaves, 0, s>>>(a);
   DP_HIP_RET(hipGetLastError());
   return DP_OK;
 }
