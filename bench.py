@@ -283,6 +283,7 @@ def main():
     ap.add_argument("--label-pairs", type=int, default=2_000_000, help="surrogate training labels (car_trailer)")
     ap.add_argument("--no-surrogate", action="store_true", help="car_trailer without the learned pre-filter")
     ap.add_argument("--no-baselines", action="store_true", help="skip CPU and plain-GPU legs (profiling runs)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (scale reference for HBM byte counters)")
     args = ap.parse_args()
@@ -323,6 +324,11 @@ def main():
         torch.cuda.synchronize()
         progress(f"copied {args.copy_mib} MiB three times")
         del src, dst
+    tiles = None
+    if not args.no_autotune:
+        # untimed: pick the check stage's launch shape for this batch
+        tiles, trials = ex.autotune(params)
+        progress(f"autotune: {tiles}")
     for _ in range(args.warmup):
         ex.expand_async(params)
         torch.cuda.synchronize()
@@ -367,6 +373,8 @@ def main():
                    "surrogate": use_sur, "pairs_per_step": int(pairs)},
         "counts": counts,
         "stage_ms": stage_ms,
+        "dominant_stage": max((k for k in stage_ms if k != "total"), key=lambda k: stage_ms[k]),
+        "tiles": tiles,
         "roofline": {"bound": "hbm", "kernel": "k_quad32+k_quad64" if is_quad else "k_check_planar",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "algorithmic_bytes_per_launch": abytes},

@@ -220,6 +220,8 @@ void dp_destroy(dp_ctx* c) {
   dp_mlp_free(c);
   dev_free(&c->d_sys);
   dev_free(&c->d_trace);
+  dev_free(&c->d_node_obs);
+  dev_free(&c->d_node_nobs);
   dev_free(&c->d_off);
   dev_free(&c->d_st);
   dev_free(&c->d_start);
@@ -341,6 +343,14 @@ int dp_upload_primitives(dp_ctx* c, const dp_prims* pr, int32_t max_motions) {
       sweep[(size_t)P + j] = hi0;
       sweep[(size_t)2 * P + j] = lo1;
       sweep[(size_t)3 * P + j] = hi1;
+    }
+    c->reach_box[0] = c->reach_box[2] = INFINITY;
+    c->reach_box[1] = c->reach_box[3] = -INFINITY;
+    for (int32_t j = 0; j < P; ++j) {
+      c->reach_box[0] = std::min(c->reach_box[0], sweep[j]);
+      c->reach_box[1] = std::max(c->reach_box[1], sweep[(size_t)P + j]);
+      c->reach_box[2] = std::min(c->reach_box[2], sweep[(size_t)2 * P + j]);
+      c->reach_box[3] = std::max(c->reach_box[3], sweep[(size_t)3 * P + j]);
     }
   }
   int rc;
@@ -510,6 +520,8 @@ int dp_reserve(dp_ctx* c, int32_t max_nodes, int64_t max_pairs) {
     if ((rc = dev_alloc(&c->d_node_cursor, max_nodes))) return rc;
     if ((rc = dev_alloc(&c->d_node_count, max_nodes))) return rc;
     if ((rc = dev_alloc(&c->d_chunk, (int64_t)max_nodes + 1))) return rc;
+    if ((rc = dev_alloc(&c->d_node_obs, (int64_t)DP_NODE_OBS * max_nodes))) return rc;
+    if ((rc = dev_alloc(&c->d_node_nobs, max_nodes))) return rc;
     c->N_cap = max_nodes;
   }
   if (max_pairs > c->pair_cap) {

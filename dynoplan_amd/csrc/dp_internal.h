@@ -10,6 +10,8 @@
 
 // Largest obstacle count staged in LDS by one workgroup.
 #define DP_MAX_OBS 1024
+// obstacles listed per frontier node for the swept-box cull of its pairs
+#define DP_NODE_OBS 128
 // Per-pair resolution value while undecided (ordered duplicate resolution).
 #define DP_R_UNDECIDED 0x7f
 // Pad added to swept boxes of the broad phase (m): bounds the rounding of
@@ -207,6 +209,9 @@ struct dp_ctx {
   int32_t* d_chunk = nullptr;       // [N_cap+1] quadrotor: 64-pair chunks per node, scanned
   float* d_dbg32 = nullptr;         // [3][pair_cap] quadrotor: FP32 final positions (error report)
   double prim_reach = 0.0;          // max over primitives of max_k |p_k - p_0|_inf
+  double reach_box[4] = {0, 0, 0, 0};  // planar: union of the canonical swept boxes (lo x, hi x, lo y, hi y)
+  int16_t* d_node_obs = nullptr;    // [N_cap][DP_NODE_OBS] obstacles within a node's reach box
+  int32_t* d_node_nobs = nullptr;   // [N_cap] their count (-1: more than DP_NODE_OBS)
   dp_mlp mlp;
   int32_t round_limit = 0;          // limit_branching_factor of the batch in flight
 

@@ -88,6 +88,8 @@ struct dp_check_args {
   int32_t num_goal, check_goal;
   const double* obs;
   int32_t nobs;
+  const int16_t* node_obs;   // [Ncap][DP_NODE_OBS] ascending obstacle indices near each node
+  const int32_t* node_nobs;  // [Ncap] their count, -1 when the list overflowed (scan all)
   const double* vis;
   int64_t vstride;
   int64_t nv;
@@ -133,6 +135,36 @@ __device__ __forceinline__ bool grid_cell_any(const dp_dev_sys& S, const dp_grid
   for (int32_t p = start[cell]; p < e; ++p)
     if (dp_within_col<NX>(S, x, pts, stride, p, r)) return true;
   return false;
+}
+
+// Per frontier node, the obstacles that can touch any of its primitives: the
+// swept box of every primitive lies inside the union box `reach`, and the
+// pair test below uses the same rounding form with the larger box, so every
+// obstacle a pair's swept-box cull keeps is on its node's list.  One wave per
+// node; lists hold ascending indices.
+__global__ __launch_bounds__(256) void k_node_obs(const double* nodes, int32_t Ncap, int32_t N, const double* obs,
+                                                  int32_t nobs, double r0, double r1, double r2, double r3,
+                                                  int16_t* node_obs, int32_t* node_nobs) {
+  const int lane = threadIdx.x % 64;
+  for (int64_t i = (int64_t)blockIdx.x * kWaves + threadIdx.x / 64; i < N; i += (int64_t)gridDim.x * kWaves) {
+    const double ox = nodes[i], oy = nodes[(int64_t)Ncap + i];
+    const double bx0 = (r0 + ox) - DP_SWEEP_PAD, bx1 = (r1 + ox) + DP_SWEEP_PAD;
+    const double by0 = (r2 + oy) - DP_SWEEP_PAD, by1 = (r3 + oy) + DP_SWEEP_PAD;
+    int cnt = 0;
+    for (int ob = 0; ob < nobs; ob += 64) {
+      const int o = ob + lane;
+      bool hit = false;
+      if (o < nobs) {
+        const double* b = obs + 4 * o;
+        hit = !(b[0] + b[2] < bx0 || b[0] - b[2] > bx1 || b[1] + b[3] < by0 || b[1] - b[3] > by1);
+      }
+      const uint64_t m = __ballot(hit);
+      const int at = cnt + __popcll(m & ((1ull << lane) - 1ull));
+      if (hit && at < DP_NODE_OBS) node_obs[i * DP_NODE_OBS + at] = (int16_t)o;
+      cnt += __popcll(m);
+    }
+    if (lane == 0) node_nobs[i] = cnt <= DP_NODE_OBS ? cnt : -1;
+  }
 }
 
 // Planar translate systems (unicycle1/2, car+trailer), FP64.
@@ -200,11 +232,21 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
         by0 = (a.sweep[2 * (int64_t)a.P + j] + oy) - DP_SWEEP_PAD;
         by1 = (a.sweep[3 * (int64_t)a.P + j] + oy) + DP_SWEEP_PAD;
       }
+      // scan the node's obstacle list (all obstacles when it overflowed)
+      int nlist = 0;
+      const int16_t* list = nullptr;
+      if (need_col) {
+        nlist = a.node_nobs[i];
+        if (nlist >= 0) list = a.node_obs + (int64_t)i * DP_NODE_OBS;
+        else nlist = nobs;
+      }
       int ncand = 0;
-      for (int ob = 0; ob < nobs; ob += LPP) {
-        const int o = ob + k0;
+      for (int ob = 0; __ballot(need_col && ob < nlist) != 0; ob += LPP) {
+        const int t = ob + k0;
         bool hit = false;
-        if (need_col && o < nobs) {
+        int o = 0;
+        if (need_col && t < nlist) {
+          o = list ? list[t] : t;
           const double* b = s_obs + 4 * o;
           hit = !(b[0] + b[2] < bx0 || b[0] - b[2] > bx1 || b[1] + b[3] < by0 || b[1] - b[3] > by1);
         }
@@ -408,6 +450,8 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   a.check_goal = p->check_intermediate_goal && p->num_check_goal > 0;
   a.obs = c->d_obs;
   a.nobs = c->nb;
+  a.node_obs = c->d_node_obs;
+  a.node_nobs = c->d_node_nobs;
   a.vis = c->d_vis_sorted;
   a.vstride = c->nv_cap;
   a.nv = c->nv;
@@ -449,6 +493,12 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   // The pair count is on the device; size the grid from the bound N*P.
   const int64_t groups = ((int64_t)N * c->P + seg - 1) / seg;
   const int blocks = dp_grid_blocks(groups, kWaves, c->tiles.check_blocks);
+  if (N > 0) {
+    k_node_obs<<<dp_grid_blocks(N, kWaves, 1024), 256, 0, s>>>(c->d_nodes, c->N_cap, N, c->d_obs, c->nb, c->reach_box[0],
+                                                              c->reach_box[1], c->reach_box[2], c->reach_box[3],
+                                                              c->d_node_obs, c->d_node_nobs);
+    DP_HIP_RET(hipGetLastError());
+  }
   rc = dispatch_nx(nx, [&](auto NXc) -> int {
     constexpr int NX = decltype(NXc)::value;
     if constexpr (NX == 13) {
