@@ -204,12 +204,6 @@ __global__ void k_conf(const dp_dev_sys* __restrict__ Sp, dp_grid_geom G, const 
   }
 }
 
-__device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
-  uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
-  uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
-  return ((uint64_t)hi << 32) | lo;
-}
-
 // Counters were zeroed at the start of the batch; nodes with pairs to decide
 // are counted into DPC_NODES_LEFT, empty nodes are finished here.
 __global__ void k_round_init(int32_t N, const int32_t* __restrict__ app, const int32_t* __restrict__ total,
@@ -270,6 +264,12 @@ __device__ __forceinline__ void poll_pause(int32_t w, int32_t cap) {
 // also stored for the round kernel, which reloads them in a later launch.
 template <int NX>
 __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, int32_t& count) {
+  // per chunk: same-chunk contender masks, undecided-entry masks, flags
+  // (1: an earlier contender is NOVEL, 2: an entry past the 64th is
+  // undecided), exclusive scan of list lengths, overflow offsets
+  __shared__ unsigned long long s_same[64], s_undec[64];
+  __shared__ int32_t s_pre[64], s_ov[64], s_flag[64];
+  constexpr int kCoop = 8;  // list entries per lane in flight
   const int lane = threadIdx.x & 63;
   const int32_t tot = *a.total;
   const int32_t end = a.app[node + 1] < tot ? a.app[node + 1] : tot;
@@ -326,43 +326,81 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
         }
         return !ext_dup;
       };
-      const int cnt = a.conf_cnt[p];
-      if (cnt >= 0 && polls == 0) {
-        // list entries and their verdicts, eight independent loads at a
-        // time.  Plain (cached) loads suffice: a decided verdict never
-        // changes, and a stale "undecided" is polled coherently later.
-        for (int g = 0; g < cnt; g += 8) {
-          int32_t os[8];
-          int rs[8];
+      // no stored list: scan the contender grid
+      double x[NX];
 #pragma unroll
-          for (int k = 0; k < 8; ++k) os[k] = g + k < cnt ? entry(p, g + k) : INT32_MAX;
-#pragma unroll
-          for (int k = 0; k < 8; ++k) rs[k] = os[k] < cb ? a.res[os[k]] : (int)DP_R_NONE;
-#pragma unroll
-          for (int k = 0; k < 8; ++k) {
-            if (os[k] == INT32_MAX) continue;
-            if (os[k] >= cb) {
-              same |= 1ull << (os[k] - cb);
-            } else if (rs[k] == DP_R_NOVEL) {
-              ext_dup = true;
-            } else if (rs[k] == DP_R_UNDECIDED) {
-              ext_block = true;
-              if (g + k < 64) undec |= 1ull << (g + k);
-              else more = true;
-            }
-          }
-        }
-      } else if (cnt >= 0) {
-        for (int e = 0; e < cnt && !ext_dup; ++e) visit(entry(p, e));
-      } else {
-        double x[NX];
-#pragma unroll
-        for (int k = 0; k < NX; ++k) x[k] = a.fin[(int64_t)k * a.pcap + p];
-        scan_earlier<NX>(*a.sys, a.grid, a.cstart, a.cidx, a.cx, a.pcap, x, p, a.r_dup, visit);
-      }
+      for (int k = 0; k < NX; ++k) x[k] = a.fin[(int64_t)k * a.pcap + p];
+      scan_earlier<NX>(*a.sys, a.grid, a.cstart, a.cidx, a.cx, a.pcap, x, p, a.r_dup, visit);
     };
     const unsigned long long t_eval = a.trace ? (unsigned long long)wall_clock64() : 0ull;
-    if (fr && !forced && !dupv) eval(p, 0);
+    const int cnt = fr && !forced && !dupv ? a.conf_cnt[p] : 0;
+    {
+      // Stored lists of the chunk's 64 pairs, evaluated by the whole wave:
+      // the entries are numbered consecutively over the pairs (exclusive
+      // scan of the counts) and lane l takes entries l, l+64, ..., so a long
+      // list costs no more than its share.  Plain (cached) loads suffice: a
+      // decided verdict never changes, and a stale "undecided" is polled
+      // coherently later.
+      const int n_l = cnt > 0 ? cnt : 0;
+      int incl = n_l;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+      }
+      const int total_e = __shfl(incl, 63, 64);
+      s_pre[lane] = incl - n_l;
+      s_ov[lane] = n_l > DP_CONF_SLOTS ? a.conf_ov[p] : 0;
+      s_same[lane] = 0;
+      s_undec[lane] = 0;
+      s_flag[lane] = 0;
+      __syncthreads();
+      for (int f0 = 0; f0 < total_e; f0 += 64 * kCoop) {
+        int32_t os[kCoop];
+        int rs[kCoop], own[kCoop], ee[kCoop];
+#pragma unroll
+        for (int k = 0; k < kCoop; ++k) {
+          const int f = f0 + 64 * k + lane;
+          own[k] = -1;
+          os[k] = INT32_MAX;
+          if (f < total_e) {
+            int lo = 0;
+#pragma unroll
+            for (int st = 32; st > 0; st >>= 1)  // lo + st <= 63
+              if (s_pre[lo + st] <= f) lo += st;
+            own[k] = lo;
+            ee[k] = f - s_pre[lo];
+            const int32_t q = cb + lo;
+            os[k] = ee[k] < DP_CONF_SLOTS ? a.conf[(int64_t)ee[k] * a.pcap + q]
+                                          : a.pool[(int64_t)s_ov[lo] + (ee[k] - DP_CONF_SLOTS)];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < kCoop; ++k) rs[k] = os[k] < cb ? a.res[os[k]] : (int)DP_R_NONE;
+#pragma unroll
+        for (int k = 0; k < kCoop; ++k) {
+          if (own[k] < 0) continue;
+          if (os[k] >= cb) {
+            atomicOr(&s_same[own[k]], 1ull << (os[k] - cb));
+          } else if (rs[k] == DP_R_NOVEL) {
+            atomicOr(&s_flag[own[k]], 1);
+          } else if (rs[k] == DP_R_UNDECIDED) {
+            if (ee[k] < 64) atomicOr(&s_undec[own[k]], 1ull << ee[k]);
+            else atomicOr(&s_flag[own[k]], 2);
+          }
+        }
+      }
+      __syncthreads();
+      if (cnt > 0) {
+        same = s_same[lane];
+        undec = s_undec[lane];
+        ext_dup = (s_flag[lane] & 1) != 0;
+        more = (s_flag[lane] & 2) != 0;
+        ext_block = !ext_dup && (undec != 0 || more);
+      }
+      __syncthreads();  // the arrays are rewritten by the next chunk
+    }
+    if (cnt < 0) eval(p, 0);
     if (a.trace) {
       __builtin_amdgcn_wave_barrier();
       if (lane == 0) a.trace[4 * (int64_t)node + 3] += (unsigned long long)wall_clock64() - t_eval;
@@ -370,11 +408,10 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
     const uint64_t free_m = __ballot(fr);
     const uint64_t forced_m = __ballot(fr && forced);
     const uint64_t dupv_m = __ballot(fr && dupv && !forced);
-    const uint64_t dup_m = __ballot(ext_dup);
-    const uint64_t block_m = __ballot(ext_block && !ext_dup);
+    uint64_t dup_m = __ballot(ext_dup);
+    uint64_t block_m = __ballot(ext_block && !ext_dup);
     uint64_t novel_m = 0, capped_m = 0, dupb_m = 0, dupvr_m = 0, pub_m = 0;
     int stop_lane = 64;
-    uint64_t rem = free_m;
     // Store the verdicts decided since the last call: other nodes' waves may
     // be waiting for them, so they go out before this wave waits itself.
     const uint64_t lane_bit = 1ull << lane;
@@ -389,23 +426,54 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
       }
       pub_m |= dec;
     };
-    while (rem) {
-      const int b = __ffsll((long long)rem) - 1;
-      rem &= rem - 1ull;
-      const uint64_t bit = 1ull << b;
-      if (a.limit > 0 && count + __popcll(novel_m) >= a.limit) capped_m |= bit;
-      else if (forced_m & bit) novel_m |= bit;
-      else if (dupv_m & bit) dupvr_m |= bit;
-      else if (dup_m & bit) dupb_m |= bit;
-      else if (block_m & bit) {
-        // Only this lane's verdict is needed next: evaluate it again,
-        // waiting for its undecided contenders.
-        publish();
+    // The sequential walk over the chunk, done wave-wide.  Below the first
+    // blocked pair fb every status is determined: forced pairs are NOVEL,
+    // duplicates of the visited set and pairs with a NOVEL earlier-chunk
+    // contender are decided, and the rest are NOVEL unless a same-chunk
+    // earlier contender is NOVEL.  That last rule follows the list order, a
+    // DAG on the lanes, so iterating it from any start reaches its unique
+    // fixed point within (longest chain + 1) rounds.  The branching cap then
+    // cuts after the (limit - count)-th NOVEL pair: every later FREE pair is
+    // CAPPED, including fb, which then needs no wait.
+    for (;;) {
+      const int fb = block_m ? __ffsll((long long)block_m) - 1 : 64;
+      const uint64_t below = fb >= 64 ? ~0ull : ((1ull << fb) - 1ull);
+      const uint64_t forced_b = forced_m & below;
+      const uint64_t cand = free_m & below & ~forced_m & ~dupv_m & ~dup_m;
+      const bool mine = (cand & lane_bit) != 0;
+      uint64_t nov = forced_b | cand;
+      for (int it = 0; it <= 64; ++it) {
+        const uint64_t nx = forced_b | __ballot(mine && (same & nov) == 0);
+        if (nx == nov) break;
+        nov = nx;
+      }
+      if (a.limit > 0 && count + __popcll(nov) >= a.limit) {
+        // lanes up to the (limit - count)-th NOVEL keep their status
+        uint64_t m = nov;
+        for (int t = count + 1; t < a.limit; ++t) m &= m - 1ull;
+        const int c = __ffsll((long long)m) - 1;
+        const uint64_t upto = c >= 63 ? ~0ull : ((2ull << c) - 1ull);
+        novel_m = nov & upto;
+        dupvr_m = dupv_m & upto;
+        dupb_m = free_m & upto & ~novel_m & ~dupvr_m;
+        capped_m = free_m & ~upto;
+        break;
+      }
+      novel_m = nov;
+      dupvr_m = dupv_m & below;
+      dupb_m = free_m & below & ~nov & ~dupv_m;
+      if (fb >= 64) break;
+      // Only lane fb's verdict is needed next: wait for its undecided
+      // contenders, after publishing what is decided (other nodes' waves
+      // may be waiting for it).
+      publish();
         bool d = false, blk = false;
-        if (a.wait > 0 && lane == b && undec && !more) {
+        if (a.wait > 0 && lane == fb && cnt >= 0) {
           // stored list: poll the undecided contenders, eight at a time,
-          // until one is NOVEL or all are decided
+          // until one is NOVEL or all are decided (entries past the 64th
+          // are re-read each round)
           uint64_t m = undec;
+          bool mo = more;
           int32_t w = 0;
           const unsigned long long t_wait = a.trace ? (unsigned long long)wall_clock64() : 0ull;
           for (;;) {
@@ -424,7 +492,22 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
                 else if (r != DP_R_UNDECIDED) m &= ~(1ull << (g + k));
               }
             }
-            if (d || !m) break;
+            if (mo && !d) {
+              mo = false;
+              for (int g = 64; g < cnt && !d; g += 8) {
+                int32_t oo[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) oo[k] = g + k < cnt ? entry(p, g + k) : INT32_MAX;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                  if (oo[k] >= cb) continue;  // past the list, or in this chunk
+                  const int r = __hip_atomic_load(&a.res[oo[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                  if (r == DP_R_NOVEL) d = true;
+                  else if (r == DP_R_UNDECIDED) mo = true;
+                }
+              }
+            }
+            if (d || (!m && !mo)) break;
             if (++w > a.wait) {
               blk = true;
               break;
@@ -432,7 +515,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
             poll_pause(w, a.poll_cap);
           }
           if (a.trace) a.trace[4 * (int64_t)node + 2] += (unsigned long long)wall_clock64() - t_wait;
-        } else if (a.wait > 0 && lane == b) {
+        } else if (a.wait > 0 && lane == fb) {
           if (!more) {
             // wait for the recorded contenders only
             for (int e = 0; e < nblk && !d && !blk; ++e) {
@@ -455,15 +538,12 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
             blk = ext_block && !ext_dup;
           }
         }
-        const bool dd = __ballot(d) != 0;
-        if (a.wait <= 0 || __ballot(blk) != 0) {
-          stop_lane = b;
-          break;
-        }
-        if (dd || (readlane64(same, b) & novel_m)) dupb_m |= bit;
-        else novel_m |= bit;
-      } else if (readlane64(same, b) & novel_m) dupb_m |= bit;
-      else novel_m |= bit;
+      if (a.wait <= 0 || __ballot(blk) != 0) {
+        stop_lane = fb;
+        break;
+      }
+      block_m &= ~(1ull << fb);
+      if (__ballot(d) != 0) dup_m |= 1ull << fb;
     }
     publish();
     count += __popcll(novel_m);
