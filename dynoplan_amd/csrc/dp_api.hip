@@ -890,14 +890,14 @@ int dp_debug_resolve_trace(dp_ctx* c, int32_t enable, uint64_t* out, int64_t n) 
   HIP_CHECK(hipSetDevice(c->device));
   HIP_CHECK(hipDeviceSynchronize());
   if (out && n > 0) {
-    ARG_CHECK(c->d_trace && n <= c->trace_cap / 4, "no trace of that size");
-    HIP_CHECK(hipMemcpy(out, c->d_trace, sizeof(uint64_t) * 4 * n, hipMemcpyDeviceToHost));
+    ARG_CHECK(c->d_trace && n <= c->trace_cap / DP_TRACE_FIELDS, "no trace of that size");
+    HIP_CHECK(hipMemcpy(out, c->d_trace, sizeof(uint64_t) * DP_TRACE_FIELDS * n, hipMemcpyDeviceToHost));
     int khz = 0;
     HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
-    out[4 * n] = (uint64_t)khz;
+    out[DP_TRACE_FIELDS * n] = (uint64_t)khz;
   }
   if (enable > 0) {
-    const int64_t need = 4 * (int64_t)std::max(c->N_cap, 1);
+    const int64_t need = DP_TRACE_FIELDS * (int64_t)std::max(c->N_cap, 1);
     if (need > c->trace_cap) {
       int rc = dev_alloc(&c->d_trace, need);
       if (rc) return rc;

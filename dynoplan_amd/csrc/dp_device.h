@@ -154,6 +154,34 @@ __device__ __forceinline__ bool dp_within_col(const dp_dev_sys& S, const double*
   return dp_within(S, x, y, r);
 }
 
+// Planar systems: a bound T on the squared position difference such that
+// dx*dx + dy*dy > T (as dp_l2_2 forms it) implies w0 * sqrt(...) > r after
+// rounding: the relative margin 1e-9 exceeds the few ulps that sqrt and the
+// product can lose.
+DP_HD double dp_pos_bound(const dp_dev_sys& S, double r) {
+  if (!(S.w[0] > 0.0)) return INFINITY;
+  const double t = r / S.w[0];
+  return t * t * (1.0 + 1e-9);
+}
+
+// dp_within_col with two exact rejections ahead of it for planar systems:
+// the squared position difference above T (dp_pos_bound), or the heading
+// term w1 * so2 alone above r (dp_distance adds non-negative terms to it, and
+// IEEE addition is monotone).  Neither changes the answer; both skip the
+// square root for most candidates of a grid scan.
+template <int NX>
+__device__ __forceinline__ bool dp_within_col_pf(const dp_dev_sys& S, const double* x, const double* col,
+                                                 int64_t stride, int64_t i, double r, double T) {
+  if constexpr (NX != 13) {
+    const double dx = x[0] - col[i], dy = x[1] - col[stride + i];
+    double s = dx * dx;
+    s = s + dy * dy;
+    if (s > T) return false;
+    if (S.w[1] * dp_so2(x[2], col[2 * stride + i]) > r) return false;
+  }
+  return dp_within_col<NX>(S, x, col, stride, i, r);
+}
+
 DP_HD double dp_max(double a, double b) { return b > a ? b : a; }
 
 // robot->lower_bound_time.

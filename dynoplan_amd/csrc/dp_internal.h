@@ -24,6 +24,9 @@
 // for an undecided contender of an earlier node, and the retries of a node
 // whose waits all ran out (bounds the kernel's run time if a wait never ends).
 #define DP_WAIT_POLLS (1 << 14)
+// Resolution trace, per node: start and end clock of its wave, cycles
+// waiting, evaluating lists and walking chunks, chunks, waits, polls.
+#define DP_TRACE_FIELDS 8
 #define DP_SPIN_LIMIT_WAITING (1 << 10)
 // Earlier-conflict slots stored per pair (more: the resolution scans the grid).
 #define DP_CONF_SLOTS 32
@@ -36,7 +39,7 @@ void dp_set_error(const std::string& msg);
 enum {
   DPC_PAIRS = 0, DPC_BOUNDS, DPC_COLLIDE, DPC_FREE, DPC_SURR, DPC_NOVEL, DPC_DUPV, DPC_DUPB,
   DPC_CAPPED, DPC_GOAL, DPC_CHECKED, DPC_RECHECK, DPC_ROUNDS, DPC_ERRFLAG, DPC_OVERFLOW,
-  DPC_NODES_LEFT, DPC_CONF, DPC_TICKET, DPC_COUNT
+  DPC_NODES_LEFT, DPC_CONF, DPC_TICKET, DPC_CELLS, DPC_COUNT
 };
 
 // Uniform grid used by the duplicate tests.  Axes 0,1 are the position

@@ -126,14 +126,15 @@ __device__ __forceinline__ void load_translated(const dp_check_args& a, int64_t 
 // Any visited state within r of x in neighbour cell cq of x's cell?
 template <int NX>
 __device__ __forceinline__ bool grid_cell_any(const dp_dev_sys& S, const dp_grid_geom& G, const int32_t* start,
-                                              const double* pts, int64_t stride, const double* x, int cq, double r) {
+                                              const double* pts, int64_t stride, const double* x, int cq, double r,
+                                              double T) {
   int32_t c[3];
   dp_cell_coords(G, x, c);
   const int64_t cell = dp_cell_neighbor(G, c, cq);
   if (cell < 0) return false;
   const int32_t e = start[cell + 1];
   for (int32_t p = start[cell]; p < e; ++p)
-    if (dp_within_col<NX>(S, x, pts, stride, p, r)) return true;
+    if (dp_within_col_pf<NX>(S, x, pts, stride, p, r, T)) return true;
   return false;
 }
 
@@ -188,6 +189,7 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
   int16_t* cand = s_cand + (int64_t)(wave * SEG + seg) * nobs;
   const int32_t total = *a.total;
   const int ncq = 27;
+  const double Tdup = dp_pos_bound(S, a.r_dup);
   long long c_b = 0, c_c = 0, c_f = 0, c_g = 0, c_k = 0;
 
   for (int64_t q0 = ((int64_t)blockIdx.x * kWaves + wave) * SEG; q0 < total;
@@ -323,7 +325,8 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
       bool hit = false;
       for (int cb = 0; cb < ncq; cb += LPP) {
         const int cq = cb + k0;
-        if (fr && cq < ncq && !hit) hit = grid_cell_any<NX>(S, a.grid, a.vstart, a.vis, a.vstride, xf, cq, a.r_dup);
+        if (fr && cq < ncq && !hit)
+          hit = grid_cell_any<NX>(S, a.grid, a.vstart, a.vis, a.vstride, xf, cq, a.r_dup, Tdup);
       }
       dupv = ((__ballot(hit) >> shift) & segbits) != 0;
     }

@@ -135,6 +135,7 @@ __device__ __forceinline__ void scan_earlier(const dp_dev_sys& S, const dp_grid_
                                              int64_t q, double r, F&& f) {
   int32_t c[3];
   dp_cell_coords(G, x, c);
+  const double T = dp_pos_bound(S, r);
   for (int cq = 0; cq < 27; ++cq) {
     const int64_t cell = dp_cell_neighbor(G, c, cq);
     if (cell < 0) continue;
@@ -142,7 +143,7 @@ __device__ __forceinline__ void scan_earlier(const dp_dev_sys& S, const dp_grid_
     for (int32_t t = cstart[cell]; t < e; ++t) {
       const int32_t o = cidx[t];
       if (o >= q) continue;
-      if (!dp_within_col<NX>(S, x, cx, pcap, t, r)) continue;
+      if (!dp_within_col_pf<NX>(S, x, cx, pcap, t, r, T)) continue;
       if (!f(o)) return;
     }
   }
@@ -153,54 +154,225 @@ __device__ __forceinline__ void scan_earlier(const dp_dev_sys& S, const dp_grid_
 // neither goal-forced nor duplicates of the visited set, at position < window
 // in their node.  conf_cnt = -1 marks a pair whose list is not stored
 // (outside the window or overflowing); the resolution scans the grid for it.
+// Lists longer than DP_CONF_SLOTS continue in the overflow pool; the order of
+// a list's entries carries no meaning.
+
+// The list of contender t (position t of the cell-ordered contender array)
+// by one thread scanning the 27 cells around it.
 template <int NX>
-__global__ void k_conf(const dp_dev_sys* __restrict__ Sp, dp_grid_geom G, const int32_t* __restrict__ total,
-                       const int32_t* __restrict__ pnode, const int32_t* __restrict__ app,
-                       const int8_t* __restrict__ verdict, const int8_t* __restrict__ flags,
-                       const double* __restrict__ fin, int64_t pcap, const int32_t* __restrict__ cstart,
-                       const int32_t* __restrict__ cidx, const double* __restrict__ cx, double r, int32_t window,
-                       int16_t* __restrict__ conf_cnt, int32_t* __restrict__ conf, int32_t* __restrict__ conf_ov,
-                       int32_t* __restrict__ pool, int64_t pool_cap, long long* __restrict__ counters) {
-  const dp_dev_sys& S = *Sp;
-  // Contenders in cell order: neighbouring threads scan overlapping cell
-  // blocks, so the grid reads are shared through the caches.  Every checker
-  // (a FREE pair neither goal-forced nor a duplicate of the visited set) is a
-  // contender.
-  const int64_t n = cstart[G.ncells];
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t q = cidx[t];
-    if (flags[q] & 2) continue;  // goal-forced: NOVEL without a list
-    if (window > 0 && q - app[pnode[q]] >= window) {
-      conf_cnt[q] = -1;
-      continue;
-    }
-    double x[NX];
+__device__ void conf_one(const dp_dev_sys& S, const dp_grid_geom& G, const int32_t* __restrict__ pnode,
+                         const int32_t* __restrict__ app, const int8_t* __restrict__ flags, int64_t pcap,
+                         const int32_t* __restrict__ cstart, const int32_t* __restrict__ cidx,
+                         const double* __restrict__ cx, double r, int32_t window, int64_t t,
+                         int16_t* __restrict__ conf_cnt, int32_t* __restrict__ conf, int32_t* __restrict__ conf_ov,
+                         int32_t* __restrict__ pool, int64_t pool_cap, long long* __restrict__ counters) {
+  const int32_t q = cidx[t];
+  if (flags[q] & 2) return;  // goal-forced: NOVEL without a list
+  if (window > 0 && q - app[pnode[q]] >= window) {
+    conf_cnt[q] = -1;
+    return;
+  }
+  double x[NX];
 #pragma unroll
-    for (int k = 0; k < NX; ++k) x[k] = cx[(int64_t)k * pcap + t];
-    int cnt = 0;
-    scan_earlier<NX>(S, G, cstart, cidx, cx, pcap, x, q, r, [&](int32_t o) {
-      if (cnt < DP_CONF_SLOTS) conf[(int64_t)cnt * pcap + q] = o;
-      ++cnt;
-      return cnt < DP_CONF_MAX;
-    });
-    if (cnt > DP_CONF_SLOTS && cnt < DP_CONF_MAX) {
-      // the rest of the list goes to the overflow pool; the scan order is
-      // the same, so a second scan lists the same entries
-      const long long off =
-          (long long)atomicAdd((unsigned long long*)&counters[DPC_CONF], (unsigned long long)(cnt - DP_CONF_SLOTS));
-      if (off + cnt - DP_CONF_SLOTS <= pool_cap) {
-        int k = 0;
-        scan_earlier<NX>(S, G, cstart, cidx, cx, pcap, x, q, r, [&](int32_t o) {
-          if (k >= DP_CONF_SLOTS) pool[off + (k - DP_CONF_SLOTS)] = o;
-          ++k;
-          return k < cnt;
-        });
-        conf_ov[q] = (int32_t)off;
-      } else {
-        cnt = DP_CONF_MAX;  // pool exhausted: scan the grid when reached
-      }
+  for (int k = 0; k < NX; ++k) x[k] = cx[(int64_t)k * pcap + t];
+  int cnt = 0;
+  scan_earlier<NX>(S, G, cstart, cidx, cx, pcap, x, q, r, [&](int32_t o) {
+    if (cnt < DP_CONF_SLOTS) conf[(int64_t)cnt * pcap + q] = o;
+    ++cnt;
+    return cnt < DP_CONF_MAX;
+  });
+  if (cnt > DP_CONF_SLOTS && cnt < DP_CONF_MAX) {
+    // the rest of the list goes to the overflow pool; the scan order is
+    // the same, so a second scan lists the same entries
+    const long long off =
+        (long long)atomicAdd((unsigned long long*)&counters[DPC_CONF], (unsigned long long)(cnt - DP_CONF_SLOTS));
+    if (off + cnt - DP_CONF_SLOTS <= pool_cap) {
+      int k = 0;
+      scan_earlier<NX>(S, G, cstart, cidx, cx, pcap, x, q, r, [&](int32_t o) {
+        if (k >= DP_CONF_SLOTS) pool[off + (k - DP_CONF_SLOTS)] = o;
+        ++k;
+        return k < cnt;
+      });
+      conf_ov[q] = (int32_t)off;
+    } else {
+      cnt = DP_CONF_MAX;  // pool exhausted: scan the grid when reached
     }
-    conf_cnt[q] = (int16_t)(cnt < DP_CONF_MAX ? cnt : -1);
+  }
+  conf_cnt[q] = (int16_t)(cnt < DP_CONF_MAX ? cnt : -1);
+}
+
+// One thread per contender, in cell order: neighbouring threads scan
+// overlapping cell blocks, so the grid reads are shared through the caches.
+template <int NX>
+__global__ void k_conf(const dp_dev_sys* __restrict__ Sp, dp_grid_geom G, const int32_t* __restrict__ pnode,
+                       const int32_t* __restrict__ app, const int8_t* __restrict__ flags, int64_t pcap,
+                       const int32_t* __restrict__ cstart, const int32_t* __restrict__ cidx,
+                       const double* __restrict__ cx, double r, int32_t window, int16_t* __restrict__ conf_cnt,
+                       int32_t* __restrict__ conf, int32_t* __restrict__ conf_ov, int32_t* __restrict__ pool,
+                       int64_t pool_cap, long long* __restrict__ counters) {
+  const int64_t n = cstart[G.ncells];
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    conf_one<NX>(*Sp, G, pnode, app, flags, pcap, cstart, cidx, cx, r, window, t, conf_cnt, conf, conf_ov, pool,
+                 pool_cap, counters);
+}
+
+// The non-empty cells of the contender grid, in no particular order.
+__global__ void k_cell_list(int64_t ncells, const int32_t* __restrict__ cstart, int32_t* __restrict__ list,
+                            long long* __restrict__ counters) {
+  const int lane = threadIdx.x % 64;
+  for (int64_t c0 = (int64_t)blockIdx.x * blockDim.x; c0 < ncells; c0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = c0 + threadIdx.x;
+    const bool ne = c < ncells && cstart[c + 1] > cstart[c];
+    const uint64_t m = __ballot(ne);
+    if (!m) continue;
+    long long base = 0;
+    if (lane == 0) base = (long long)atomicAdd((unsigned long long*)&counters[DPC_CELLS], (unsigned long long)__popcll(m));
+    base = __shfl(base, 0, 64);
+    if (ne) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)c;
+  }
+}
+
+// Planar systems: one workgroup per non-empty grid cell.  The 27 cells
+// around it form 9 runs of 3 cells consecutive in memory (x is the fastest
+// grid dimension).  Their contenders, at most kConfStage, are staged in LDS,
+// and each of the cell's own contenders is tested against all of them by the
+// lanes of one wave, with ballot compaction into its list.  A larger
+// neighbourhood is tested the same way from global memory.
+constexpr int kConfStage = 768;
+template <int NX>
+__global__ __launch_bounds__(256) void k_conf_cells(const dp_dev_sys* __restrict__ Sp, dp_grid_geom G,
+                                                    const int8_t* __restrict__ flags, int64_t pcap,
+                                                    const int32_t* __restrict__ cstart,
+                                                    const int32_t* __restrict__ cidx, const double* __restrict__ cx,
+                                                    const int32_t* __restrict__ cells, double r,
+                                                    int16_t* __restrict__ conf_cnt, int32_t* __restrict__ conf,
+                                                    int32_t* __restrict__ conf_ov, int32_t* __restrict__ pool,
+                                                    int64_t pool_cap, long long* __restrict__ counters) {
+  __shared__ double s_x[NX * kConfStage];
+  __shared__ int32_t s_idx[kConfStage];
+  __shared__ int8_t s_fl[kConfStage];
+  __shared__ int32_t s_run[9], s_pre[10];
+  const dp_dev_sys& S = *Sp;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const uint64_t ltmask = (1ull << lane) - 1ull;
+  const long long ncl = counters[DPC_CELLS];
+  const double T = dp_pos_bound(S, r);
+  const int64_t d01 = (int64_t)G.dims[0] * G.dims[1];
+  for (long long w = blockIdx.x; w < ncl; w += gridDim.x) {
+    const int64_t cell = cells[w];
+    const int32_t c0 = (int32_t)(cell % G.dims[0]), c1 = (int32_t)((cell / G.dims[0]) % G.dims[1]);
+    const int32_t c2 = (int32_t)(cell / d01);
+    if (wave == 0) {
+      // run k: rows c1 + k%3 - 1, headings c2 + k/3 - 1 (the rules of dp_cell_neighbor)
+      int n = 0, st = 0;
+      if (lane < 9) {
+        const int32_t n1 = c1 + lane % 3 - 1;
+        const int32_t d2 = lane / 3 - 1;
+        int32_t n2 = c2 + d2;
+        bool ok = n1 >= 0 && n1 < G.dims[1];
+        if (G.dims[2] == 1) {
+          ok = ok && d2 == 0;
+          n2 = 0;
+        } else if (G.periodic2) {
+          n2 = n2 < 0 ? n2 + G.dims[2] : (n2 >= G.dims[2] ? n2 - G.dims[2] : n2);
+        } else {
+          ok = ok && n2 >= 0 && n2 < G.dims[2];
+        }
+        if (ok) {
+          const int64_t row = ((int64_t)n2 * G.dims[1] + n1) * G.dims[0];
+          const int32_t x0 = c0 > 0 ? c0 - 1 : 0, x1 = c0 + 1 < G.dims[0] ? c0 + 1 : G.dims[0] - 1;
+          st = cstart[row + x0];
+          n = cstart[row + x1 + 1] - st;
+        }
+      }
+      int incl = n;
+#pragma unroll
+      for (int d = 1; d < 16; d <<= 1) {
+        const int v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+      }
+      if (lane < 9) {
+        s_run[lane] = st;
+        s_pre[lane] = incl - n;
+      }
+      if (lane == 8) s_pre[9] = incl;
+    }
+    __syncthreads();
+    const int K = s_pre[9];
+    const int32_t b = cstart[cell], e = cstart[cell + 1];
+    // staged position of the cell's first contender: inside run 4 (same row and heading)
+    const int jb = s_pre[4] + (b - s_run[4]);
+    auto locate = [&](int j) -> int64_t {
+      int k = 0;
+#pragma unroll
+      for (int st = 8; st > 0; st >>= 1)  // s_pre[9], the total, exceeds j
+        if (k + st <= 9 && s_pre[k + st] <= j) k += st;
+      return (int64_t)s_run[k] + (j - s_pre[k]);
+    };
+    const bool staged = K <= kConfStage;
+    if (staged) {
+      for (int j = threadIdx.x; j < K; j += blockDim.x) {
+        const int64_t t = locate(j);
+        const int32_t o = cidx[t];
+        s_idx[j] = o;
+#pragma unroll
+        for (int d = 0; d < NX; ++d) s_x[d * kConfStage + j] = cx[(int64_t)d * pcap + t];
+        if (t >= b && t < e) s_fl[j] = flags[o];
+      }
+      __syncthreads();
+    }
+    for (int64_t i = b + wave; i < e; i += blockDim.x / 64) {
+      const int js = jb + (int)(i - b);
+      const int32_t q = staged ? s_idx[js] : cidx[i];
+      if ((staged ? s_fl[js] : flags[q]) & 2) continue;  // goal-forced: NOVEL without a list
+      double x[NX];
+#pragma unroll
+      for (int k = 0; k < NX; ++k) x[k] = staged ? s_x[k * kConfStage + js] : cx[(int64_t)k * pcap + i];
+      // candidate j: staged, or read through its grid position
+      auto test = [&](int j, int32_t& o) -> bool {
+        if (j >= K) return false;
+        if (staged) {
+          o = s_idx[j];
+          return o < q && dp_within_col_pf<NX>(S, x, s_x, kConfStage, j, r, T);
+        }
+        const int64_t t = locate(j);
+        o = cidx[t];
+        return o < q && dp_within_col_pf<NX>(S, x, cx, pcap, t, r, T);
+      };
+      int cnt = 0;
+      for (int j0 = 0; j0 < K; j0 += 64) {
+        int32_t o = 0;
+        const bool hit = test(j0 + lane, o);
+        const uint64_t m = __ballot(hit);
+        const int pos = cnt + __popcll(m & ltmask);
+        if (hit && pos < DP_CONF_SLOTS) conf[(int64_t)pos * pcap + q] = o;
+        cnt += __popcll(m);
+      }
+      if (cnt > DP_CONF_SLOTS && cnt < DP_CONF_MAX) {
+        // the rest goes to the overflow pool, listed by a second pass in the same order
+        long long off = 0;
+        if (lane == 0)
+          off = (long long)atomicAdd((unsigned long long*)&counters[DPC_CONF],
+                                     (unsigned long long)(cnt - DP_CONF_SLOTS));
+        off = __shfl(off, 0, 64);
+        if (off + cnt - DP_CONF_SLOTS <= pool_cap) {
+          int k = 0;
+          for (int j0 = 0; j0 < K; j0 += 64) {
+            int32_t o = 0;
+            const bool hit = test(j0 + lane, o);
+            const uint64_t m = __ballot(hit);
+            const int pos = k + __popcll(m & ltmask);
+            if (hit && pos >= DP_CONF_SLOTS) pool[off + (pos - DP_CONF_SLOTS)] = o;
+            k += __popcll(m);
+          }
+          if (lane == 0) conf_ov[q] = (int32_t)off;
+        } else {
+          cnt = DP_CONF_MAX;  // pool exhausted: scan the grid when reached
+        }
+      }
+      if (lane == 0) conf_cnt[q] = (int16_t)(cnt < DP_CONF_MAX ? cnt : -1);
+    }
+    __syncthreads();  // the staging arrays are rewritten for the next cell
   }
 }
 
@@ -245,7 +417,7 @@ struct round_args {
   int32_t limit;
   int32_t wait;      // polls a lane may spend on one undecided earlier contender
   int32_t poll_cap;  // longest pause between polls, in 64-cycle sleeps
-  unsigned long long* trace;  // optional [2N]: GPU clock when each node's wave starts and ends
+  unsigned long long* trace;  // optional [DP_TRACE_FIELDS * N], fields listed at DP_TRACE_FIELDS
   long long* counters;
 };
 
@@ -275,17 +447,42 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
   const int32_t end = a.app[node + 1] < tot ? a.app[node + 1] : tot;
   if (cur >= end) return true;
   long long n_nov = 0, n_cap = 0, n_dv = 0, n_db = 0;
+  // diagnostics (trace only): cycles evaluating lists, cycles in the
+  // in-chunk walk, cycles waiting, chunks, waits, polls
+  unsigned long long tr_eval = 0, tr_walk = 0, tr_wait = 0, tr_chunks = 0, tr_waits = 0, tr_polls = 0;
   bool stopped = false;
+  // per-pair inputs of a chunk (verdict, flags, list length, overflow
+  // offset) are loaded together, and those of the next chunk while this one
+  // is decided; the values of pairs that are not contenders go unused
+  int8_t m_vd = -1, m_fl = 0;
+  int16_t m_cc = 0;
+  int32_t m_ov = 0;
+  if (cur + lane < end) {
+    m_vd = a.verdict[cur + lane];
+    m_fl = a.flags[cur + lane];
+    m_cc = a.conf_cnt[cur + lane];
+    m_ov = a.conf_ov[cur + lane];
+  }
   for (int32_t cb = cur; cb < end; cb += 64) {
     const int32_t p = cb + lane;
-    const bool fr = p < end && a.verdict[p] == DP_V_FREE;
+    const int8_t vd_p = m_vd, fl_p = m_fl;
+    const int16_t cc_p = m_cc;
+    const int32_t ov_p = m_ov;
+    m_vd = -1;
+    if (p + 64 < end) {
+      m_vd = a.verdict[p + 64];
+      m_fl = a.flags[p + 64];
+      m_cc = a.conf_cnt[p + 64];
+      m_ov = a.conf_ov[p + 64];
+    }
+    const bool fr = p < end && vd_p == DP_V_FREE;
     if (a.limit > 0 && count >= a.limit) {
       // the reference has left this node's loop: every later FREE pair is capped
       if (fr) __hip_atomic_store(&a.res[p], (int)DP_R_CAPPED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       n_cap += __popcll(__ballot(fr));
       continue;
     }
-    const int8_t fl = fr ? a.flags[p] : 0;
+    const int8_t fl = fr ? fl_p : 0;
     const bool forced = (fl & 2) != 0;
     const bool dupv = (fl & 1) != 0;
     bool ext_dup = false, ext_block = false;
@@ -297,9 +494,9 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
     int nblk = 0;
     bool more = false;
     uint64_t undec = 0;  // entries (< 64) of a stored list whose contender was undecided
-    // entry e of pair q's stored earlier-conflict list
-    auto entry = [&](int32_t q, int e) -> int32_t {
-      return e < DP_CONF_SLOTS ? a.conf[(int64_t)e * a.pcap + q] : a.pool[(int64_t)a.conf_ov[q] + (e - DP_CONF_SLOTS)];
+    // entry e of this lane's stored earlier-conflict list
+    auto entry = [&](int e) -> int32_t {
+      return e < DP_CONF_SLOTS ? a.conf[(int64_t)e * a.pcap + p] : a.pool[(int64_t)ov_p + (e - DP_CONF_SLOTS)];
     };
     // Earlier contenders of pair p: those in this chunk go to `same` (the
     // walk below decides them in order); an earlier-chunk contender already
@@ -333,7 +530,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
       scan_earlier<NX>(*a.sys, a.grid, a.cstart, a.cidx, a.cx, a.pcap, x, p, a.r_dup, visit);
     };
     const unsigned long long t_eval = a.trace ? (unsigned long long)wall_clock64() : 0ull;
-    const int cnt = fr && !forced && !dupv ? a.conf_cnt[p] : 0;
+    const int cnt = fr && !forced && !dupv ? cc_p : 0;
     {
       // Stored lists of the chunk's 64 pairs, evaluated by the whole wave:
       // the entries are numbered consecutively over the pairs (exclusive
@@ -350,7 +547,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
       }
       const int total_e = __shfl(incl, 63, 64);
       s_pre[lane] = incl - n_l;
-      s_ov[lane] = n_l > DP_CONF_SLOTS ? a.conf_ov[p] : 0;
+      s_ov[lane] = n_l > DP_CONF_SLOTS ? ov_p : 0;
       s_same[lane] = 0;
       s_undec[lane] = 0;
       s_flag[lane] = 0;
@@ -401,9 +598,11 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
       __syncthreads();  // the arrays are rewritten by the next chunk
     }
     if (cnt < 0) eval(p, 0);
+    unsigned long long t_walk = 0;
     if (a.trace) {
-      __builtin_amdgcn_wave_barrier();
-      if (lane == 0) a.trace[4 * (int64_t)node + 3] += (unsigned long long)wall_clock64() - t_eval;
+      t_walk = (unsigned long long)wall_clock64();
+      tr_eval += t_walk - t_eval;
+      ++tr_chunks;
     }
     const uint64_t free_m = __ballot(fr);
     const uint64_t forced_m = __ballot(fr && forced);
@@ -435,6 +634,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
     // fixed point within (longest chain + 1) rounds.  The branching cap then
     // cuts after the (limit - count)-th NOVEL pair: every later FREE pair is
     // CAPPED, including fb, which then needs no wait.
+    bool refreshed = false;
     for (;;) {
       const int fb = block_m ? __ffsll((long long)block_m) - 1 : 64;
       const uint64_t below = fb >= 64 ? ~0ull : ((1ull << fb) - 1ull);
@@ -467,6 +667,40 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
       // contenders, after publishing what is decided (other nodes' waves
       // may be waiting for it).
       publish();
+      if (!refreshed) {
+        // First read the undecided contenders of every blocked lane again,
+        // all lanes at once: contenders decided since the chunk's list
+        // evaluation then cost one round trip instead of one wait each.
+        refreshed = true;
+        bool nov_found = false, still = false;
+        if ((block_m & lane_bit) && cnt >= 0) {
+          for (int g = 0; g < 64 && !nov_found; g += 8) {
+            if (!((undec >> g) & 0xffull)) continue;
+            int32_t oo[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) oo[k] = (undec >> (g + k)) & 1ull ? entry(g + k) : -1;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+              if (oo[k] < 0) continue;
+              const int r = __hip_atomic_load(&a.res[oo[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              if (r == DP_R_NOVEL) nov_found = true;
+              else if (r != DP_R_UNDECIDED) undec &= ~(1ull << (g + k));
+            }
+          }
+          still = !nov_found && (undec != 0 || more);
+        } else if (block_m & lane_bit) {
+          still = true;  // grid-scanned lane: its wait re-scans
+        }
+        dup_m |= __ballot(nov_found);
+        const uint64_t nb = __ballot(still);
+        if (!(nb & (1ull << fb))) {
+          block_m = nb;
+          continue;  // fb is decided now: walk on
+        }
+        block_m = nb;
+      }
+      int32_t polled = 0;
+      const unsigned long long t_wait = a.trace ? (unsigned long long)wall_clock64() : 0ull;
         bool d = false, blk = false;
         if (a.wait > 0 && lane == fb && cnt >= 0) {
           // stored list: poll the undecided contenders, eight at a time,
@@ -475,13 +709,12 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
           uint64_t m = undec;
           bool mo = more;
           int32_t w = 0;
-          const unsigned long long t_wait = a.trace ? (unsigned long long)wall_clock64() : 0ull;
           for (;;) {
             for (int g = 0; g < 64 && !d; g += 8) {
               if (!((m >> g) & 0xffull)) continue;
               int32_t oo[8];
 #pragma unroll
-              for (int k = 0; k < 8; ++k) oo[k] = (m >> (g + k)) & 1ull ? entry(p, g + k) : -1;
+              for (int k = 0; k < 8; ++k) oo[k] = (m >> (g + k)) & 1ull ? entry(g + k) : -1;
 #pragma unroll
               for (int k = 0; k < 8; ++k) {
                 const int r = oo[k] >= 0
@@ -497,7 +730,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
               for (int g = 64; g < cnt && !d; g += 8) {
                 int32_t oo[8];
 #pragma unroll
-                for (int k = 0; k < 8; ++k) oo[k] = g + k < cnt ? entry(p, g + k) : INT32_MAX;
+                for (int k = 0; k < 8; ++k) oo[k] = g + k < cnt ? entry(g + k) : INT32_MAX;
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                   if (oo[k] >= cb) continue;  // past the list, or in this chunk
@@ -514,7 +747,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
             }
             poll_pause(w, a.poll_cap);
           }
-          if (a.trace) a.trace[4 * (int64_t)node + 2] += (unsigned long long)wall_clock64() - t_wait;
+          polled = w;
         } else if (a.wait > 0 && lane == fb) {
           if (!more) {
             // wait for the recorded contenders only
@@ -538,14 +771,23 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
             blk = ext_block && !ext_dup;
           }
         }
+      if (a.trace) {
+        const unsigned long long t1 = (unsigned long long)wall_clock64();
+        tr_wait += t1 - t_wait;
+        t_walk += t1 - t_wait;  // the walk clock excludes the wait
+        ++tr_waits;
+        tr_polls += (unsigned long long)__shfl(polled, fb, 64);
+      }
       if (a.wait <= 0 || __ballot(blk) != 0) {
         stop_lane = fb;
         break;
       }
       block_m &= ~(1ull << fb);
       if (__ballot(d) != 0) dup_m |= 1ull << fb;
+      refreshed = false;  // later blocked lanes may have been decided meanwhile
     }
     publish();
+    if (a.trace) tr_walk += (unsigned long long)wall_clock64() - t_walk;
     count += __popcll(novel_m);
     n_nov += __popcll(novel_m);
     n_cap += __popcll(capped_m);
@@ -558,6 +800,15 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
     }
   }
   if (!stopped) cur = end;
+  if (a.trace && lane == 0) {
+    unsigned long long* t = a.trace + (int64_t)DP_TRACE_FIELDS * node;
+    t[2] += tr_wait;
+    t[3] += tr_eval;
+    t[4] += tr_walk;
+    t[5] += tr_chunks;
+    t[6] += tr_waits;
+    t[7] += tr_polls;
+  }
   if (lane == 0) {
     a.cursor[node] = cur;
     a.count[node] = count;
@@ -598,9 +849,9 @@ __global__ __launch_bounds__(64) void k_resolve_ticketed(round_args a) {
   const int32_t node = s_node;
   if (node >= a.N) return;
   if (a.trace && threadIdx.x == 0) {
-    a.trace[4 * (int64_t)node] = (unsigned long long)wall_clock64();
-    a.trace[4 * (int64_t)node + 2] = 0;
-    a.trace[4 * (int64_t)node + 3] = 0;
+    unsigned long long* t = a.trace + (int64_t)DP_TRACE_FIELDS * node;
+    t[0] = (unsigned long long)wall_clock64();
+    for (int k = 2; k < DP_TRACE_FIELDS; ++k) t[k] = 0;
   }
   __syncthreads();
   int32_t spins = 0;
@@ -614,7 +865,7 @@ __global__ __launch_bounds__(64) void k_resolve_ticketed(round_args a) {
     __builtin_amdgcn_s_sleep(2);
   }
   if (threadIdx.x == 0 && spins) atomicAdd((unsigned long long*)&a.counters[DPC_ROUNDS], (unsigned long long)spins);
-  if (a.trace && threadIdx.x == 0) a.trace[4 * (int64_t)node + 1] = (unsigned long long)wall_clock64();
+  if (a.trace && threadIdx.x == 0) a.trace[(int64_t)DP_TRACE_FIELDS * node + 1] = (unsigned long long)wall_clock64();
 }
 
 // Append the NOVEL final states of the batch to the visited set in canonical
@@ -742,10 +993,23 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
     dp_fill_i32(c->d_cont_cursor, nc, 0, s);
     k_cont_scatter<NX><<<pb, 256, 0, s>>>(c->grid, c->d_total, c->d_verdict, c->d_flags, c->d_final, c->pair_cap,
                                           c->d_cont_start, c->d_cont_cursor, c->d_cont_idx, c->d_cont_x);
-    k_conf<NX><<<pb, 256, 0, s>>>(c->d_sys, c->grid, c->d_total, c->d_pnode, c->d_app, c->d_verdict, c->d_flags,
-                                  c->d_final, c->pair_cap, c->d_cont_start, c->d_cont_idx, c->d_cont_x, r_dup,
-                                  window, c->d_conf_cnt, c->d_conf, c->d_conf_ov, c->d_conf_pool, c->conf_pool_cap,
-                                  c->d_counters);
+    bool cells = false;
+    if constexpr (NX <= 4) {
+      if (window == 0) {
+        // the scatter cursors are free again: they hold the list of non-empty cells
+        k_cell_list<<<dp_grid_blocks(nc, 256, 2048), 256, 0, s>>>(nc, c->d_cont_start, c->d_cont_cursor,
+                                                                  c->d_counters);
+        k_conf_cells<NX><<<(int)std::min<int64_t>(nc, 8192), 256, 0, s>>>(
+            c->d_sys, c->grid, c->d_flags, c->pair_cap, c->d_cont_start, c->d_cont_idx,
+            c->d_cont_x, c->d_cont_cursor, r_dup, c->d_conf_cnt, c->d_conf, c->d_conf_ov, c->d_conf_pool, c->conf_pool_cap,
+            c->d_counters);
+        cells = true;
+      }
+    }
+    if (!cells)
+      k_conf<NX><<<pb, 256, 0, s>>>(c->d_sys, c->grid, c->d_pnode, c->d_app, c->d_flags, c->pair_cap,
+                                    c->d_cont_start, c->d_cont_idx, c->d_cont_x, r_dup, window, c->d_conf_cnt,
+                                    c->d_conf, c->d_conf_ov, c->d_conf_pool, c->conf_pool_cap, c->d_counters);
     k_round_init<<<dp_grid_blocks(N, 256, 1024), 256, 0, s>>>(N, c->d_app, c->d_total, c->d_node_cursor,
                                                               c->d_node_count, c->d_node_novel, c->d_counters);
     round_args ra = make_round_args(c);

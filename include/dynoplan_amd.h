@@ -334,11 +334,11 @@ int dp_debug_gemm_bf16(dp_ctx* ctx, const float* A, const float* B, int32_t M, i
 int dp_debug_fp32_positions(dp_ctx* ctx, float* out, int64_t n);
 
 /* Diagnostics of the duplicate resolution (no counterpart in the reference):
- * enable > 0 records, for each node of later batches, the GPU clock when its
- * wave starts and ends, the number of waits on other nodes' verdicts and the
- * polls they took; enable < 0 stops.  With out != NULL, copies the record of
- * the last batch, out[4n + 1]: (start, end, waits, polls) per node, then the
- * clock rate in kHz. */
+ * enable > 0 records, for each node of later batches, its wave's timeline;
+ * enable < 0 stops.  With out != NULL, copies the record of the last batch,
+ * out[8n + 1]: per node (start clock, end clock, cycles waiting on other
+ * nodes' verdicts, cycles evaluating conflict lists, cycles walking chunks,
+ * chunks, waits, polls), then the clock rate in kHz. */
 int dp_debug_resolve_trace(dp_ctx* ctx, int32_t enable, uint64_t* out, int64_t n);
 
 /* ---- db-A* search on the expansion path (dbastar.cpp:523-1010) -------------
