@@ -216,31 +216,48 @@ __global__ void k_conf(const dp_dev_sys* __restrict__ Sp, dp_grid_geom G, const 
                  pool_cap, counters);
 }
 
-// The non-empty cells of the contender grid, in no particular order.
-__global__ void k_cell_list(int64_t ncells, const int32_t* __restrict__ cstart, int32_t* __restrict__ list,
-                            long long* __restrict__ counters) {
+// The non-empty cells of the contender grid.  Each workgroup takes a
+// contiguous range of cells, counts its non-empty ones, reserves room with
+// one global atomic, then lists them in cell order.
+__global__ __launch_bounds__(256) void k_cell_list(int64_t ncells, const int32_t* __restrict__ cstart,
+                                                   int32_t* __restrict__ list, long long* __restrict__ counters) {
+  __shared__ int32_t s_cnt;
+  __shared__ long long s_base;
   const int lane = threadIdx.x % 64;
-  for (int64_t c0 = (int64_t)blockIdx.x * blockDim.x; c0 < ncells; c0 += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t per = (ncells + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * per;
+  const int64_t hi = lo + per < ncells ? lo + per : ncells;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int64_t c = lo + threadIdx.x; c < hi; c += blockDim.x) mine += cstart[c + 1] > cstart[c] ? 1 : 0;
+  if (mine) atomicAdd(&s_cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s_base = s_cnt ? (long long)atomicAdd((unsigned long long*)&counters[DPC_CELLS], (unsigned long long)s_cnt) : 0;
+    s_cnt = 0;
+  }
+  __syncthreads();
+  for (int64_t c0 = lo; c0 < hi; c0 += blockDim.x) {
     const int64_t c = c0 + threadIdx.x;
-    const bool ne = c < ncells && cstart[c + 1] > cstart[c];
+    const bool ne = c < hi && cstart[c + 1] > cstart[c];
     const uint64_t m = __ballot(ne);
-    if (!m) continue;
-    long long base = 0;
-    if (lane == 0) base = (long long)atomicAdd((unsigned long long*)&counters[DPC_CELLS], (unsigned long long)__popcll(m));
-    base = __shfl(base, 0, 64);
-    if (ne) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)c;
+    int woff = 0;
+    if (lane == 0 && m) woff = atomicAdd(&s_cnt, __popcll(m));
+    woff = __shfl(woff, 0, 64);
+    if (ne) list[s_base + woff + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)c;
   }
 }
 
-// Planar systems: one workgroup per non-empty grid cell.  The 27 cells
-// around it form 9 runs of 3 cells consecutive in memory (x is the fastest
-// grid dimension).  Their contenders, at most kConfStage, are staged in LDS,
-// and each of the cell's own contenders is tested against all of them by the
-// lanes of one wave, with ballot compaction into its list.  A larger
-// neighbourhood is tested the same way from global memory.
-constexpr int kConfStage = 768;
+// Planar systems, one wave per non-empty grid cell: the 27 cells around it
+// form 9 runs of 3 cells consecutive in memory (x is the fastest grid
+// dimension).  For each of the cell's contenders in turn, the lanes read the
+// runs' contenders (coalesced, cached: the same few kilobytes for every
+// contender of the cell) and test them, kU per lane in flight, with ballot
+// compaction into the list.  No LDS and no barriers, so many cells are in
+// flight per CU.
 template <int NX>
-__global__ __launch_bounds__(256) void k_conf_cells(const dp_dev_sys* __restrict__ Sp, dp_grid_geom G,
+__global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict__ Sp, dp_grid_geom G,
                                                     const int8_t* __restrict__ flags, int64_t pcap,
                                                     const int32_t* __restrict__ cstart,
                                                     const int32_t* __restrict__ cidx, const double* __restrict__ cx,
@@ -248,131 +265,148 @@ __global__ __launch_bounds__(256) void k_conf_cells(const dp_dev_sys* __restrict
                                                     int16_t* __restrict__ conf_cnt, int32_t* __restrict__ conf,
                                                     int32_t* __restrict__ conf_ov, int32_t* __restrict__ pool,
                                                     int64_t pool_cap, long long* __restrict__ counters) {
-  __shared__ double s_x[NX * kConfStage];
-  __shared__ int32_t s_idx[kConfStage];
-  __shared__ int8_t s_fl[kConfStage];
-  __shared__ int32_t s_run[9], s_pre[10];
   const dp_dev_sys& S = *Sp;
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int lane = threadIdx.x % 64;
   const uint64_t ltmask = (1ull << lane) - 1ull;
   const long long ncl = counters[DPC_CELLS];
   const double T = dp_pos_bound(S, r);
   const int64_t d01 = (int64_t)G.dims[0] * G.dims[1];
-  for (long long w = blockIdx.x; w < ncl; w += gridDim.x) {
+  const long long nwaves = (long long)gridDim.x * (blockDim.x / 64);
+  for (long long w = (long long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; w < ncl; w += nwaves) {
     const int64_t cell = cells[w];
     const int32_t c0 = (int32_t)(cell % G.dims[0]), c1 = (int32_t)((cell / G.dims[0]) % G.dims[1]);
     const int32_t c2 = (int32_t)(cell / d01);
-    if (wave == 0) {
-      // run k: rows c1 + k%3 - 1, headings c2 + k/3 - 1 (the rules of dp_cell_neighbor)
-      int n = 0, st = 0;
-      if (lane < 9) {
-        const int32_t n1 = c1 + lane % 3 - 1;
-        const int32_t d2 = lane / 3 - 1;
-        int32_t n2 = c2 + d2;
-        bool ok = n1 >= 0 && n1 < G.dims[1];
-        if (G.dims[2] == 1) {
-          ok = ok && d2 == 0;
-          n2 = 0;
-        } else if (G.periodic2) {
-          n2 = n2 < 0 ? n2 + G.dims[2] : (n2 >= G.dims[2] ? n2 - G.dims[2] : n2);
-        } else {
-          ok = ok && n2 >= 0 && n2 < G.dims[2];
-        }
-        if (ok) {
-          const int64_t row = ((int64_t)n2 * G.dims[1] + n1) * G.dims[0];
-          const int32_t x0 = c0 > 0 ? c0 - 1 : 0, x1 = c0 + 1 < G.dims[0] ? c0 + 1 : G.dims[0] - 1;
-          st = cstart[row + x0];
-          n = cstart[row + x1 + 1] - st;
-        }
+    // run k (lanes 0..8): rows c1 + k%3 - 1, headings c2 + k/3 - 1 (the rules of dp_cell_neighbor)
+    int n = 0, st = 0;
+    if (lane < 9) {
+      const int32_t n1 = c1 + lane % 3 - 1;
+      const int32_t d2 = lane / 3 - 1;
+      int32_t n2 = c2 + d2;
+      bool ok = n1 >= 0 && n1 < G.dims[1];
+      if (G.dims[2] == 1) {
+        ok = ok && d2 == 0;
+        n2 = 0;
+      } else if (G.periodic2) {
+        n2 = n2 < 0 ? n2 + G.dims[2] : (n2 >= G.dims[2] ? n2 - G.dims[2] : n2);
+      } else {
+        ok = ok && n2 >= 0 && n2 < G.dims[2];
       }
-      int incl = n;
-#pragma unroll
-      for (int d = 1; d < 16; d <<= 1) {
-        const int v = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += v;
+      if (ok) {
+        const int64_t row = ((int64_t)n2 * G.dims[1] + n1) * G.dims[0];
+        const int32_t x0 = c0 > 0 ? c0 - 1 : 0, x1 = c0 + 1 < G.dims[0] ? c0 + 1 : G.dims[0] - 1;
+        st = cstart[row + x0];
+        n = cstart[row + x1 + 1] - st;
       }
-      if (lane < 9) {
-        s_run[lane] = st;
-        s_pre[lane] = incl - n;
-      }
-      if (lane == 8) s_pre[9] = incl;
     }
-    __syncthreads();
-    const int K = s_pre[9];
+    int incl = n;
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1) {
+      const int v = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += v;
+    }
+    // run starts and exclusive prefix, wave-uniform
+    int32_t rs[9], rp[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      rs[k] = __builtin_amdgcn_readlane(st, k);
+      rp[k] = __builtin_amdgcn_readlane(incl - n, k);
+    }
+    const int K = __builtin_amdgcn_readlane(incl, 8);
     const int32_t b = cstart[cell], e = cstart[cell + 1];
-    // staged position of the cell's first contender: inside run 4 (same row and heading)
-    const int jb = s_pre[4] + (b - s_run[4]);
-    auto locate = [&](int j) -> int64_t {
-      int k = 0;
+    // the cell's contenders, 64 at a time, one per lane
+    for (int32_t g0 = b; g0 < e; g0 += 64) {
+      const int m = e - g0 < 64 ? e - g0 : 64;
+      int32_t q_l = INT32_MAX;
+      bool list_l = false;
+      double x_l[NX];
 #pragma unroll
-      for (int st = 8; st > 0; st >>= 1)  // s_pre[9], the total, exceeds j
-        if (k + st <= 9 && s_pre[k + st] <= j) k += st;
-      return (int64_t)s_run[k] + (j - s_pre[k]);
-    };
-    const bool staged = K <= kConfStage;
-    if (staged) {
-      for (int j = threadIdx.x; j < K; j += blockDim.x) {
-        const int64_t t = locate(j);
-        const int32_t o = cidx[t];
-        s_idx[j] = o;
+      for (int k = 0; k < NX; ++k) x_l[k] = 0.0;
+      if (lane < m) {
+        q_l = cidx[g0 + lane];
+        list_l = !(flags[q_l] & 2);  // goal-forced: NOVEL without a list
 #pragma unroll
-        for (int d = 0; d < NX; ++d) s_x[d * kConfStage + j] = cx[(int64_t)d * pcap + t];
-        if (t >= b && t < e) s_fl[j] = flags[o];
+        for (int k = 0; k < NX; ++k) x_l[k] = cx[(int64_t)k * pcap + g0 + lane];
       }
-      __syncthreads();
-    }
-    for (int64_t i = b + wave; i < e; i += blockDim.x / 64) {
-      const int js = jb + (int)(i - b);
-      const int32_t q = staged ? s_idx[js] : cidx[i];
-      if ((staged ? s_fl[js] : flags[q]) & 2) continue;  // goal-forced: NOVEL without a list
-      double x[NX];
+      const uint64_t lists = __ballot(list_l);
+      int cnt_l = 0;           // list length of this lane's contender so far
+      long long off_l = 0;     // its overflow pool offset (pool pass)
+      // Candidates in order, kU per lane in flight, each batch tested
+      // against every contender of the group.  Entries at list position
+      // < DP_CONF_SLOTS go inline; the pool pass writes the others.
+      constexpr int kU = 4;
+      auto pass = [&](bool pool_pass, uint64_t who) {
+        for (int j0 = 0; j0 < K; j0 += 64 * kU) {
+          int32_t o[kU];
+          double y[kU][NX];
 #pragma unroll
-      for (int k = 0; k < NX; ++k) x[k] = staged ? s_x[k * kConfStage + js] : cx[(int64_t)k * pcap + i];
-      // candidate j: staged, or read through its grid position
-      auto test = [&](int j, int32_t& o) -> bool {
-        if (j >= K) return false;
-        if (staged) {
-          o = s_idx[j];
-          return o < q && dp_within_col_pf<NX>(S, x, s_x, kConfStage, j, r, T);
-        }
-        const int64_t t = locate(j);
-        o = cidx[t];
-        return o < q && dp_within_col_pf<NX>(S, x, cx, pcap, t, r, T);
-      };
-      int cnt = 0;
-      for (int j0 = 0; j0 < K; j0 += 64) {
-        int32_t o = 0;
-        const bool hit = test(j0 + lane, o);
-        const uint64_t m = __ballot(hit);
-        const int pos = cnt + __popcll(m & ltmask);
-        if (hit && pos < DP_CONF_SLOTS) conf[(int64_t)pos * pcap + q] = o;
-        cnt += __popcll(m);
-      }
-      if (cnt > DP_CONF_SLOTS && cnt < DP_CONF_MAX) {
-        // the rest goes to the overflow pool, listed by a second pass in the same order
-        long long off = 0;
-        if (lane == 0)
-          off = (long long)atomicAdd((unsigned long long*)&counters[DPC_CONF],
-                                     (unsigned long long)(cnt - DP_CONF_SLOTS));
-        off = __shfl(off, 0, 64);
-        if (off + cnt - DP_CONF_SLOTS <= pool_cap) {
-          int k = 0;
-          for (int j0 = 0; j0 < K; j0 += 64) {
-            int32_t o = 0;
-            const bool hit = test(j0 + lane, o);
-            const uint64_t m = __ballot(hit);
-            const int pos = k + __popcll(m & ltmask);
-            if (hit && pos >= DP_CONF_SLOTS) pool[off + (pos - DP_CONF_SLOTS)] = o;
-            k += __popcll(m);
+          for (int u = 0; u < kU; ++u) {
+            const int j = j0 + 64 * u + lane;
+            o[u] = INT32_MAX;
+            if (j < K) {
+              int k = 0;
+#pragma unroll
+              for (int kk = 1; kk < 9; ++kk) k += j >= rp[kk] ? 1 : 0;
+              int32_t sk = rs[0], pk = rp[0];
+#pragma unroll
+              for (int kk = 1; kk < 9; ++kk)
+                if (k == kk) sk = rs[kk], pk = rp[kk];
+              const int64_t t = (int64_t)sk + (j - pk);
+              o[u] = cidx[t];
+#pragma unroll
+              for (int d = 0; d < NX; ++d) y[u][d] = cx[(int64_t)d * pcap + t];
+            }
           }
-          if (lane == 0) conf_ov[q] = (int32_t)off;
-        } else {
-          cnt = DP_CONF_MAX;  // pool exhausted: scan the grid when reached
+          for (uint64_t rem = who; rem; rem &= rem - 1ull) {
+            const int c = __ffsll((long long)rem) - 1;
+            const int32_t q = __builtin_amdgcn_readlane(q_l, c);
+            double x[NX];
+#pragma unroll
+            for (int k = 0; k < NX; ++k) {
+              const uint64_t bits = (uint64_t)__double_as_longlong(x_l[k]);
+              const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)bits, c);
+              const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(bits >> 32), c);
+              x[k] = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+            }
+            int cc = __builtin_amdgcn_readlane(cnt_l, c);
+            const long long off = pool_pass ? (long long)__builtin_amdgcn_readlane((int)off_l, c) : 0;
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+              bool hit = false;
+              if (o[u] < q) {
+                const double dx = x[0] - y[u][0], dy = x[1] - y[u][1];
+                double s2 = dx * dx;
+                s2 = s2 + dy * dy;
+                hit = !(s2 > T) && !(S.w[1] * dp_so2(x[2], y[u][2]) > r) && !(dp_pos_term(S, x, y[u]) > r) &&
+                      dp_within(S, x, y[u], r);
+              }
+              const uint64_t mk = __ballot(hit);
+              const int pos = cc + __popcll(mk & ltmask);
+              if (hit && !pool_pass && pos < DP_CONF_SLOTS) conf[(int64_t)pos * pcap + q] = o[u];
+              if (hit && pool_pass && pos >= DP_CONF_SLOTS) pool[off + (pos - DP_CONF_SLOTS)] = o[u];
+              cc += __popcll(mk);
+            }
+            if (lane == c) cnt_l = cc;
+          }
         }
+      };
+      pass(false, lists);
+      // lists longer than the inline slots continue in the overflow pool
+      const bool over = list_l && cnt_l > DP_CONF_SLOTS && cnt_l < DP_CONF_MAX;
+      if (over) {
+        off_l = (long long)atomicAdd((unsigned long long*)&counters[DPC_CONF],
+                                     (unsigned long long)(cnt_l - DP_CONF_SLOTS));
+        if (off_l + cnt_l - DP_CONF_SLOTS > pool_cap) cnt_l = DP_CONF_MAX;  // pool exhausted: scan when reached
       }
-      if (lane == 0) conf_cnt[q] = (int16_t)(cnt < DP_CONF_MAX ? cnt : -1);
+      const uint64_t pooled = __ballot(over && cnt_l < DP_CONF_MAX);
+      if (pooled) {
+        const int keep = cnt_l;
+        cnt_l = 0;
+        pass(true, pooled);
+        cnt_l = keep;
+        if (over && cnt_l < DP_CONF_MAX) conf_ov[q_l] = (int32_t)off_l;
+      }
+      if (list_l) conf_cnt[q_l] = (int16_t)(cnt_l < DP_CONF_MAX ? cnt_l : -1);
     }
-    __syncthreads();  // the staging arrays are rewritten for the next cell
   }
 }
 
@@ -994,12 +1028,12 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
     k_cont_scatter<NX><<<pb, 256, 0, s>>>(c->grid, c->d_total, c->d_verdict, c->d_flags, c->d_final, c->pair_cap,
                                           c->d_cont_start, c->d_cont_cursor, c->d_cont_idx, c->d_cont_x);
     bool cells = false;
-    if constexpr (NX <= 4) {
+    if constexpr (NX != 13) {
       if (window == 0) {
         // the scatter cursors are free again: they hold the list of non-empty cells
-        k_cell_list<<<dp_grid_blocks(nc, 256, 2048), 256, 0, s>>>(nc, c->d_cont_start, c->d_cont_cursor,
+        k_cell_list<<<dp_grid_blocks(nc, 4096, 1024), 256, 0, s>>>(nc, c->d_cont_start, c->d_cont_cursor,
                                                                   c->d_counters);
-        k_conf_cells<NX><<<(int)std::min<int64_t>(nc, 8192), 256, 0, s>>>(
+        k_conf_waves<NX><<<(int)std::min<int64_t>((nc + 3) / 4, 8192), 256, 0, s>>>(
             c->d_sys, c->grid, c->d_flags, c->pair_cap, c->d_cont_start, c->d_cont_idx,
             c->d_cont_x, c->d_cont_cursor, r_dup, c->d_conf_cnt, c->d_conf, c->d_conf_ov, c->d_conf_pool, c->conf_pool_cap,
             c->d_counters);
