@@ -154,10 +154,10 @@ __device__ __forceinline__ bool dp_within_col(const dp_dev_sys& S, const double*
   return dp_within(S, x, y, r);
 }
 
-// Planar systems: a bound T on the squared position difference such that
-// dx*dx + dy*dy > T (as dp_l2_2 forms it) implies w0 * sqrt(...) > r after
-// rounding: the relative margin 1e-9 exceeds the few ulps that sqrt and the
-// product can lose.
+// A bound T on the squared position difference such that s > T, with s the
+// sum of squares as dp_l2_2 (planar) or dp_l2_3 (quadrotor) forms it,
+// implies w0 * sqrt(s) > r after rounding: the relative margin 1e-9 exceeds
+// the few ulps that sqrt and the product can lose.
 DP_HD double dp_pos_bound(const dp_dev_sys& S, double r) {
   if (!(S.w[0] > 0.0)) return INFINITY;
   const double t = r / S.w[0];

@@ -128,11 +128,13 @@ __global__ void k_cont_scatter(dp_grid_geom G, const int32_t* __restrict__ total
 }
 
 // Earlier contenders of pair q within r: scans the 27 cells around x.
-// Calls f(o) for every contender o < q within r; stops when f returns false.
-template <int NX, typename F>
+// Calls f(o) for every contender o < q within r that keep(o) accepts (keep
+// runs before the metric and may read memory); stops when f returns false.
+// Candidates are taken eight at a time so that their loads overlap.
+template <int NX, typename F, typename Keep>
 __device__ __forceinline__ void scan_earlier(const dp_dev_sys& S, const dp_grid_geom& G, const int32_t* cstart,
                                              const int32_t* cidx, const double* cx, int64_t pcap, const double* x,
-                                             int64_t q, double r, F&& f) {
+                                             int64_t q, double r, F&& f, Keep&& keep) {
   int32_t c[3];
   dp_cell_coords(G, x, c);
   const double T = dp_pos_bound(S, r);
@@ -140,81 +142,29 @@ __device__ __forceinline__ void scan_earlier(const dp_dev_sys& S, const dp_grid_
     const int64_t cell = dp_cell_neighbor(G, c, cq);
     if (cell < 0) continue;
     const int32_t e = cstart[cell + 1];
-    for (int32_t t = cstart[cell]; t < e; ++t) {
-      const int32_t o = cidx[t];
-      if (o >= q) continue;
-      if (!dp_within_col_pf<NX>(S, x, cx, pcap, t, r, T)) continue;
-      if (!f(o)) return;
-    }
-  }
-}
-
-// Earlier-conflict lists, stored inline (DP_CONF_SLOTS per pair, slot-major),
-// for the pairs a node's cursor is likely to reach: FREE pairs that are
-// neither goal-forced nor duplicates of the visited set, at position < window
-// in their node.  conf_cnt = -1 marks a pair whose list is not stored
-// (outside the window or overflowing); the resolution scans the grid for it.
-// Lists longer than DP_CONF_SLOTS continue in the overflow pool; the order of
-// a list's entries carries no meaning.
-
-// The list of contender t (position t of the cell-ordered contender array)
-// by one thread scanning the 27 cells around it.
-template <int NX>
-__device__ void conf_one(const dp_dev_sys& S, const dp_grid_geom& G, const int32_t* __restrict__ pnode,
-                         const int32_t* __restrict__ app, const int8_t* __restrict__ flags, int64_t pcap,
-                         const int32_t* __restrict__ cstart, const int32_t* __restrict__ cidx,
-                         const double* __restrict__ cx, double r, int32_t window, int64_t t,
-                         int16_t* __restrict__ conf_cnt, int32_t* __restrict__ conf, int32_t* __restrict__ conf_ov,
-                         int32_t* __restrict__ pool, int64_t pool_cap, long long* __restrict__ counters) {
-  const int32_t q = cidx[t];
-  if (flags[q] & 2) return;  // goal-forced: NOVEL without a list
-  if (window > 0 && q - app[pnode[q]] >= window) {
-    conf_cnt[q] = -1;
-    return;
-  }
-  double x[NX];
+    for (int32_t t0 = cstart[cell]; t0 < e; t0 += 8) {
+      int32_t o[8];
 #pragma unroll
-  for (int k = 0; k < NX; ++k) x[k] = cx[(int64_t)k * pcap + t];
-  int cnt = 0;
-  scan_earlier<NX>(S, G, cstart, cidx, cx, pcap, x, q, r, [&](int32_t o) {
-    if (cnt < DP_CONF_SLOTS) conf[(int64_t)cnt * pcap + q] = o;
-    ++cnt;
-    return cnt < DP_CONF_MAX;
-  });
-  if (cnt > DP_CONF_SLOTS && cnt < DP_CONF_MAX) {
-    // the rest of the list goes to the overflow pool; the scan order is
-    // the same, so a second scan lists the same entries
-    const long long off =
-        (long long)atomicAdd((unsigned long long*)&counters[DPC_CONF], (unsigned long long)(cnt - DP_CONF_SLOTS));
-    if (off + cnt - DP_CONF_SLOTS <= pool_cap) {
-      int k = 0;
-      scan_earlier<NX>(S, G, cstart, cidx, cx, pcap, x, q, r, [&](int32_t o) {
-        if (k >= DP_CONF_SLOTS) pool[off + (k - DP_CONF_SLOTS)] = o;
-        ++k;
-        return k < cnt;
-      });
-      conf_ov[q] = (int32_t)off;
-    } else {
-      cnt = DP_CONF_MAX;  // pool exhausted: scan the grid when reached
+      for (int k = 0; k < 8; ++k) o[k] = t0 + k < e ? cidx[t0 + k] : INT32_MAX;
+      uint32_t km = 0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) km |= (o[k] < q && keep(o[k]) ? 1u : 0u) << k;
+      for (; km; km &= km - 1u) {
+        const int32_t t = t0 + __ffs(km) - 1;
+        if (!dp_within_col_pf<NX>(S, x, cx, pcap, t, r, T)) continue;
+        if (!f(cidx[t])) return;
+      }
     }
   }
-  conf_cnt[q] = (int16_t)(cnt < DP_CONF_MAX ? cnt : -1);
 }
 
-// One thread per contender, in cell order: neighbouring threads scan
-// overlapping cell blocks, so the grid reads are shared through the caches.
-template <int NX>
-__global__ void k_conf(const dp_dev_sys* __restrict__ Sp, dp_grid_geom G, const int32_t* __restrict__ pnode,
-                       const int32_t* __restrict__ app, const int8_t* __restrict__ flags, int64_t pcap,
-                       const int32_t* __restrict__ cstart, const int32_t* __restrict__ cidx,
-                       const double* __restrict__ cx, double r, int32_t window, int16_t* __restrict__ conf_cnt,
-                       int32_t* __restrict__ conf, int32_t* __restrict__ conf_ov, int32_t* __restrict__ pool,
-                       int64_t pool_cap, long long* __restrict__ counters) {
-  const int64_t n = cstart[G.ncells];
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
-    conf_one<NX>(*Sp, G, pnode, app, flags, pcap, cstart, cidx, cx, r, window, t, conf_cnt, conf, conf_ov, pool,
-                 pool_cap, counters);
-}
+// Earlier-conflict lists, stored inline (DP_CONF_SLOTS per pair, slot-major)
+// for contenders: FREE pairs that are neither goal-forced nor duplicates of
+// the visited set (with a window, only those at position < window in their
+// node).  Lists longer than DP_CONF_SLOTS continue in the overflow pool;
+// conf_cnt = -1 marks a pair without a stored list (past the window, or the
+// pool ran out), for which the resolution scans the grid.  The order of a
+// list's entries carries no meaning.
 
 // The non-empty cells of the contender grid.  Each workgroup takes a
 // contiguous range of cells, counts its non-empty ones, reserves room with
@@ -249,7 +199,7 @@ __global__ __launch_bounds__(256) void k_cell_list(int64_t ncells, const int32_t
   }
 }
 
-// Planar systems, one wave per non-empty grid cell: the 27 cells around it
+// One wave per non-empty grid cell: the 27 cells around it
 // form 9 runs of 3 cells consecutive in memory (x is the fastest grid
 // dimension).  For each of the cell's contenders in turn, the lanes read the
 // runs' contenders (coalesced, cached: the same few kilobytes for every
@@ -258,7 +208,9 @@ __global__ __launch_bounds__(256) void k_cell_list(int64_t ncells, const int32_t
 // flight per CU.
 template <int NX>
 __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict__ Sp, dp_grid_geom G,
-                                                    const int8_t* __restrict__ flags, int64_t pcap,
+                                                    const int8_t* __restrict__ flags,
+                                                    const int32_t* __restrict__ pnode,
+                                                    const int32_t* __restrict__ app, int32_t window, int64_t pcap,
                                                     const int32_t* __restrict__ cstart,
                                                     const int32_t* __restrict__ cidx, const double* __restrict__ cx,
                                                     const int32_t* __restrict__ cells, double r,
@@ -276,7 +228,8 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
     const int64_t cell = cells[w];
     const int32_t c0 = (int32_t)(cell % G.dims[0]), c1 = (int32_t)((cell / G.dims[0]) % G.dims[1]);
     const int32_t c2 = (int32_t)(cell / d01);
-    // run k (lanes 0..8): rows c1 + k%3 - 1, headings c2 + k/3 - 1 (the rules of dp_cell_neighbor)
+    // run k (lanes 0..8): rows c1 + k%3 - 1, layers c2 + k/3 - 1 (headings, or z for the
+    // quadrotor; the rules of dp_cell_neighbor)
     int n = 0, st = 0;
     if (lane < 9) {
       const int32_t n1 = c1 + lane % 3 - 1;
@@ -318,14 +271,13 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
       const int m = e - g0 < 64 ? e - g0 : 64;
       int32_t q_l = INT32_MAX;
       bool list_l = false;
-      double x_l[NX];
-#pragma unroll
-      for (int k = 0; k < NX; ++k) x_l[k] = 0.0;
       if (lane < m) {
         q_l = cidx[g0 + lane];
         list_l = !(flags[q_l] & 2);  // goal-forced: NOVEL without a list
-#pragma unroll
-        for (int k = 0; k < NX; ++k) x_l[k] = cx[(int64_t)k * pcap + g0 + lane];
+        if (list_l && window > 0 && q_l - app[pnode[q_l]] >= window) {
+          conf_cnt[q_l] = -1;  // past the window: scanned when reached
+          list_l = false;
+        }
       }
       const uint64_t lists = __ballot(list_l);
       int cnt_l = 0;           // list length of this lane's contender so far
@@ -333,15 +285,19 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
       // Candidates in order, kU per lane in flight, each batch tested
       // against every contender of the group.  Entries at list position
       // < DP_CONF_SLOTS go inline; the pool pass writes the others.
-      constexpr int kU = 4;
+      constexpr int kU = NX == 13 ? 2 : 4;
       auto pass = [&](bool pool_pass, uint64_t who) {
         for (int j0 = 0; j0 < K; j0 += 64 * kU) {
+          // candidate index, position and (planar) heading; the other
+          // components are read only for candidates that pass the rejections
           int32_t o[kU];
-          double y[kU][NX];
+          int64_t tt[kU];
+          double y[kU][3];
 #pragma unroll
           for (int u = 0; u < kU; ++u) {
             const int j = j0 + 64 * u + lane;
             o[u] = INT32_MAX;
+            tt[u] = 0;
             if (j < K) {
               int k = 0;
 #pragma unroll
@@ -350,34 +306,44 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
 #pragma unroll
               for (int kk = 1; kk < 9; ++kk)
                 if (k == kk) sk = rs[kk], pk = rp[kk];
-              const int64_t t = (int64_t)sk + (j - pk);
-              o[u] = cidx[t];
+              tt[u] = (int64_t)sk + (j - pk);
+              o[u] = cidx[tt[u]];
 #pragma unroll
-              for (int d = 0; d < NX; ++d) y[u][d] = cx[(int64_t)d * pcap + t];
+              for (int d = 0; d < 3; ++d) y[u][d] = cx[(int64_t)d * pcap + tt[u]];
             }
           }
           for (uint64_t rem = who; rem; rem &= rem - 1ull) {
             const int c = __ffsll((long long)rem) - 1;
             const int32_t q = __builtin_amdgcn_readlane(q_l, c);
+            // the contender's state: wave-uniform loads
             double x[NX];
 #pragma unroll
-            for (int k = 0; k < NX; ++k) {
-              const uint64_t bits = (uint64_t)__double_as_longlong(x_l[k]);
-              const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)bits, c);
-              const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(bits >> 32), c);
-              x[k] = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-            }
+            for (int k = 0; k < NX; ++k) x[k] = cx[(int64_t)k * pcap + g0 + c];
             int cc = __builtin_amdgcn_readlane(cnt_l, c);
             const long long off = pool_pass ? (long long)__builtin_amdgcn_readlane((int)off_l, c) : 0;
 #pragma unroll
             for (int u = 0; u < kU; ++u) {
               bool hit = false;
               if (o[u] < q) {
-                const double dx = x[0] - y[u][0], dy = x[1] - y[u][1];
+                // exact rejections of dp_within_col_pf, then the metric
+                const double dx = x[0] - y[u][0], dy = x[1] - y[u][1], dz = x[2] - y[u][2];
                 double s2 = dx * dx;
                 s2 = s2 + dy * dy;
-                hit = !(s2 > T) && !(S.w[1] * dp_so2(x[2], y[u][2]) > r) && !(dp_pos_term(S, x, y[u]) > r) &&
-                      dp_within(S, x, y[u], r);
+                bool pass;
+                if constexpr (NX == 13) {
+                  s2 = s2 + dz * dz;
+                  pass = !(s2 > T);
+                } else {
+                  pass = !(s2 > T) && !(S.w[1] * dp_so2(x[2], y[u][2]) > r);
+                }
+                if (pass) {
+                  double yf[NX];
+#pragma unroll
+                  for (int d = 0; d < 3; ++d) yf[d] = y[u][d];
+#pragma unroll
+                  for (int d = 3; d < NX; ++d) yf[d] = cx[(int64_t)d * pcap + tt[u]];
+                  hit = !(dp_pos_term(S, x, yf) > r) && dp_within(S, x, yf, r);
+                }
               }
               const uint64_t mk = __ballot(hit);
               const int pos = cc + __popcll(mk & ltmask);
@@ -557,11 +523,17 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
         }
         return !ext_dup;
       };
-      // no stored list: scan the contender grid
+      // No stored list: scan the contender grid.  An earlier-chunk
+      // contender already decided as not NOVEL cannot matter, so its verdict
+      // is read first and the metric skipped for it.
       double x[NX];
 #pragma unroll
       for (int k = 0; k < NX; ++k) x[k] = a.fin[(int64_t)k * a.pcap + p];
-      scan_earlier<NX>(*a.sys, a.grid, a.cstart, a.cidx, a.cx, a.pcap, x, p, a.r_dup, visit);
+      scan_earlier<NX>(*a.sys, a.grid, a.cstart, a.cidx, a.cx, a.pcap, x, p, a.r_dup, visit, [&](int32_t o) {
+        if (o >= cb) return true;
+        const int v = a.res[o];
+        return v == DP_R_NOVEL || v == DP_R_UNDECIDED;
+      });
     };
     const unsigned long long t_eval = a.trace ? (unsigned long long)wall_clock64() : 0ull;
     const int cnt = fr && !forced && !dupv ? cc_p : 0;
@@ -1012,10 +984,10 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
   if (rc) return rc;
   c->round_limit = p->limit_branching_factor;
   c->round_r_dup = r_dup;
-  // Conflict lists are stored for every pair of planar batches, and for the
-  // first 64 pairs of each node for the quadrotor, whose cap is usually
-  // reached earlier (median position 22) and whose FP64 metric makes the
-  // lists dear; later quadrotor pairs scan the grid when reached.
+  // Lists are stored for every contender of planar batches.  Quadrotor nodes
+  // reach their cap early (the median cap position is 22 of ~550 pairs) and
+  // its neighbourhoods are large, so only the first 64 pairs of each node get
+  // a list; later pairs scan the grid if the walk reaches them.
   const int32_t window = p->limit_branching_factor > 0 && c->desc.system == DP_SYS_QUADROTOR ? 64 : 0;
   const int pb = dp_grid_blocks(c->pair_cap, 256, 8192);
   rc = dispatch_nx(c->desc.nx, [&](auto NXc) -> int {
@@ -1027,23 +999,12 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
     dp_fill_i32(c->d_cont_cursor, nc, 0, s);
     k_cont_scatter<NX><<<pb, 256, 0, s>>>(c->grid, c->d_total, c->d_verdict, c->d_flags, c->d_final, c->pair_cap,
                                           c->d_cont_start, c->d_cont_cursor, c->d_cont_idx, c->d_cont_x);
-    bool cells = false;
-    if constexpr (NX != 13) {
-      if (window == 0) {
-        // the scatter cursors are free again: they hold the list of non-empty cells
-        k_cell_list<<<dp_grid_blocks(nc, 4096, 1024), 256, 0, s>>>(nc, c->d_cont_start, c->d_cont_cursor,
-                                                                  c->d_counters);
-        k_conf_waves<NX><<<(int)std::min<int64_t>((nc + 3) / 4, 8192), 256, 0, s>>>(
-            c->d_sys, c->grid, c->d_flags, c->pair_cap, c->d_cont_start, c->d_cont_idx,
-            c->d_cont_x, c->d_cont_cursor, r_dup, c->d_conf_cnt, c->d_conf, c->d_conf_ov, c->d_conf_pool, c->conf_pool_cap,
-            c->d_counters);
-        cells = true;
-      }
-    }
-    if (!cells)
-      k_conf<NX><<<pb, 256, 0, s>>>(c->d_sys, c->grid, c->d_pnode, c->d_app, c->d_flags, c->pair_cap,
-                                    c->d_cont_start, c->d_cont_idx, c->d_cont_x, r_dup, window, c->d_conf_cnt,
-                                    c->d_conf, c->d_conf_ov, c->d_conf_pool, c->conf_pool_cap, c->d_counters);
+    // earlier-conflict lists; the scatter cursors are free again and hold
+    // the list of non-empty cells
+    k_cell_list<<<dp_grid_blocks(nc, 4096, 1024), 256, 0, s>>>(nc, c->d_cont_start, c->d_cont_cursor, c->d_counters);
+    k_conf_waves<NX><<<(int)std::min<int64_t>((nc + 3) / 4, 8192), 256, 0, s>>>(
+        c->d_sys, c->grid, c->d_flags, c->d_pnode, c->d_app, window, c->pair_cap, c->d_cont_start, c->d_cont_idx, c->d_cont_x, c->d_cont_cursor,
+        r_dup, c->d_conf_cnt, c->d_conf, c->d_conf_ov, c->d_conf_pool, c->conf_pool_cap, c->d_counters);
     k_round_init<<<dp_grid_blocks(N, 256, 1024), 256, 0, s>>>(N, c->d_app, c->d_total, c->d_node_cursor,
                                                               c->d_node_count, c->d_node_novel, c->d_counters);
     round_args ra = make_round_args(c);
