@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
 # HBM bytes per launch of the check kernel, measured with rocprofv3 --pmc
 # FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy (profiles/ holds the run).
 MEASURED_TRAFFIC = {
-    "unicycle1_scale": (1.998e9, "profiles/k_check_planar_unicycle1_scale_hbm_traffic.txt"),
+    "unicycle1_scale": (1.732e9, "profiles/k_check_planar_unicycle1_scale_hbm_traffic.txt"),
 }
 
 
