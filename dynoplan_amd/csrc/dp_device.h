@@ -164,6 +164,39 @@ DP_HD double dp_pos_bound(const dp_dev_sys& S, double r) {
   return t * t * (1.0 + 1e-9);
 }
 
+// dp_within(S, x, y, r) for planar systems, deciding most pairs without the
+// FP64 square root.  Rejections first (exact, see dp_within_col_pf); then the
+// distance with the position term from an FP32 square root: its relative
+// error (conversion plus sqrtf, under 3e-7) keeps the estimate within
+// 3.1e-7 * r of the FP64 distance whenever s <= T, so an estimate outside
+// r * (1 -+ 1e-6) decides the comparison; inside that band the FP64
+// expression decides.  The answer is always dp_within's.
+template <int NX>
+__device__ __forceinline__ bool dp_within_planar_fast(const dp_dev_sys& S, const double* x, const double* y, double r,
+                                                      double T) {
+  static_assert(NX != 13, "planar systems only");
+  const double dx = x[0] - y[0], dy = x[1] - y[1];
+  double s = dx * dx;
+  s = s + dy * dy;
+  if (s > T) return false;
+  const double th = S.w[1] * dp_so2(x[2], y[2]);
+  if (th > r) return false;
+  double rest = th;
+  if constexpr (NX >= 5) {
+    if (S.system == DP_SYS_UNICYCLE2) {
+      rest = rest + S.w[2] * fabs(x[3] - y[3]);
+      rest = rest + S.w[3] * fabs(x[4] - y[4]);
+    }
+  }
+  if constexpr (NX >= 4) {
+    if (S.system == DP_SYS_CAR_TRAILER) rest = rest + S.w[2] * dp_so2(x[3], y[3]);
+  }
+  const double est = S.w[0] * (double)sqrtf((float)s) + rest;
+  if (est < r * (1.0 - 1e-6)) return true;
+  if (est > r * (1.0 + 1e-6)) return false;
+  return dp_within(S, x, y, r);
+}
+
 // dp_within_col with two exact rejections ahead of it for planar systems:
 // the squared position difference above T (dp_pos_bound), or the heading
 // term w1 * so2 alone above r (dp_distance adds non-negative terms to it, and
@@ -178,8 +211,13 @@ __device__ __forceinline__ bool dp_within_col_pf(const dp_dev_sys& S, const doub
     s = s + dy * dy;
     if (s > T) return false;
     if (S.w[1] * dp_so2(x[2], col[2 * stride + i]) > r) return false;
+    double y[NX];
+#pragma unroll
+    for (int k = 0; k < NX; ++k) y[k] = col[(int64_t)k * stride + i];
+    return dp_within_planar_fast<NX>(S, x, y, r, T);
+  } else {
+    return dp_within_col<NX>(S, x, col, stride, i, r);
   }
-  return dp_within_col<NX>(S, x, col, stride, i, r);
 }
 
 DP_HD double dp_max(double a, double b) { return b > a ? b : a; }

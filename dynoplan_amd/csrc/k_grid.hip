@@ -342,7 +342,8 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
                   for (int d = 0; d < 3; ++d) yf[d] = y[u][d];
 #pragma unroll
                   for (int d = 3; d < NX; ++d) yf[d] = cx[(int64_t)d * pcap + tt[u]];
-                  hit = !(dp_pos_term(S, x, yf) > r) && dp_within(S, x, yf, r);
+                  if constexpr (NX == 13) hit = !(dp_pos_term(S, x, yf) > r) && dp_within(S, x, yf, r);
+                  else hit = dp_within_planar_fast<NX>(S, x, yf, r, T);
                 }
               }
               const uint64_t mk = __ballot(hit);
@@ -477,10 +478,28 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
     }
     const bool fr = p < end && vd_p == DP_V_FREE;
     if (a.limit > 0 && count >= a.limit) {
-      // the reference has left this node's loop: every later FREE pair is capped
+      // The reference has left this node's loop: every later FREE pair is
+      // capped.  Other nodes may be waiting for any of them, so the whole
+      // tail is published in one pass with its verdict loads in flight
+      // together, not chunk by chunk.
       if (fr) __hip_atomic_store(&a.res[p], (int)DP_R_CAPPED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       n_cap += __popcll(__ballot(fr));
-      continue;
+      for (int32_t t0 = cb + 64; t0 < end; t0 += 64 * 8) {
+        bool f8[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int32_t t = t0 + 64 * k + lane;
+          f8[k] = t < end && a.verdict[t] == DP_V_FREE;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          if (f8[k])
+            __hip_atomic_store(&a.res[t0 + 64 * k + lane], (int)DP_R_CAPPED, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+          n_cap += __popcll(__ballot(f8[k]));
+        }
+      }
+      break;
     }
     const int8_t fl = fr ? fl_p : 0;
     const bool forced = (fl & 2) != 0;
