@@ -227,6 +227,7 @@ void dp_destroy(dp_ctx* c) {
   dev_free(&c->d_start);
   dev_free(&c->d_geo);
   dev_free(&c->d_sweep);
+  dev_free(&c->d_srange);
   dev_free(&c->d_act);
   dev_free(&c->d_act32);
   dev_free(&c->d_goalidx);
@@ -353,16 +354,38 @@ int dp_upload_primitives(dp_ctx* c, const dp_prims* pr, int32_t max_motions) {
       c->reach_box[3] = std::max(c->reach_box[3], sweep[(size_t)3 * P + j]);
     }
   }
+  // Per-primitive range of every state component: the check kernel proves
+  // "no state out of bounds" from it without reading the states.  A NaN
+  // component gives a NaN range, which fails every comparison and so never
+  // proves anything.
+  std::vector<double> srange((size_t)2 * nx * P);
+  for (int32_t j = 0; j < P; ++j) {
+    for (int k = 0; k < nx; ++k) {
+      double mn = INFINITY, mx = -INFINITY;
+      bool nan = false;
+      for (int32_t s = off[j]; s < off[j + 1]; ++s) {
+        const double v = st[(int64_t)s * nx + k];
+        nan = nan || v != v;
+        mn = std::min(mn, v);
+        mx = std::max(mx, v);
+      }
+      if (nan) mn = mx = NAN;
+      srange[(size_t)k * P + j] = mn;
+      srange[((size_t)nx + k) * P + j] = mx;
+    }
+  }
   int rc;
   if ((rc = dev_alloc(&c->d_off, P + 1))) return rc;
   if ((rc = dev_alloc(&c->d_st, (int64_t)nx * S))) return rc;
   if ((rc = dev_alloc(&c->d_start, (int64_t)nx * P))) return rc;
   if ((rc = dev_alloc(&c->d_sweep, (int64_t)4 * P))) return rc;
+  if ((rc = dev_alloc(&c->d_srange, (int64_t)2 * nx * P))) return rc;
   if ((rc = dev_alloc(&c->d_goalidx, (int64_t)8 * P))) return rc;
   HIP_CHECK(hipMemcpy(c->d_off, off.data(), sizeof(int32_t) * (P + 1), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(c->d_st, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(c->d_start, start.data(), sizeof(double) * start.size(), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(c->d_sweep, sweep.data(), sizeof(double) * sweep.size(), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(c->d_srange, srange.data(), sizeof(double) * srange.size(), hipMemcpyHostToDevice));
   if (!quad) {
     if ((rc = dev_alloc(&c->d_geo, (int64_t)geo.size()))) return rc;
     HIP_CHECK(hipMemcpy(c->d_geo, geo.data(), sizeof(double) * geo.size(), hipMemcpyHostToDevice));

@@ -149,6 +149,7 @@ struct dp_ctx {
   double* d_start = nullptr;     // [nx][P]
   double* d_geo = nullptr;       // [4*nbox][S] : relx rely c s (per box)
   double* d_sweep = nullptr;     // [4][P] canonical swept footprint box (planar) / centre box (quad)
+  double* d_srange = nullptr;    // [2][nx][P] per-primitive min / max of each state component (canonical)
   double* d_act = nullptr;       // [A][nu] (quadrotor: nu = 4, 32-byte rows)
   float* d_act32 = nullptr;      // [A][nu] (16-byte rows)
   int32_t* d_goalidx = nullptr;  // [8][P]

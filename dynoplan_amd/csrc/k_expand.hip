@@ -83,6 +83,7 @@ struct dp_check_args {
   const double* geo;
   int32_t nbox;
   const double* sweep;
+  const double* srange;  // [2][NX][P] per-primitive min / max of each component
   int32_t P;
   const int32_t* goalidx;
   int32_t num_goal, check_goal;
@@ -207,11 +208,26 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
       oy = a.nodes[(int64_t)a.Ncap + i];
     }
     // ---- is_state_valid on every translated state -----------------------
+    // First from the primitive's component ranges: with the translation
+    // rounded the same way, (min + offset) >= lo and (max + offset) <= hi
+    // bound every translated state (rounding is monotone); other components
+    // compare their range directly.  Only pairs not cleared this way scan
+    // their states for the first offending index.
+    bool clear = true;
+    if (vp) {
+#pragma unroll
+      for (int c = 0; c < NX; ++c) {
+        const double o = c == 0 ? ox : (c == 1 ? oy : 0.0);
+        const double mn = a.srange[(int64_t)c * a.P + j], mx = a.srange[((int64_t)NX + c) * a.P + j];
+        const double tmn = c < 2 ? mn + o : mn, tmx = c < 2 ? mx + o : mx;
+        clear = clear && tmn >= S.lo[c] && tmx <= S.hi[c];
+      }
+    }
     int first_bad = INT_MAX;
-    for (int kb = 0; __ballot(vp && kb < L1) != 0; kb += LPP) {
+    for (int kb = 0; __ballot(vp && !clear && kb < L1) != 0; kb += LPP) {
       const int k = kb + k0;
       bool bad = false;
-      if (vp && k < L1) {
+      if (vp && !clear && k < L1) {
         const int64_t sidx = (int64_t)st0 + k;
         double s[NX];
         load_translated<NX>(a, sidx, ox, oy, s);
@@ -257,10 +273,10 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
         ncand += __popcll(m);
       }
       wave_lds_sync();
-      for (int kb = 0; __ballot(need_col && kb < L1) != 0; kb += LPP) {
+      for (int kb = 0; __ballot(need_col && ncand > 0 && kb < L1) != 0; kb += LPP) {
         const int k = kb + k0;
         bool col = false;
-        if (need_col && k < L1 && first_col == INT_MAX) {
+        if (need_col && ncand > 0 && k < L1 && first_col == INT_MAX) {
           const int64_t sidx = (int64_t)st0 + k;
           const double x = a.st[sidx] + ox;
           const double y = a.st[(int64_t)a.S + sidx] + oy;
@@ -447,6 +463,7 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   a.geo = c->d_geo;
   a.nbox = c->nbox;
   a.sweep = c->d_sweep;
+  a.srange = c->d_srange;
   a.P = c->P;
   a.goalidx = c->d_goalidx;
   a.num_goal = p->num_check_goal;
