@@ -124,17 +124,32 @@ __device__ __forceinline__ void load_translated(const dp_check_args& a, int64_t 
   for (int c = 2; c < NX; ++c) s[c] = a.st[(int64_t)c * a.S + sidx];
 }
 
-// Any visited state within r of x in neighbour cell cq of x's cell?
+// Any visited state within r of x in run k (0..8) of x's 27-cell block?  The
+// block is 9 runs of 3 cells adjacent in memory (x is the fastest grid axis):
+// run k covers rows c1 + k%3 - 1 and layers c2 + k/3 - 1 (the rules of
+// dp_cell_neighbor), x from c0 - 1 to c0 + 1 clipped to the grid.
 template <int NX>
-__device__ __forceinline__ bool grid_cell_any(const dp_dev_sys& S, const dp_grid_geom& G, const int32_t* start,
-                                              const double* pts, int64_t stride, const double* x, int cq, double r,
-                                              double T) {
+__device__ __forceinline__ bool grid_run_any(const dp_dev_sys& S, const dp_grid_geom& G, const int32_t* start,
+                                             const double* pts, int64_t stride, const double* x, int k, double r,
+                                             double T) {
   int32_t c[3];
   dp_cell_coords(G, x, c);
-  const int64_t cell = dp_cell_neighbor(G, c, cq);
-  if (cell < 0) return false;
-  const int32_t e = start[cell + 1];
-  for (int32_t p = start[cell]; p < e; ++p)
+  const int32_t n1 = c[1] + k % 3 - 1;
+  const int32_t d2 = k / 3 - 1;
+  if (n1 < 0 || n1 >= G.dims[1]) return false;
+  int32_t n2 = c[2] + d2;
+  if (G.dims[2] == 1) {
+    if (d2 != 0) return false;
+    n2 = 0;
+  } else if (G.periodic2) {
+    n2 = n2 < 0 ? n2 + G.dims[2] : (n2 >= G.dims[2] ? n2 - G.dims[2] : n2);
+  } else if (n2 < 0 || n2 >= G.dims[2]) {
+    return false;
+  }
+  const int64_t row = ((int64_t)n2 * G.dims[1] + n1) * G.dims[0];
+  const int32_t x0 = c[0] > 0 ? c[0] - 1 : 0, x1 = c[0] + 1 < G.dims[0] ? c[0] + 1 : G.dims[0] - 1;
+  const int32_t e = start[row + x1 + 1];
+  for (int32_t p = start[row + x0]; p < e; ++p)
     if (dp_within_col_pf<NX>(S, x, pts, stride, p, r, T)) return true;
   return false;
 }
@@ -172,6 +187,7 @@ __global__ __launch_bounds__(256) void k_node_obs(const double* nodes, int32_t N
 // Planar translate systems (unicycle1/2, car+trailer), FP64.
 template <int NX, int LPP>
 __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
+  static_assert(LPP >= 16, "the visited query takes 9 lanes of a pair");
   extern __shared__ double smem[];
   const dp_dev_sys& S = *a.sys;
   const int nobs = a.nobs;
@@ -189,7 +205,6 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
   const uint64_t ltmask = (1ull << k0) - 1ull;
   int16_t* cand = s_cand + (int64_t)(wave * SEG + seg) * nobs;
   const int32_t total = *a.total;
-  const int ncq = 27;
   const double Tdup = dp_pos_bound(S, a.r_dup);
   long long c_b = 0, c_c = 0, c_f = 0, c_g = 0, c_k = 0;
 
@@ -339,11 +354,8 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
     bool dupv = false;
     if (a.nv > 0 && __ballot(fr)) {
       bool hit = false;
-      for (int cb = 0; cb < ncq; cb += LPP) {
-        const int cq = cb + k0;
-        if (fr && cq < ncq && !hit)
-          hit = grid_cell_any<NX>(S, a.grid, a.vstart, a.vis, a.vstride, xf, cq, a.r_dup, Tdup);
-      }
+      // LPP >= 16 lanes per pair: the 9 runs in one round
+      if (fr && k0 < 9) hit = grid_run_any<NX>(S, a.grid, a.vstart, a.vis, a.vstride, xf, k0, a.r_dup, Tdup);
       dupv = ((__ballot(hit) >> shift) & segbits) != 0;
     }
     if (vp && k0 == 0) {
