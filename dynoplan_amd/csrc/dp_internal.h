@@ -14,6 +14,9 @@
 #define DP_NODE_OBS 128
 // Per-pair resolution value while undecided (ordered duplicate resolution).
 #define DP_R_UNDECIDED 0x7f
+// Pair flag set by the pre-resolution: every earlier contender is decided
+// and not NOVEL, so the pair is NOVEL unless its node's cap falls before it.
+#define DP_F_NUC 0x10
 // Pad added to swept boxes of the broad phase (m): bounds the rounding of
 // "canonical sweep + offset" versus per-state "state + offset" by 1e6x.
 #define DP_SWEEP_PAD 1e-9

@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) void k_cell_list(int64_t ncells, const int32_t
 // flight per CU.
 template <int NX>
 __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict__ Sp, dp_grid_geom G,
-                                                    const int8_t* __restrict__ flags,
+                                                    int8_t* __restrict__ flags,
                                                     const int32_t* __restrict__ pnode,
                                                     const int32_t* __restrict__ app, int32_t window, int64_t pcap,
                                                     const int32_t* __restrict__ cstart,
@@ -372,7 +372,63 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
         cnt_l = keep;
         if (over && cnt_l < DP_CONF_MAX) conf_ov[q_l] = (int32_t)off_l;
       }
-      if (list_l) conf_cnt[q_l] = (int16_t)(cnt_l < DP_CONF_MAX ? cnt_l : -1);
+      if (list_l) {
+        conf_cnt[q_l] = (int16_t)(cnt_l < DP_CONF_MAX ? cnt_l : -1);
+        // no earlier contender at all: NOVEL unless the cap falls before it
+        // (a duplicate of the visited set is not a contender)
+        if (cnt_l == 0) flags[q_l] = (int8_t)(flags[q_l] | DP_F_NUC);
+      }
+    }
+  }
+}
+
+// ---- Pre-resolution: statuses that are certain before the walk ----------
+// A contender without any earlier contender (flag DP_F_NUC, set by
+// k_conf_waves) or goal-forced is NOVEL unless its node's cap falls before
+// it.  Each conclusion below is a certainty, so the walk reaches the same
+// statuses; waiting waves only meet fewer undecided contenders.
+//
+// One wave per node, its FREE pairs in order: after the limit-th pair that
+// is NOVEL unless capped, every FREE pair is CAPPED; a pair that is NOVEL
+// unless capped, with fewer than limit possibly NOVEL pairs before it, is
+// NOVEL.
+__global__ __launch_bounds__(64) void k_pre_nodes(int32_t N, const int32_t* __restrict__ app,
+                                                  const int32_t* __restrict__ total,
+                                                  const int8_t* __restrict__ verdict,
+                                                  const int8_t* __restrict__ flags, int32_t* __restrict__ res,
+                                                  int32_t limit) {
+  const int32_t node = blockIdx.x;
+  if (node >= N) return;
+  const int lane = threadIdx.x;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const int32_t tot = *total;
+  const int32_t b = app[node] < tot ? app[node] : tot;
+  const int32_t e = app[node + 1] < tot ? app[node + 1] : tot;
+  int nuc = 0, possible = 0;
+  for (int32_t cb = b; cb < e; cb += 64) {
+    const int32_t p = cb + lane;
+    const bool valid = p < e && verdict[p] == DP_V_FREE;
+    const int8_t fl = valid ? flags[p] : 0;
+    const int r = valid ? res[p] : DP_R_NONE;
+    const bool dupv = (fl & 1) && !(fl & 2);
+    const bool not_nov = valid && (dupv || r == DP_R_CAPPED);
+    const bool nuc_p = valid && (r == DP_R_NOVEL || (fl & (2 | DP_F_NUC)) != 0);
+    const uint64_t nuc_m = __ballot(nuc_p);
+    const uint64_t pos_m = __ballot(valid && !not_nov);
+    const int nuc_before = nuc + __popcll(nuc_m & lt);
+    const int pos_before = possible + __popcll(pos_m & lt);
+    if (limit > 0 && nuc_before >= limit) {
+      if (valid && r != DP_R_CAPPED) res[p] = DP_R_CAPPED;
+    } else if (nuc_p && r != DP_R_NOVEL && (limit <= 0 || pos_before < limit)) {
+      res[p] = DP_R_NOVEL;
+    }
+    nuc += __popcll(nuc_m);
+    possible += __popcll(pos_m);
+    if (limit > 0 && nuc >= limit) {
+      // every later pair has at least limit such pairs before it: capped
+      for (int32_t t = cb + 64 + lane; t < e; t += 64)
+        if (verdict[t] == DP_V_FREE && res[t] != DP_R_CAPPED) res[t] = DP_R_CAPPED;
+      break;
     }
   }
 }
@@ -1026,6 +1082,9 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
         r_dup, c->d_conf_cnt, c->d_conf, c->d_conf_ov, c->d_conf_pool, c->conf_pool_cap, c->d_counters);
     k_round_init<<<dp_grid_blocks(N, 256, 1024), 256, 0, s>>>(N, c->d_app, c->d_total, c->d_node_cursor,
                                                               c->d_node_count, c->d_node_novel, c->d_counters);
+    if (p->limit_branching_factor > 0)
+      k_pre_nodes<<<N, 64, 0, s>>>(N, c->d_app, c->d_total, c->d_verdict, c->d_flags, c->d_res,
+                                   p->limit_branching_factor);
     round_args ra = make_round_args(c);
     ra.wait = DP_WAIT_POLLS;
     k_resolve_ticketed<NX><<<N, 64, 0, s>>>(ra);
