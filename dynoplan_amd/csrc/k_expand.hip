@@ -187,7 +187,7 @@ __global__ __launch_bounds__(256) void k_node_obs(const double* nodes, int32_t N
 // Planar translate systems (unicycle1/2, car+trailer), FP64.
 template <int NX, int LPP>
 __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
-  static_assert(LPP >= 16, "the visited query takes 9 lanes of a pair");
+  static_assert(LPP >= 4 && 64 % LPP == 0, "a pair takes 4 to 64 lanes of a wave");
   extern __shared__ double smem[];
   const dp_dev_sys& S = *a.sys;
   const int nobs = a.nobs;
@@ -325,15 +325,19 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
     const bool fr = vp && vd == DP_V_FREE;
     int chosen = -1;
     if (a.check_goal && __ballot(fr)) {
-      bool hit = false;
-      if (fr && k0 < a.num_goal) {
-        const int idx = a.goalidx[(int64_t)k0 * a.P + j];
-        double s[NX];
-        load_translated<NX>(a, (int64_t)st0 + idx, ox, oy, s);
-        hit = dp_within(S, s, a.goal, a.goal_bound);
+      // goal indices LPP at a time; the first in index order that hits wins
+      for (int gb = 0; gb < a.num_goal; gb += LPP) {
+        const int gk = gb + k0;
+        bool hit = false;
+        if (fr && chosen == -1 && gk < a.num_goal) {
+          const int idx = a.goalidx[(int64_t)gk * a.P + j];
+          double s[NX];
+          load_translated<NX>(a, (int64_t)st0 + idx, ox, oy, s);
+          hit = dp_within(S, s, a.goal, a.goal_bound);
+        }
+        const uint64_t m = (__ballot(hit) >> shift) & segbits;
+        if (fr && chosen == -1 && m) chosen = a.goalidx[(int64_t)(gb + __ffsll((long long)m) - 1) * a.P + j];
       }
-      const uint64_t m = (__ballot(hit) >> shift) & segbits;
-      if (fr && m) chosen = a.goalidx[(int64_t)(__ffsll((long long)m) - 1) * a.P + j];
     }
     double xf[NX];
     double gq = 0.0, hq = 0.0;
@@ -354,8 +358,9 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
     bool dupv = false;
     if (a.nv > 0 && __ballot(fr)) {
       bool hit = false;
-      // LPP >= 16 lanes per pair: the 9 runs in one round
-      if (fr && k0 < 9) hit = grid_run_any<NX>(S, a.grid, a.vstart, a.vis, a.vstride, xf, k0, a.r_dup, Tdup);
+      // the 9 runs over the pair's lanes (one round from 16 lanes per pair)
+      for (int k = k0; fr && k < 9 && !hit; k += LPP)
+        hit = grid_run_any<NX>(S, a.grid, a.vstart, a.vis, a.vstride, xf, k, a.r_dup, Tdup);
       dupv = ((__ballot(hit) >> shift) & segbits) != 0;
     }
     if (vp && k0 == 0) {
@@ -536,7 +541,9 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
     if constexpr (NX == 13) {
       return DP_ERR_ARG;
     } else {
-      if (lpp == 16) k_check_planar<NX, 16><<<blocks, 256, lds, s>>>(a);
+      if (lpp == 4) k_check_planar<NX, 4><<<blocks, 256, lds, s>>>(a);
+      else if (lpp == 8) k_check_planar<NX, 8><<<blocks, 256, lds, s>>>(a);
+      else if (lpp == 16) k_check_planar<NX, 16><<<blocks, 256, lds, s>>>(a);
       else if (lpp == 32) k_check_planar<NX, 32><<<blocks, 256, lds, s>>>(a);
       else k_check_planar<NX, 64><<<blocks, 256, lds, s>>>(a);
       return DP_OK;
