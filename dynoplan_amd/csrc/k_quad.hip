@@ -375,15 +375,21 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
     const int32_t s0 = a.off[j];
     const int32_t L1 = a.off[j + 1] - s0;
     const int32_t a0 = s0 - j;
-    double node[13];
+    // The node and the first translated state only enter g: its
+    // lower-bound term is formed here, so neither stays live in registers
+    // through the rollout.
+    double s[13];
+    double lb0;
+    {
+      double node[13];
 #pragma unroll
-    for (int c = 0; c < 13; ++c) node[c] = a.nodes[(int64_t)c * a.Ncap + i];
-    double s[13], t0[13], x[13];
+      for (int c = 0; c < 13; ++c) node[c] = a.nodes[(int64_t)c * a.Ncap + i];
 #pragma unroll
-    for (int c = 0; c < 13; ++c) {
-      const double v = a.st[(int64_t)c * a.S + s0];
-      s[c] = dp_is_offset_dim(S, c) ? v + node[c] : v;
-      t0[c] = s[c];
+      for (int c = 0; c < 13; ++c) {
+        const double v = a.st[(int64_t)c * a.S + s0];
+        s[c] = dp_is_offset_dim(S, c) ? v + node[c] : v;
+      }
+      lb0 = dp_lower_bound_time(S, node, s);
     }
     int first_bad = INT_MAX, first_col = INT_MAX;
     int chosen = -1, nn = 0;
@@ -406,8 +412,9 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
       while (chosen == -1 && nn < nng && a.goalidx[(int64_t)nn * a.P + j] == k) {
         if (dp_within(S, s, a.goal, a.goal_bound)) {
           chosen = k;
+          // the final state is the goal state: stored now, read back below
 #pragma unroll
-          for (int c = 0; c < 13; ++c) x[c] = s[c];
+          for (int c = 0; c < 13; ++c) a.fin[(int64_t)c * a.pcap + q] = s[c];
         }
         ++nn;
       }
@@ -432,13 +439,17 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
       a.flags[q] = 0;
       continue;
     }
+    double x[13];
     if (chosen == -1) {
 #pragma unroll
       for (int c = 0; c < 13; ++c) x[c] = s[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 13; ++c) x[c] = a.fin[(int64_t)c * a.pcap + q];
     }
     const double hq = dp_lower_bound_time(S, x, a.goal);
     const double cm = chosen != -1 ? chosen * S.dt : (L1 - 1) * S.dt;
-    const double gq = a.gpar[i] + cm + a.cdf * dp_lower_bound_time(S, node, t0);
+    const double gq = a.gpar[i] + cm + a.cdf * lb0;
     bool dupv = false;
     if (a.nv > 0) {
       int32_t cc[3];
