@@ -271,7 +271,17 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
       const int m = e - g0 < 64 ? e - g0 : 64;
       int32_t q_l = INT32_MAX;
       bool list_l = false;
+      // planar systems hold the group's states one per lane (broadcast by
+      // readlane below); the quadrotor's 13 components are read uniformly
+      constexpr int XL = NX == 13 ? 1 : NX;
+      double x_l[XL];
+#pragma unroll
+      for (int k = 0; k < XL; ++k) x_l[k] = 0.0;
       if (lane < m) {
+        if constexpr (NX != 13) {
+#pragma unroll
+          for (int k = 0; k < XL; ++k) x_l[k] = cx[(int64_t)k * pcap + g0 + lane];
+        }
         q_l = cidx[g0 + lane];
         list_l = !(flags[q_l] & 2);  // goal-forced: NOVEL without a list
         if (list_l && window > 0 && q_l - app[pnode[q_l]] >= window) {
@@ -315,10 +325,19 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
           for (uint64_t rem = who; rem; rem &= rem - 1ull) {
             const int c = __ffsll((long long)rem) - 1;
             const int32_t q = __builtin_amdgcn_readlane(q_l, c);
-            // the contender's state: wave-uniform loads
             double x[NX];
+            if constexpr (NX == 13) {
 #pragma unroll
-            for (int k = 0; k < NX; ++k) x[k] = cx[(int64_t)k * pcap + g0 + c];
+              for (int k = 0; k < NX; ++k) x[k] = cx[(int64_t)k * pcap + g0 + c];
+            } else {
+#pragma unroll
+              for (int k = 0; k < NX; ++k) {
+                const uint64_t bits = (uint64_t)__double_as_longlong(x_l[k < XL ? k : 0]);
+                const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)bits, c);
+                const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(bits >> 32), c);
+                x[k] = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+              }
+            }
             int cc = __builtin_amdgcn_readlane(cnt_l, c);
             const long long off = pool_pass ? (long long)__builtin_amdgcn_readlane((int)off_l, c) : 0;
 #pragma unroll
