@@ -27,7 +27,13 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
 # HBM bytes per launch of the check kernel, measured with rocprofv3 --pmc
 # FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy (profiles/ holds the run).
 MEASURED_TRAFFIC = {
-    "unicycle1_scale": (1.732e9, "profiles/k_check_planar_unicycle1_scale_hbm_traffic.txt"),
+    "unicycle1_scale": (1.391e9, "profiles/k_check_planar_unicycle1_scale_hbm_traffic.txt"),
+}
+# HBM bytes per launch of the resolution's two large kernels (k_conf_waves,
+# k_resolve_ticketed), from the same counter runs.  The resolution is bound
+# by its dependency chain, not by HBM; the figure shows how far from it.
+MEASURED_RESOLVE_TRAFFIC = {
+    "unicycle1_scale": (1.456e9, "profiles/k_check_planar_unicycle1_scale_hbm_traffic.txt"),
 }
 
 
@@ -381,6 +387,13 @@ def main():
     }
     if args.workload in MEASURED_TRAFFIC and not use_sur:
         line["roofline"]["traffic"], line["roofline"]["traffic_source"] = MEASURED_TRAFFIC[args.workload]
+    if args.workload in MEASURED_RESOLVE_TRAFFIC and not use_sur:
+        rb, src = MEASURED_RESOLVE_TRAFFIC[args.workload]
+        line["resolution_bound"] = {
+            "kernels": "k_conf_waves + k_resolve_ticketed", "bound": "dependency chain (latency)",
+            "stage_ms": stage_ms["resolve"], "traffic": rb, "traffic_source": src,
+            "hbm_gbs": rb / (stage_ms["resolve"] * 1e-3) / 1e9,
+            "hbm_frac": rb / (stage_ms["resolve"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
     if use_sur:
         # Untimed validation: the exact path on the same batch decides every
         # pair; the surrogate run must keep every feasible edge and give the
