@@ -63,6 +63,24 @@ def test_unicycle1_goal_hits(need_gpu):
     assert_parity(gpu, ora, "goal")
 
 
+@pytest.mark.parametrize("lpp", [4, 8, 16, 64])
+def test_goal_hits_every_pair_shape(need_gpu, lpp):
+    """Eight intermediate-goal indices (the maximum) with 4 to 64 lanes per
+    pair: narrow shapes test the indices in several rounds and must still
+    pick the first hit in index order."""
+    wl = synth.unicycle1_small(seed=14)
+    goal = wl.nodes[3].copy()
+    p = synth.make_params(goal=goal, delta=0.6, limit_branching_factor=5, num_check_goal=8)
+    with expander_for(wl) as ex:
+        ex.set_tiles(lanes_per_pair=lpp, check_blocks=64)
+        cg = ex.expand(wl.nodes, wl.g_parent, p)
+        gpu = ex.fetch()
+    ora, co = O.expand(wl, params=p)
+    assert co["goal_hits"] > 0
+    assert_parity(gpu, ora, f"goal-lpp{lpp}")
+    counts_consistent(cg)
+
+
 def test_unicycle1_no_goal_check_alpha(need_gpu):
     wl = synth.unicycle1_small(seed=13)
     p = synth.make_params(goal=wl.params.goal[:3], alpha=0.3, delta=0.5, check_intermediate_goal=False)
