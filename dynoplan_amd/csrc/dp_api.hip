@@ -841,8 +841,8 @@ int dp_last_timing(dp_ctx* c, float* ms5) {
 int dp_set_tiles(dp_ctx* c, const dp_tile_config* t) {
   ARG_CHECK(c && t, "null argument");
   const int l = t->lanes_per_pair, q = t->quad64_threads;
-  ARG_CHECK(l == 0 || l == 4 || l == 8 || l == 16 || l == 32 || l == 64,
-            "lanes_per_pair must be 0, 4, 8, 16, 32 or 64");
+  ARG_CHECK(l == 0 || l == 2 || l == 4 || l == 8 || l == 16 || l == 32 || l == 64,
+            "lanes_per_pair must be 0, 2, 4, 8, 16, 32 or 64");
   ARG_CHECK(q == 64 || q == 128 || q == 256, "quad64_threads must be 64, 128 or 256");
   ARG_CHECK(t->check_blocks >= 1 && t->check_blocks <= 65535 && t->quad32_blocks >= 1 &&
                 t->quad32_blocks <= 65535 && t->quad64_blocks >= 1 && t->quad64_blocks <= 65535,
@@ -874,7 +874,7 @@ int dp_autotune(dp_ctx* c, const dp_expand_params* p, dp_tile_config* best, dp_t
           cand.push_back(t);
         }
   } else {
-    for (int l : {4, 8, 16, 32, 64})
+    for (int l : {2, 4, 8, 16, 32, 64})
       for (int b : {1024, 2048, 4096, 8192}) {
         dp_tile_config t = base;
         t.lanes_per_pair = l;

@@ -30,6 +30,8 @@
 // Resolution trace, per node: start and end clock of its wave, cycles
 // waiting, evaluating lists and walking chunks, chunks, waits, polls.
 #define DP_TRACE_FIELDS 8
+// LDS per workgroup on gfx950
+#define DP_LDS_MAX 163840
 #define DP_SPIN_LIMIT_WAITING (1 << 10)
 // Earlier-conflict slots stored per pair (more: the resolution scans the grid).
 #define DP_CONF_SLOTS 32

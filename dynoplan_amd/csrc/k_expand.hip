@@ -187,7 +187,7 @@ __global__ __launch_bounds__(256) void k_node_obs(const double* nodes, int32_t N
 // Planar translate systems (unicycle1/2, car+trailer), FP64.
 template <int NX, int LPP>
 __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
-  static_assert(LPP >= 4 && 64 % LPP == 0, "a pair takes 4 to 64 lanes of a wave");
+  static_assert(LPP >= 2 && 64 % LPP == 0, "a pair takes 2 to 64 lanes of a wave");
   extern __shared__ double smem[];
   const dp_dev_sys& S = *a.sys;
   const int nobs = a.nobs;
@@ -523,10 +523,16 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
     c->timing_valid = !c->no_events;
     return DP_OK;
   }
-  const int lpp = c->tiles.lanes_per_pair > 0 ? c->tiles.lanes_per_pair
-                                             : (c->maxlen <= 16 ? 16 : (c->maxlen <= 32 ? 32 : 64));
+  int lpp = c->tiles.lanes_per_pair > 0 ? c->tiles.lanes_per_pair
+                                       : (c->maxlen <= 16 ? 16 : (c->maxlen <= 32 ? 32 : 64));
+  // LDS: the obstacles plus one candidate list per pair group; narrow shapes
+  // widen until that fits a workgroup
+  auto lds_of = [&](int l) {
+    return sizeof(double) * 4 * c->nb + sizeof(int16_t) * (size_t)kWaves * (64 / l) * (c->nb > 0 ? c->nb : 1);
+  };
+  while (lpp < 64 && lds_of(lpp) > (size_t)DP_LDS_MAX) lpp *= 2;
   const int seg = 64 / lpp;
-  const size_t lds = sizeof(double) * 4 * c->nb + sizeof(int16_t) * (size_t)kWaves * seg * (c->nb > 0 ? c->nb : 1);
+  const size_t lds = lds_of(lpp);
   // The pair count is on the device; size the grid from the bound N*P.
   const int64_t groups = ((int64_t)N * c->P + seg - 1) / seg;
   const int blocks = dp_grid_blocks(groups, kWaves, c->tiles.check_blocks);
@@ -541,7 +547,8 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
     if constexpr (NX == 13) {
       return DP_ERR_ARG;
     } else {
-      if (lpp == 4) k_check_planar<NX, 4><<<blocks, 256, lds, s>>>(a);
+      if (lpp == 2) k_check_planar<NX, 2><<<blocks, 256, lds, s>>>(a);
+      else if (lpp == 4) k_check_planar<NX, 4><<<blocks, 256, lds, s>>>(a);
       else if (lpp == 8) k_check_planar<NX, 8><<<blocks, 256, lds, s>>>(a);
       else if (lpp == 16) k_check_planar<NX, 16><<<blocks, 256, lds, s>>>(a);
       else if (lpp == 32) k_check_planar<NX, 32><<<blocks, 256, lds, s>>>(a);

@@ -221,7 +221,8 @@ int dp_last_timing(dp_ctx* ctx, float* ms5);
 /* Launch shapes of the check stage.  Every configuration gives identical
  * results; dp_autotune picks the fastest for the uploaded batch. */
 typedef struct dp_tile_config {
-  int32_t lanes_per_pair;   /* planar check: lanes per pair, 4, 8, 16, 32 or 64 (0: from the longest primitive) */
+  int32_t lanes_per_pair;   /* planar check: lanes per pair, 2 to 64 (0: from the longest primitive;
+                                 a shape whose LDS would not fit widens) */
   int32_t check_blocks;     /* planar check: workgroups (grid-stride) */
   int32_t quad32_blocks;    /* quadrotor FP32 pass: workgroups */
   int32_t quad64_threads;   /* quadrotor FP64 pass: threads per workgroup, 64, 128 or 256 */

@@ -63,9 +63,9 @@ def test_unicycle1_goal_hits(need_gpu):
     assert_parity(gpu, ora, "goal")
 
 
-@pytest.mark.parametrize("lpp", [4, 8, 16, 64])
+@pytest.mark.parametrize("lpp", [2, 4, 8, 16, 64])
 def test_goal_hits_every_pair_shape(need_gpu, lpp):
-    """Eight intermediate-goal indices (the maximum) with 4 to 64 lanes per
+    """Eight intermediate-goal indices (the maximum) with 2 to 64 lanes per
     pair: narrow shapes test the indices in several rounds and must still
     pick the first hit in index order."""
     wl = synth.unicycle1_small(seed=14)

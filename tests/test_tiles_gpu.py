@@ -21,7 +21,7 @@ def test_planar_shapes_and_autotune(need_gpu):
     with expander_for(wl) as ex:
         c0 = ex.expand(wl.nodes, wl.g_parent, wl.params)
         ref = ex.fetch()
-        for lpp in (4, 8, 16, 32, 64):
+        for lpp in (2, 4, 8, 16, 32, 64):
             for blocks in (7, 4096):
                 ex.set_tiles(lanes_per_pair=lpp, check_blocks=blocks)
                 c = ex.expand(wl.nodes, wl.g_parent, wl.params)
@@ -31,7 +31,7 @@ def test_planar_shapes_and_autotune(need_gpu):
         ex.upload_nodes(wl.nodes, wl.g_parent)
         best, trials = ex.autotune(wl.params)
         print("autotune", best, [(t["lanes_per_pair"], t["check_blocks"], round(t["check_ms"], 4)) for t in trials])
-        assert len(trials) == 20 and best["check_ms"] == min(t["check_ms"] for t in trials)
+        assert len(trials) == 24 and best["check_ms"] == min(t["check_ms"] for t in trials)
         assert ex.tiles() == best
         ex.expand(wl.nodes, wl.g_parent, wl.params)
         same(ex.fetch(), ref)
