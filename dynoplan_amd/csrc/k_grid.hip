@@ -517,7 +517,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
   // undecided), exclusive scan of list lengths, overflow offsets
   __shared__ unsigned long long s_same[64], s_undec[64];
   __shared__ int32_t s_pre[64], s_ov[64], s_flag[64];
-  constexpr int kCoop = 8;  // list entries per lane in flight
+  constexpr int kCoop = 16;  // list entries per lane in flight
   const int lane = threadIdx.x & 63;
   const int32_t tot = *a.total;
   const int32_t end = a.app[node + 1] < tot ? a.app[node + 1] : tot;
@@ -654,36 +654,37 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
       __syncthreads();
       for (int f0 = 0; f0 < total_e; f0 += 64 * kCoop) {
         int32_t os[kCoop];
-        int rs[kCoop], own[kCoop], ee[kCoop];
+        int rs[kCoop], oe[kCoop];  // oe: owner lane << 16 | entry index, or -1
 #pragma unroll
         for (int k = 0; k < kCoop; ++k) {
           const int f = f0 + 64 * k + lane;
-          own[k] = -1;
+          oe[k] = -1;
           os[k] = INT32_MAX;
           if (f < total_e) {
             int lo = 0;
 #pragma unroll
             for (int st = 32; st > 0; st >>= 1)  // lo + st <= 63
               if (s_pre[lo + st] <= f) lo += st;
-            own[k] = lo;
-            ee[k] = f - s_pre[lo];
+            const int e = f - s_pre[lo];  // < DP_CONF_MAX
+            oe[k] = (lo << 16) | e;
             const int32_t q = cb + lo;
-            os[k] = ee[k] < DP_CONF_SLOTS ? a.conf[(int64_t)ee[k] * a.pcap + q]
-                                          : a.pool[(int64_t)s_ov[lo] + (ee[k] - DP_CONF_SLOTS)];
+            os[k] = e < DP_CONF_SLOTS ? a.conf[(int64_t)e * a.pcap + q]
+                                      : a.pool[(int64_t)s_ov[lo] + (e - DP_CONF_SLOTS)];
           }
         }
 #pragma unroll
         for (int k = 0; k < kCoop; ++k) rs[k] = os[k] < cb ? a.res[os[k]] : (int)DP_R_NONE;
 #pragma unroll
         for (int k = 0; k < kCoop; ++k) {
-          if (own[k] < 0) continue;
+          if (oe[k] < 0) continue;
+          const int own = oe[k] >> 16, ee = oe[k] & 0xffff;
           if (os[k] >= cb) {
-            atomicOr(&s_same[own[k]], 1ull << (os[k] - cb));
+            atomicOr(&s_same[own], 1ull << (os[k] - cb));
           } else if (rs[k] == DP_R_NOVEL) {
-            atomicOr(&s_flag[own[k]], 1);
+            atomicOr(&s_flag[own], 1);
           } else if (rs[k] == DP_R_UNDECIDED) {
-            if (ee[k] < 64) atomicOr(&s_undec[own[k]], 1ull << ee[k]);
-            else atomicOr(&s_flag[own[k]], 2);
+            if (ee < 64) atomicOr(&s_undec[own], 1ull << ee);
+            else atomicOr(&s_flag[own], 2);
           }
         }
       }
