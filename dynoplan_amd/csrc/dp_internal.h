@@ -302,3 +302,33 @@ void dp_bump_epoch();
 uint64_t dp_epoch();
 int dp_prepare_goal_index(dp_ctx* c, const dp_expand_params* p, hipStream_t s);
 int dp_size_pairs(dp_ctx* c, const dp_expand_params* p, hipStream_t s);
+
+// Any visited state within r of x in run k (0..8) of x's 27-cell block?  The
+// block is 9 runs of 3 cells adjacent in memory (x is the fastest grid axis):
+// run k covers rows c1 + k%3 - 1 and layers c2 + k/3 - 1 (the rules of
+// dp_cell_neighbor), x from c0 - 1 to c0 + 1 clipped to the grid.
+template <int NX>
+__device__ __forceinline__ bool dp_grid_run_any(const dp_dev_sys& S, const dp_grid_geom& G, const int32_t* start,
+                                             const double* pts, int64_t stride, const double* x, int k, double r,
+                                             double T) {
+  int32_t c[3];
+  dp_cell_coords(G, x, c);
+  const int32_t n1 = c[1] + k % 3 - 1;
+  const int32_t d2 = k / 3 - 1;
+  if (n1 < 0 || n1 >= G.dims[1]) return false;
+  int32_t n2 = c[2] + d2;
+  if (G.dims[2] == 1) {
+    if (d2 != 0) return false;
+    n2 = 0;
+  } else if (G.periodic2) {
+    n2 = n2 < 0 ? n2 + G.dims[2] : (n2 >= G.dims[2] ? n2 - G.dims[2] : n2);
+  } else if (n2 < 0 || n2 >= G.dims[2]) {
+    return false;
+  }
+  const int64_t row = ((int64_t)n2 * G.dims[1] + n1) * G.dims[0];
+  const int32_t x0 = c[0] > 0 ? c[0] - 1 : 0, x1 = c[0] + 1 < G.dims[0] ? c[0] + 1 : G.dims[0] - 1;
+  const int32_t e = start[row + x1 + 1];
+  for (int32_t p = start[row + x0]; p < e; ++p)
+    if (dp_within_col_pf<NX>(S, x, pts, stride, p, r, T)) return true;
+  return false;
+}
