@@ -204,7 +204,7 @@ struct dp_ctx {
   int32_t* d_node_novel = nullptr;  // [N_cap+1] NOVEL pairs per node, then offsets
   double* d_cont_x = nullptr;       // [nx][pair_cap] contender final states sorted by cell
   int16_t* d_conf_cnt = nullptr;    // [pair_cap] stored earlier conflicts per pair (-1: scan)
-  int32_t* d_conf = nullptr;        // [DP_CONF_SLOTS][pair_cap] earlier conflicting contenders
+  int32_t* d_conf = nullptr;        // [pair_cap][DP_CONF_SLOTS] earlier conflicting contenders
   int32_t* d_conf_ov = nullptr;     // [pair_cap] offset of entries DP_CONF_SLOTS.. in the overflow pool
   int32_t* d_conf_pool = nullptr;   // [pool_cap] overflow entries (bump-allocated per batch)
   int64_t conf_pool_cap = 0;

@@ -158,7 +158,8 @@ __device__ __forceinline__ void scan_earlier(const dp_dev_sys& S, const dp_grid_
   }
 }
 
-// Earlier-conflict lists, stored inline (DP_CONF_SLOTS per pair, slot-major)
+// Earlier-conflict lists, stored inline (DP_CONF_SLOTS per pair, pair-major:
+// a list's entries are adjacent for its writer and a chunk's lists for its walk)
 // for contenders: FREE pairs that are neither goal-forced nor duplicates of
 // the visited set (with a window, only those at position < window in their
 // node).  Lists longer than DP_CONF_SLOTS continue in the overflow pool;
@@ -367,7 +368,7 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
               }
               const uint64_t mk = __ballot(hit);
               const int pos = cc + __popcll(mk & ltmask);
-              if (hit && !pool_pass && pos < DP_CONF_SLOTS) conf[(int64_t)pos * pcap + q] = o[u];
+              if (hit && !pool_pass && pos < DP_CONF_SLOTS) conf[(int64_t)q * DP_CONF_SLOTS + pos] = o[u];
               if (hit && pool_pass && pos >= DP_CONF_SLOTS) pool[off + (pos - DP_CONF_SLOTS)] = o[u];
               cc += __popcll(mk);
             }
@@ -590,7 +591,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
     uint64_t undec = 0;  // entries (< 64) of a stored list whose contender was undecided
     // entry e of this lane's stored earlier-conflict list
     auto entry = [&](int e) -> int32_t {
-      return e < DP_CONF_SLOTS ? a.conf[(int64_t)e * a.pcap + p] : a.pool[(int64_t)ov_p + (e - DP_CONF_SLOTS)];
+      return e < DP_CONF_SLOTS ? a.conf[(int64_t)p * DP_CONF_SLOTS + e] : a.pool[(int64_t)ov_p + (e - DP_CONF_SLOTS)];
     };
     // Earlier contenders of pair p: those in this chunk go to `same` (the
     // walk below decides them in order); an earlier-chunk contender already
@@ -668,7 +669,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
             const int e = f - s_pre[lo];  // < DP_CONF_MAX
             oe[k] = (lo << 16) | e;
             const int32_t q = cb + lo;
-            os[k] = e < DP_CONF_SLOTS ? a.conf[(int64_t)e * a.pcap + q]
+            os[k] = e < DP_CONF_SLOTS ? a.conf[(int64_t)q * DP_CONF_SLOTS + e]
                                       : a.pool[(int64_t)s_ov[lo] + (e - DP_CONF_SLOTS)];
           }
         }
