@@ -730,11 +730,7 @@ int dp_fetch_pairs(dp_ctx* c, dp_pairs_out* o) {
   if (o->verdict) HIP_CHECK(hipMemcpy(o->verdict, c->d_verdict, n, hipMemcpyDeviceToHost));
   if (o->first_k) HIP_CHECK(hipMemcpy(o->first_k, c->d_firstk, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
   if (o->goal_index) HIP_CHECK(hipMemcpy(o->goal_index, c->d_goal, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-  if (o->resolution) {
-    std::vector<int32_t> r((size_t)n);
-    HIP_CHECK(hipMemcpy(r.data(), c->d_res, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-    for (int64_t q = 0; q < n; ++q) o->resolution[q] = (int8_t)r[q];
-  }
+  if (o->resolution) HIP_CHECK(hipMemcpy(o->resolution, c->d_res, n, hipMemcpyDeviceToHost));
   std::vector<int8_t> vd((size_t)n);
   HIP_CHECK(hipMemcpy(vd.data(), c->d_verdict, n, hipMemcpyDeviceToHost));
   const double nan = NAN;

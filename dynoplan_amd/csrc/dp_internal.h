@@ -190,7 +190,7 @@ struct dp_ctx {
   int8_t* d_verdict = nullptr;
   int32_t* d_firstk = nullptr;
   int32_t* d_goal = nullptr;
-  int32_t* d_res = nullptr;
+  int8_t* d_res = nullptr;
   int8_t* d_flags = nullptr;  // bit0 dup-vs-visited, bit1 forced (goal), bit2 near-contact
   double* d_final = nullptr;  // [nx][pair_cap]
   double* d_g = nullptr;

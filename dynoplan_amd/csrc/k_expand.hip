@@ -102,7 +102,7 @@ struct dp_check_args {
   int8_t* verdict;
   int32_t* firstk;
   int32_t* goal_out;
-  int32_t* res;
+  int8_t* res;
   int8_t* flags;
   double* fin;
   int64_t pcap;

@@ -415,7 +415,7 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
 __global__ __launch_bounds__(64) void k_pre_nodes(int32_t N, const int32_t* __restrict__ app,
                                                   const int32_t* __restrict__ total,
                                                   const int8_t* __restrict__ verdict,
-                                                  const int8_t* __restrict__ flags, int32_t* __restrict__ res,
+                                                  const int8_t* __restrict__ flags, int8_t* __restrict__ res,
                                                   int32_t limit) {
   const int32_t node = blockIdx.x;
   if (node >= N) return;
@@ -487,7 +487,7 @@ struct round_args {
   const int32_t* cidx;
   const double* cx;
   double r_dup;
-  int32_t* res;
+  int8_t* res;
   int32_t* cursor;
   int32_t* count;
   int32_t* node_novel;
@@ -558,7 +558,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
       // capped.  Other nodes may be waiting for any of them, so the whole
       // tail is published in one pass with its verdict loads in flight
       // together, not chunk by chunk.
-      if (fr) __hip_atomic_store(&a.res[p], (int)DP_R_CAPPED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (fr) __hip_atomic_store(&a.res[p], (int8_t)DP_R_CAPPED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       n_cap += __popcll(__ballot(fr));
       for (int32_t t0 = cb + 64; t0 < end; t0 += 64 * 8) {
         bool f8[8];
@@ -570,7 +570,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
           if (f8[k])
-            __hip_atomic_store(&a.res[t0 + 64 * k + lane], (int)DP_R_CAPPED, __ATOMIC_RELAXED,
+            __hip_atomic_store(&a.res[t0 + 64 * k + lane], (int8_t)DP_R_CAPPED, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
           n_cap += __popcll(__ballot(f8[k]));
         }
@@ -723,7 +723,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
                       : (novel_m & lane_bit) ? DP_R_NOVEL
                       : (dupvr_m & lane_bit) ? DP_R_DUP_VISITED
                                              : DP_R_DUP_BATCH;
-        __hip_atomic_store(&a.res[p], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&a.res[p], (int8_t)r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
       pub_m |= dec;
     };
@@ -975,7 +975,7 @@ __global__ __launch_bounds__(64) void k_resolve_ticketed(round_args a) {
 template <int NX>
 __global__ __launch_bounds__(64) void k_gather_novel(int32_t N, const int32_t* __restrict__ app,
                                                      const int32_t* __restrict__ total,
-                                                     const int32_t* __restrict__ res, const int32_t* __restrict__ dst,
+                                                     const int8_t* __restrict__ res, const int32_t* __restrict__ dst,
                                                      const double* __restrict__ fin, int64_t pcap,
                                                      double* __restrict__ vis, int64_t vstride, int64_t vbase) {
   const int32_t node = blockIdx.x;

@@ -148,7 +148,7 @@ struct infer_args {
   int8_t* verdict;
   int32_t* firstk;
   int32_t* goal_out;
-  int32_t* res;
+  int8_t* res;
   long long* counters;
 };
 

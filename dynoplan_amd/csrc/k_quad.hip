@@ -56,7 +56,7 @@ struct dp_quad_args {
   int8_t* verdict;
   int32_t* firstk;
   int32_t* goal_out;
-  int32_t* res;
+  int8_t* res;
   int8_t* flags;
   double* fin;
   int64_t pcap;
