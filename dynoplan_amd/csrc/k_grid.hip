@@ -1,6 +1,6 @@
 // k_grid.hip -- duplicate tests of the expansion: the visited-set grid (T_n),
-// the in-batch contender grid, earlier-conflict lists and the round-based
-// resolution that reproduces the reference's sequential insert semantics
+// the in-batch contender grid, earlier-conflict lists and the resolution that
+// reproduces the reference's sequential insert semantics
 // (dbastar.cpp:850-918) for a whole batch.
 //
 // Sequential semantics restated: in canonical (node, primitive) order, a FREE
@@ -10,11 +10,14 @@
 // otherwise it is a duplicate.
 //
 // Parallel form: each FREE pair that could become NOVEL ("contender") gets
-// the list of earlier contenders within the radius.  Rounds then advance a
-// cursor per node through its pairs in order; a pair is decided as soon as
-// every earlier conflicting contender is decided (or one is NOVEL).  The
-// smallest undecided pair of the batch is always decidable, so the rounds
-// terminate; measured depth on the 4096-node unicycle batch is ~20-30.
+// the list of earlier contenders within the radius (k_conf_waves).  Statuses
+// that are certain from the lists alone are published first (k_pre_nodes).
+// Then one wave per node, claimed in node order by a ticket, walks its pairs
+// 64 at a time (k_resolve_ticketed); a pair is decided as soon as every
+// earlier conflicting contender is decided (or one is NOVEL).  The smallest
+// undecided pair of the batch is always decidable, so the walk terminates;
+// every wait is bounded, and a host-driven round kernel (k_round) finishes
+// what a wave gives up on.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
