@@ -450,24 +450,13 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
     const double hq = dp_lower_bound_time(S, x, a.goal);
     const double cm = chosen != -1 ? chosen * S.dt : (L1 - 1) * S.dt;
     const double gq = a.gpar[i] + cm + a.cdf * lb0;
-    bool dupv = false;
-    if (a.nv > 0) {
-      int32_t cc[3];
-      dp_cell_coords(a.grid, x, cc);
-      for (int cq = 0; cq < 27 && !dupv; ++cq) {
-        const int64_t cell = dp_cell_neighbor(a.grid, cc, cq);
-        if (cell < 0) continue;
-        const int32_t e = a.vstart[cell + 1];
-        for (int32_t p = a.vstart[cell]; p < e && !dupv; ++p) dupv = dp_within_col<13>(S, x, a.vis, a.vstride, p, a.r_dup);
-      }
-    }
 #pragma unroll
     for (int c = 0; c < 13; ++c) a.fin[(int64_t)c * a.pcap + q] = x[c];
     a.gout[q] = gq;
     a.hout[q] = hq;
     a.goal_out[q] = chosen;
     a.res[q] = DP_R_UNDECIDED;
-    a.flags[q] = (int8_t)((dupv ? 1 : 0) | (chosen != -1 ? 2 : 0));
+    a.flags[q] = (int8_t)(chosen != -1 ? 2 : 0);  // the duplicate bit is set by k_quad_dupv
     ++c_f;
     c_g += chosen != -1;
   }
@@ -477,6 +466,25 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
   if (c_g) atomicAdd((unsigned long long*)&a.counters[DPC_GOAL], (unsigned long long)c_g);
   if (c_k) atomicAdd((unsigned long long*)&a.counters[DPC_CHECKED], (unsigned long long)c_k);
   if (c_r) atomicAdd((unsigned long long*)&a.counters[DPC_RECHECK], (unsigned long long)c_r);
+}
+
+// Duplicate-of-visited test of the FREE pairs' final states, after the FP64
+// pass: one thread per pair, the 9 runs of the 27-cell block, apart from the
+// rollout so that neither kernel's registers limit the other's occupancy.
+__global__ __launch_bounds__(256) void k_quad_dupv(dp_quad_args a) {
+  const dp_dev_sys& S = *a.sys;
+  const int32_t tot = *a.total;
+  const double T = dp_pos_bound(S, a.r_dup);
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+    if (a.verdict[q] != DP_V_FREE) continue;
+    double x[13];
+#pragma unroll
+    for (int c = 0; c < 13; ++c) x[c] = a.fin[(int64_t)c * a.pcap + q];
+    bool dupv = false;
+    for (int k = 0; k < 9 && !dupv; ++k)
+      dupv = dp_grid_run_any<13>(S, a.grid, a.vstart, a.vis, a.vstride, x, k, a.r_dup, T);
+    if (dupv) a.flags[q] = (int8_t)(a.flags[q] | 1);
+  }
 }
 
 __global__ void k_quad_chunks(int32_t N, const int32_t* __restrict__ app, const int32_t* __restrict__ total,
@@ -552,6 +560,7 @@ int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, doub
   }
   const int t64 = c->tiles.quad64_threads;
   k_quad64<<<dp_grid_blocks(c->pair_cap, t64, c->tiles.quad64_blocks), t64, 0, s>>>(a);
+  if (c->nv > 0) k_quad_dupv<<<dp_grid_blocks(c->pair_cap, 256, 8192), 256, 0, s>>>(a);
   DP_HIP_RET(hipGetLastError());
   return DP_OK;
 }
