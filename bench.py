@@ -27,13 +27,13 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
 # HBM bytes per launch of the check kernel, measured with rocprofv3 --pmc
 # FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy (profiles/ holds the run).
 MEASURED_TRAFFIC = {
-    "unicycle1_scale": (1.392e9, "profiles/k_check_planar_unicycle1_scale_hbm_traffic.txt"),
+    "unicycle1_scale": (1.386e9, "profiles/k_check_planar_unicycle1_scale_hbm_traffic.txt"),
 }
 # HBM bytes per launch of the resolution's two large kernels (k_conf_waves,
 # k_resolve_ticketed), from the same counter runs.  The resolution is bound
 # by its dependency chain, not by HBM; the figure shows how far from it.
 MEASURED_RESOLVE_TRAFFIC = {
-    "unicycle1_scale": (0.865e9, "profiles/k_check_planar_unicycle1_scale_hbm_traffic.txt"),
+    "unicycle1_scale": (0.804e9, "profiles/k_check_planar_unicycle1_scale_hbm_traffic.txt"),
 }
 
 
