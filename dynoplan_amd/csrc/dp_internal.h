@@ -33,8 +33,9 @@
 // LDS per workgroup on gfx950
 #define DP_LDS_MAX 163840
 #define DP_SPIN_LIMIT_WAITING (1 << 10)
-// Earlier-conflict slots stored per pair (more: the resolution scans the grid).
-#define DP_CONF_SLOTS 32
+// Earlier-conflict entries stored inline per pair (longer lists continue in
+// the overflow pool).
+#define DP_CONF_SLOTS 64
 // longest stored earlier-conflict list (inline slots + overflow pool)
 #define DP_CONF_MAX 4096
 
