@@ -129,6 +129,106 @@ def test_lower_bound_time_closed_forms(lib):
     assert lbt([0, 0, 0], [0.1, 0, 1.0]) == 2.0
 
 
+def _desc_distinct(sysid):
+    """Default description with pairwise distinct weights and speeds, so that
+    a term read with the wrong weight or speed index changes the value."""
+    d = synth.default_desc_static(sysid)
+    for k, w in enumerate((1.0, 0.5, 0.375, 0.75)):
+        d.dist_w[k] = w
+    d.max_vel, d.max_angular_vel, d.max_acc, d.max_angular_acc = 0.5, 0.25, 0.125, 2.0
+    return d
+
+
+def test_metric_component_closed_forms(lib):
+    """States that differ in one component only give exactly w_k * |delta_k|
+    (dyadic values, so every product and sum is exact).  Dropping a term,
+    reading it with another weight or from a neighbouring component fails."""
+    u2 = _desc_distinct(N.SYS_UNICYCLE2)
+    a = [1.0, 2.0, 0.5, 0.125, -0.25]
+    b = list(a)
+    b[3] = 0.375
+    assert dist(lib, u2, a, b) == 0.375 * 0.25
+    b = list(a)
+    b[4] = 0.5
+    assert dist(lib, u2, a, b) == 0.75 * 0.75
+    b = [1.0, 2.0, 0.5, 0.375, 0.5]
+    assert dist(lib, u2, a, b) == 0.375 * 0.25 + 0.75 * 0.75
+
+    car = _desc_distinct(N.SYS_CAR_TRAILER)
+    a = [1.0, 2.0, 0.5, 0.5]
+    b = [1.0, 2.0, 0.5, -0.25]
+    assert dist(lib, car, a, b) == 0.375 * 0.75
+    # trailer angle across +-pi: the short way round
+    a = [1.0, 2.0, 0.5, 3.0]
+    b = [1.0, 2.0, 0.5, -3.0]
+    assert dist(lib, car, a, b) == 0.375 * (math.tau - 6.0)
+    # heading of the tractor and trailer are separate terms
+    b = [1.0, 2.0, -0.25, -3.0]
+    assert dist(lib, car, a, b) == 0.5 * 0.75 + 0.375 * (math.tau - 6.0)
+
+    qd = _desc_distinct(N.SYS_QUADROTOR)
+    a = np.zeros(13)
+    a[:3] = (1.0, 2.0, 3.0)
+    a[6] = 1.0
+    a[7:10] = (0.5, -0.5, 0.25)
+    a[10:13] = (1.0, 0.0, -1.0)
+    # every component of a block changes, by different amounts, so reading
+    # the block one component early or late changes the norm
+    b = a.copy()
+    b[7:10] += (0.5, 0.5, 0.25)          # |dv| = 0.75 exactly
+    assert dist(lib, qd, a, b) == 0.375 * 0.75
+    b = a.copy()
+    b[10:13] += (0.25, -0.5, 0.5)        # |dw| = 0.75 exactly
+    assert dist(lib, qd, a, b) == 0.75 * 0.75
+    b = a.copy()
+    b[:3] += (0.5, 0.25, 0.5)            # |dp| = 0.75 exactly
+    assert dist(lib, qd, a, b) == 1.0 * 0.75
+    b = a.copy()
+    b[7:10] += (0.5, 0.5, 0.25)
+    b[10:13] += (0.25, -0.5, 0.5)
+    assert dist(lib, qd, a, b) == 0.375 * 0.75 + 0.75 * 0.75
+
+
+def test_lower_bound_time_closed_forms_all_systems(lib):
+    """Each max term of lower_bound_time on its own, then combined: a dropped
+    term or a speed read from the wrong field fails."""
+    def lbt(d, a, b):
+        return lib.oracle_lower_bound_time(C.byref(d), O.dptr(arr(a)), O.dptr(arr(b)))
+
+    u2 = _desc_distinct(N.SYS_UNICYCLE2)
+    z = [0.0] * 5
+    assert lbt(u2, z, [3, 4, 0, 0, 0]) == 5.0 / 0.5
+    assert lbt(u2, z, [0, 0, 0.5, 0, 0]) == 0.5 / 0.25
+    assert lbt(u2, z, [0, 0, 0, 0.5, 0]) == 0.5 / 0.125
+    assert lbt(u2, z, [0, 0, 0, 0, 0.75]) == 0.75 / 2.0
+    assert lbt(u2, z, [0.3, 0.4, 0.25, 0.5, 0.75]) == max(1.0, 1.0, 4.0, 0.375)
+
+    car = _desc_distinct(N.SYS_CAR_TRAILER)
+    z = [0.0] * 4
+    assert lbt(car, z, [3, 4, 0, 0]) == 10.0
+    assert lbt(car, z, [0, 0, 0.5, 0]) == 2.0
+    assert lbt(car, z, [0, 0, 3.5, 0]) == (math.tau - 3.5) / 0.25
+    assert lbt(car, z, [0, 0, 0, 1.0]) == 0.0  # the trailer angle does not enter
+
+    qd = _desc_distinct(N.SYS_QUADROTOR)
+    a = np.zeros(13)
+    a[6] = 1.0
+    b = a.copy()
+    b[:3] = (0.75, 1.0, 0.0)
+    assert lbt(qd, a, b) == 1.25 / 0.5
+    b = a.copy()
+    b[7:10] = (0.0, 0.75, 1.0)
+    assert lbt(qd, a, b) == 1.25 / 0.125
+    b = a.copy()
+    b[10:13] = (5.0, 5.0, 5.0)
+    b[3:7] = (0.0, 0.0, 1.0, 0.0)
+    assert lbt(qd, a, b) == 0.0  # neither attitude nor angular rate enters
+    b = a.copy()
+    b[:3] = (3.0, 4.0, 0.0)
+    b[7:10] = (0.0, 0.3, 0.4)
+    assert lbt(qd, a, b) == max(5.0 / 0.5, 0.5 / 0.125)
+
+
 # ---- collision ---------------------------------------------------------------
 def poly_obb(cx, cy, c, s, hl, hw):
     u, v = np.array([c, s]), np.array([-s, c])
@@ -225,6 +325,44 @@ def test_collision_invariants(lib):
     # the numpy footprint test of the generator agrees away from contact
     ref = synth.planar_footprint_collides(d, boxes, states)
     assert (np.array(base, bool) == ref).mean() > 0.99
+
+
+def test_trailer_box_touch_and_miss(lib):
+    """The trailer box of car_first_order_with_1_trailers sits at
+    p - hitch * (cos th1, sin th1) with half sizes (length, width) along
+    (cos th1, sin th1).  At th1 in {0, pi/2, pi} an obstacle placed to touch
+    its far edge collides, one 2^-20 m further out does not, and the tractor
+    box never reaches it.  A hitch sign slip, swapped half sizes or the
+    tractor heading used for the trailer all fail."""
+    d = synth.default_desc_static(N.SYS_CAR_TRAILER)
+    d.hitch = 0.5
+    d.trailer_half[0], d.trailer_half[1] = 0.25, 0.125
+    d.body_half[0], d.body_half[1] = 0.25, 0.125
+    eps = 2.0 ** -20
+
+    def col(state, box):
+        w = O.world_struct([0, 0, 0], [10, 10, 0], arr([box]), np.zeros((0, 4)))
+        return lib.oracle_state_collides(C.byref(d), C.byref(w), O.dptr(arr(state)))
+
+    # th1 = 0: trailer spans x in [4.25, 4.75]; obstacle of half 0.25 to its left
+    s = [5.0, 5.0, 0.0, 0.0]
+    assert col(s, [4.0, 5.0, 0, 0.25, 0.25, 0]) == 1
+    assert col(s, [4.0 - eps, 5.0, 0, 0.25, 0.25, 0]) == 0
+    # th1 = pi/2: trailer spans y in [4.25, 4.75]; obstacle below it
+    s = [5.0, 5.0, 0.0, math.pi / 2]
+    assert col(s, [5.0, 4.0, 0, 0.25, 0.25, 0]) == 1
+    assert col(s, [5.0, 4.0 - eps, 0, 0.25, 0.25, 0]) == 0
+    # ... and its width (0.125) lies across, along x
+    assert col(s, [5.375, 4.5, 0, 0.25, 0.25, 0]) == 1
+    assert col(s, [5.375 + eps, 4.5, 0, 0.25, 0.25, 0]) == 0
+    # th1 = pi: trailer spans x in [5.25, 5.75]; obstacle to its right
+    s = [5.0, 5.0, 0.0, math.pi]
+    assert col(s, [6.0, 5.0, 0, 0.25, 0.25, 0]) == 1
+    assert col(s, [6.0 + eps, 5.0, 0, 0.25, 0.25, 0]) == 0
+    # the tractor heading does not move the trailer
+    s = [5.0, 5.0, math.pi / 2, 0.0]
+    assert col(s, [4.0, 5.0, 0, 0.25, 0.25, 0]) == 1
+    assert col(s, [4.0 - eps, 5.0, 0, 0.25, 0.25, 0]) == 0
 
 
 def test_sphere_obstacles_brute_force(lib):
