@@ -235,6 +235,8 @@ void dp_destroy(dp_ctx* c) {
   dev_free(&c->d_vis);
   dev_free(&c->d_vis_sorted);
   dev_free(&c->d_vis_cell);
+  dev_free(&c->d_vis_perm);
+  dp_radius_free(c);
   dev_free(&c->d_vis_start);
   dev_free(&c->d_nodes);
   dev_free(&c->d_gpar);
@@ -494,6 +496,7 @@ static int ensure_visited_cap(dp_ctx* c, int64_t need) {
   int rc;
   if ((rc = dev_alloc(&c->d_vis_sorted, (int64_t)nx * cap))) return rc;
   if ((rc = dev_alloc(&c->d_vis_cell, cap))) return rc;
+  if ((rc = dev_alloc(&c->d_vis_perm, cap))) return rc;
   c->nv_cap = cap;
   c->grid_dirty = true;
   return DP_OK;
@@ -767,41 +770,6 @@ int dp_visited_append_novel(dp_ctx* c) {
   rc = dp_launch_visited_append_novel(c, cnt.novel, c->stream);
   if (rc) return rc;
   HIP_CHECK(hipStreamSynchronize(c->stream));
-  return DP_OK;
-}
-
-int dp_visited_radius(dp_ctx* c, const double* queries, int64_t n, double radius, int64_t* out_offsets,
-                      int64_t* out_idx, int64_t idx_cap) {
-  ARG_CHECK(c && (n == 0 || queries) && out_offsets && n >= 0, "invalid argument");
-  HIP_CHECK(hipSetDevice(c->device));
-  HIP_CHECK(hipDeviceSynchronize());
-  const int nx = c->desc.nx;
-  std::vector<double> vis((size_t)nx * c->nv);
-  for (int k = 0; k < nx && c->nv > 0; ++k)
-    HIP_CHECK(hipMemcpy(vis.data() + (size_t)k * c->nv, c->d_vis + (size_t)k * c->nv_cap, sizeof(double) * c->nv,
-                        hipMemcpyDeviceToHost));
-  // The device metric, evaluated on the host (same expressions, same rounding).
-  const dp_dev_sys& S = c->hsys;
-  auto dist = [&](const double* a, int64_t v) {
-    double b[DP_MAX_NX];
-    for (int k = 0; k < nx; ++k) b[k] = vis[(size_t)k * c->nv + v];
-    return dp_distance(S, a, b);
-  };
-
-  int64_t w = 0;
-  out_offsets[0] = 0;
-  for (int64_t q = 0; q < n; ++q) {
-    for (int64_t v = 0; v < c->nv; ++v)
-      if (dist(queries + q * nx, v) <= radius) {
-        if (out_idx && w < idx_cap) out_idx[w] = v;
-        ++w;
-      }
-    out_offsets[q + 1] = w;
-  }
-  if (w > idx_cap) {
-    dp_set_error("radius query output capacity exceeded");
-    return DP_ERR_CAPACITY;
-  }
   return DP_OK;
 }
 

@@ -172,6 +172,7 @@ struct dp_ctx {
   double* d_vis = nullptr;         // [nx][nv_cap] append order
   double* d_vis_sorted = nullptr;  // [nx][nv_cap] sorted by cell
   int32_t* d_vis_cell = nullptr;   // scratch [nv_cap]
+  int32_t* d_vis_perm = nullptr;   // [nv_cap] position in d_vis_sorted -> append index
   int32_t* d_vis_start = nullptr;  // [ncells+1]
   int64_t vis_cells_cap = 0;
   bool grid_dirty = true;
@@ -239,6 +240,7 @@ struct dp_ctx {
   int64_t trace_cap = 0;
   dp_tile_config tiles{0, 2048, 4096, 256, 8192, 0.0f};  // launch shapes (dp_autotune)
   int64_t last_pairs = -1;
+  void* radius = nullptr;           // device buffers of dp_visited_radius (k_radius.hip)
 };
 
 #define DP_HIP_RET(expr)                                                        \
@@ -279,6 +281,9 @@ int dp_launch_visited_rebuild(dp_ctx* c, double radius, hipStream_t s);
 int dp_launch_visited_append_novel(dp_ctx* c, int64_t novel_total, hipStream_t s);
 int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStream_t s);
 int dp_launch_more_rounds(dp_ctx* c, int rounds, hipStream_t s);
+
+// ---- k_radius.hip -------------------------------------------------------
+void dp_radius_free(dp_ctx* c);
 
 // ---- k_quad.hip ---------------------------------------------------------
 int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, double r_dup, hipStream_t s);

@@ -89,12 +89,13 @@ __global__ void k_vis_count(dp_grid_geom G, const double* __restrict__ pts, int6
 template <int NX>
 __global__ void k_vis_scatter(const double* __restrict__ pts, int64_t stride, int64_t n,
                               const int32_t* __restrict__ cellof, const int32_t* __restrict__ start,
-                              int32_t* __restrict__ cursor, double* __restrict__ out) {
+                              int32_t* __restrict__ cursor, double* __restrict__ out, int32_t* __restrict__ perm) {
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
     const int32_t c = cellof[p];
     const int32_t pos = start[c] + atomicAdd(&cursor[c], 1);
 #pragma unroll
     for (int k = 0; k < NX; ++k) out[(int64_t)k * stride + pos] = pts[(int64_t)k * stride + p];
+    perm[pos] = (int32_t)p;
   }
 }
 
@@ -1064,7 +1065,7 @@ int dp_launch_visited_rebuild(dp_ctx* c, double radius, hipStream_t s) {
       dp_fill_i32(c->d_cont_cursor, nc, 0, s);
       k_vis_scatter<NX><<<dp_grid_blocks(c->nv, 256, 8192), 256, 0, s>>>(c->d_vis, c->nv_cap, c->nv, c->d_vis_cell,
                                                                         c->d_vis_start, c->d_cont_cursor,
-                                                                        c->d_vis_sorted);
+                                                                        c->d_vis_sorted, c->d_vis_perm);
       return DP_OK;
     });
     if (rc) return rc;

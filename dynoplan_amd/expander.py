@@ -90,6 +90,24 @@ class GpuExpander:
     def visited_size(self):
         return int(self.lib.dp_visited_size(self.ctx))
 
+    def visited_radius(self, queries, radius, idx_cap=None):
+        """T_n->nearestR for each query: (offsets[n+1], indices) in CSR form,
+        each list ascending (append order of the visited set)."""
+        q = np.ascontiguousarray(queries, np.float64).reshape(-1, self.nx)
+        off = np.zeros(q.shape[0] + 1, np.int64)
+        if idx_cap is None:
+            # counts first: with no room for indices the call fills the offsets
+            rc = self.lib.dp_visited_radius(self.ctx, _dptr(q), q.shape[0], float(radius),
+                                            off.ctypes.data_as(C.POINTER(C.c_int64)), None, 0)
+            if rc not in (N.DP_OK, N.DP_ERR_CAPACITY):
+                N.check(rc)
+            idx_cap = int(off[-1])
+        idx = np.zeros(max(idx_cap, 1), np.int64)
+        N.check(self.lib.dp_visited_radius(self.ctx, _dptr(q), q.shape[0], float(radius),
+                                           off.ctypes.data_as(C.POINTER(C.c_int64)),
+                                           idx.ctypes.data_as(C.POINTER(C.c_int64)), idx_cap))
+        return off, idx[: off[-1]]
+
     def reserve(self, max_nodes, max_pairs):
         N.check(self.lib.dp_reserve(self.ctx, int(max_nodes), int(max_pairs)))
 

@@ -181,9 +181,13 @@ int dp_visited_append(dp_ctx* ctx, const double* states /*[n][nx]*/, int64_t n);
 int dp_visited_append_novel(dp_ctx* ctx);
 int64_t dp_visited_size(const dp_ctx* ctx);
 
-/* Radius query over the visited set (T_n->nearestR): for each query state,
- * the indices of visited states with distance <= radius, CSR output.
- * out_offsets has n+1 entries; out_idx capacity idx_cap.  Used for rewiring. */
+/* Radius query over the visited set (T_n->nearestR, nigh_custom_spaces.hpp:236-247,
+ * the rewiring query of dbastar.cpp:883-912), on the device grid: for each
+ * query state, the indices (append order) of visited states with distance
+ * <= radius, ascending, CSR output.  out_offsets has n+1 entries (always
+ * complete); out_idx receives the first min(total, idx_cap) entries and the
+ * call returns DP_ERR_CAPACITY when total > idx_cap.  Host buffers;
+ * synchronous; device buffers grow on demand. */
 int dp_visited_radius(dp_ctx* ctx, const double* queries /*[n][nx]*/, int64_t n, double radius,
                       int64_t* out_offsets, int64_t* out_idx, int64_t idx_cap);
 

@@ -534,6 +534,26 @@ static void swept_candidates(const dp_system_desc* sys, const dp_world* world, c
   *ncs = ns;
 }
 
+/* T_n->nearestR(x, radius) (nigh_custom_spaces.hpp:236-247), the rewiring
+ * query of dbastar.cpp:883-912, by brute force: for each query, every
+ * visited index v with distance <= radius, ascending (CSR).  Returns the
+ * total number of hits; out_idx receives at most idx_cap of them. */
+int64_t oracle_radius(const dp_system_desc* sys, const double* visited, int64_t nv, const double* queries,
+                      int64_t n, double radius, int64_t* out_offsets, int64_t* out_idx, int64_t idx_cap) {
+  const int nx = sys->nx;
+  int64_t w = 0;
+  out_offsets[0] = 0;
+  for (int64_t q = 0; q < n; ++q) {
+    for (int64_t v = 0; v < nv; ++v)
+      if (oracle_distance(sys, queries + q * nx, visited + v * nx) <= radius) {
+        if (out_idx && w < idx_cap) out_idx[w] = v;
+        ++w;
+      }
+    out_offsets[q + 1] = w;
+  }
+  return w;
+}
+
 int64_t oracle_expand(const dp_system_desc* sys, const dp_prims* prims, int32_t max_motions,
                       const dp_world* world, const double* visited, int64_t nv,
                       const double* nodes, const double* g_parent, int32_t N,

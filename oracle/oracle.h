@@ -52,6 +52,10 @@ int64_t oracle_expand_mt(const dp_system_desc* sys, const dp_prims* prims, int32
                          const dp_expand_params* par, int32_t nn_mode, int32_t col_mode,
                          dp_pairs_out* out, dp_counts* counts, int32_t nthreads);
 
+/* Radius query over the visited set (brute force, ascending CSR). */
+int64_t oracle_radius(const dp_system_desc* sys, const double* visited, int64_t nv, const double* queries,
+                      int64_t n, double radius, int64_t* out_offsets, int64_t* out_idx, int64_t idx_cap);
+
 /* Building blocks, exported for the pinning tests. */
 double oracle_distance(const dp_system_desc* sys, const double* a, const double* b);
 double oracle_lower_bound_time(const dp_system_desc* sys, const double* a, const double* b);

@@ -35,6 +35,9 @@ def load():
                                   C.c_int32, P(N.PairsOut), P(N.Counts)]
     lib.oracle_expand_mt.restype = C.c_int64
     lib.oracle_expand_mt.argtypes = lib.oracle_expand.argtypes + [C.c_int32]
+    lib.oracle_radius.restype = C.c_int64
+    lib.oracle_radius.argtypes = [P(N.SystemDesc), P(C.c_double), C.c_int64, P(C.c_double), C.c_int64, C.c_double,
+                                  P(C.c_int64), P(C.c_int64), C.c_int64]
     lib.oracle_distance.restype = C.c_double
     lib.oracle_distance.argtypes = [P(N.SystemDesc), P(C.c_double), P(C.c_double)]
     lib.oracle_lower_bound_time.restype = C.c_double
@@ -127,3 +130,17 @@ def expand(wl, nodes=None, g_parent=None, visited=None, params=None, nn_mode=NN_
     if n < 0:
         raise RuntimeError(f"oracle_expand failed: {n}")
     return {k: v[:n] for k, v in out.items()}, cnt.as_dict()
+
+
+def radius(desc, visited, queries, r):
+    """Brute-force T_n->nearestR of every query: (offsets, indices), ascending."""
+    lib = load()
+    visited = np.ascontiguousarray(visited, np.float64).reshape(-1, desc.nx)
+    queries = np.ascontiguousarray(queries, np.float64).reshape(-1, desc.nx)
+    off = np.zeros(queries.shape[0] + 1, np.int64)
+    total = lib.oracle_radius(C.byref(desc), dptr(visited), visited.shape[0], dptr(queries), queries.shape[0],
+                              float(r), off.ctypes.data_as(C.POINTER(C.c_int64)), None, 0)
+    idx = np.zeros(max(total, 1), np.int64)
+    lib.oracle_radius(C.byref(desc), dptr(visited), visited.shape[0], dptr(queries), queries.shape[0], float(r),
+                      off.ctypes.data_as(C.POINTER(C.c_int64)), idx.ctypes.data_as(C.POINTER(C.c_int64)), total)
+    return off, idx[:total]

@@ -552,3 +552,29 @@ def test_oracle_sequential_duplicates_restated(lib, limit):
         else:
             assert out["resolution"][q] == N.R_DUP_BATCH
     assert c["novel"] == len(tree) - n_vis
+
+
+def test_oracle_radius_query_closed_form(lib):
+    """oracle_radius (T_n->nearestR by brute force) against the unicycle1
+    metric written out in numpy: |dp| + w1 * |wrap(dth)|, radius inclusive,
+    ascending indices; radius 0 finds exact copies only."""
+    wl = small(N.SYS_UNICYCLE1, seed=12)
+    rng = np.random.default_rng(5)
+    vis = wl.visited[:3000]
+    q = np.concatenate([vis[rng.integers(0, vis.shape[0], 40)],            # exact copies
+                        vis[rng.integers(0, vis.shape[0], 40)] + rng.normal(0, 0.1, (40, 3)),
+                        rng.uniform([0, 0, -np.pi], [10, 10, np.pi], (40, 3))])
+    q[40:80, 2] = np.mod(q[40:80, 2] + np.pi, 2 * np.pi) - np.pi  # headings in [-pi, pi), as states are
+    w = wl.desc.dist_w
+    for r in (0.0, 0.15, 0.6):
+        off, idx = O.radius(wl.desc, vis, q, r)
+        for i in range(q.shape[0]):
+            dth = np.abs(np.mod(q[i, 2] - vis[:, 2] + np.pi, 2 * np.pi) - np.pi)
+            d = w[0] * np.hypot(q[i, 0] - vis[:, 0], q[i, 1] - vis[:, 1]) + w[1] * dth
+            got = idx[off[i]:off[i + 1]]
+            assert np.all(np.diff(got) > 0)
+            sure_in = np.flatnonzero(d <= r - 1e-12)
+            sure_out = np.flatnonzero(d > r + 1e-12)
+            assert set(sure_in) <= set(got) and not (set(sure_out) & set(got)), (r, i)
+        if r == 0.0:
+            assert np.all(off[1:41] - off[:40] >= 1)
