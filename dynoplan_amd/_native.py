@@ -22,6 +22,7 @@ DP_ERR_IO = -5
 DP_ERR_FORMAT = -6
 DP_ERR_NOMEM = -7
 DP_ERR_NOTCONV = -8
+DP_NOALLOC_MAX_QUERIES = 65536
 
 SYS_UNICYCLE1 = 0
 SYS_UNICYCLE2 = 1
@@ -186,6 +187,8 @@ SIGNATURES = {
     "dp_visited_size": (C.c_int64, [C.c_void_p]),
     "dp_visited_radius": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_int64, C.c_double,
                                     C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int64]),
+    "dp_visited_radius_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_double, C.c_void_p, C.c_void_p,
+                                            C.c_int64, C.c_void_p]),
     "dp_reserve": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
     "dp_upload_nodes": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32]),
     "dp_expand": (C.c_int, [C.c_void_p, C.POINTER(ExpandParams), C.c_void_p]),

@@ -205,6 +205,7 @@ dp_ctx* dp_create(const dp_system_desc* sys, int32_t device) {
   ok = ok && hipMemset(c->d_total, 0, sizeof(int32_t)) == hipSuccess;
   ok = ok && upload_sys(c) == DP_OK;
   ok = ok && hipHostMalloc((void**)&c->h_pinned, sizeof(long long) * 4) == hipSuccess;
+  ok = ok && dp_noalloc_init_device(device) == DP_OK;
   if (!ok) {
     if (g_err.empty()) dp_set_error("context creation failed");
     dp_destroy(c);

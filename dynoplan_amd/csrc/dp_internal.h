@@ -283,6 +283,7 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
 int dp_launch_more_rounds(dp_ctx* c, int rounds, hipStream_t s);
 
 // ---- k_radius.hip -------------------------------------------------------
+int dp_noalloc_init_device(int device);
 void dp_radius_free(dp_ctx* c);
 
 // ---- k_quad.hip ---------------------------------------------------------

@@ -12,12 +12,15 @@
 //   k_radius<NX, true>   the same scan again, hits written at their offsets
 //   k_radius_sort        one workgroup per query: its list sorted in LDS
 //
-// Entry point: dp_visited_radius (host buffers; device buffers grown on
-// demand).
+// Two entry points share the kernels: dp_visited_radius (host buffers;
+// device buffers grown on demand) and dp_visited_radius_noalloc (device
+// buffers; no allocation: its per-query counts and status words live in the
+// file-scope array `scratch` below, one copy per device).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 
 #include "dp_internal.h"
@@ -271,6 +274,39 @@ int grow(T** p, int64_t* cap, int64_t need) {
 
 }  // namespace
 
+// ---- the scratch of the allocation-free entry point -------------------------
+// Per-query hit counts (int32, DP_NOALLOC_MAX_QUERIES) followed by two status
+// words (total hits, longest list).  One copy per device, for the life of
+// the process; every call on a device reuses it.
+#define DP_NOALLOC_COUNTS_BYTES ((size_t)DP_NOALLOC_MAX_QUERIES * sizeof(int32_t))
+#define DP_NOALLOC_SCRATCH_BYTES (DP_NOALLOC_COUNTS_BYTES + 256)
+static __device__ __align__(256) unsigned char scratch[DP_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` per device
+}  // namespace
+
+// Called by dp_create with `device` current: resolves the device address of
+// `scratch` once per device.
+int dp_noalloc_init_device(int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    dp_set_error("device index beyond the scratch table");
+    return DP_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return DP_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    dp_set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    return DP_ERR_HIP;
+  }
+  g_scratch_ptr[device] = ptr;
+  return DP_OK;
+}
+
 void dp_radius_free(dp_ctx* c) {
   if (!c->radius) return;
   radius_bufs* b = (radius_bufs*)c->radius;
@@ -342,6 +378,59 @@ int dp_visited_radius(dp_ctx* c, const double* queries, int64_t n, double radius
     }
   if (st[0] > idx_cap) {
     dp_set_error("radius query output capacity exceeded");
+    return DP_ERR_CAPACITY;
+  }
+  return DP_OK;
+}
+
+int dp_visited_radius_noalloc(dp_ctx* c, const double* d_queries, int64_t n, double radius, int64_t* d_offsets,
+                              int64_t* d_idx, int64_t idx_cap, void* stream) {
+  if (!c || (n > 0 && !d_queries) || !d_offsets || n < 0 || idx_cap < 0 || (idx_cap > 0 && !d_idx)) {
+    dp_set_error("invalid argument");
+    return DP_ERR_ARG;
+  }
+  if (n > DP_NOALLOC_MAX_QUERIES) {
+    dp_set_error("dp_visited_radius_noalloc takes at most " + std::to_string(DP_NOALLOC_MAX_QUERIES) +
+                 " queries per call");
+    return DP_ERR_ARG;
+  }
+  if (!(radius >= 0.0) || !std::isfinite(radius)) {
+    dp_set_error("radius must be finite and >= 0");
+    return DP_ERR_ARG;
+  }
+  if (c->nv > 0 && (c->grid_dirty || !(c->grid_radius >= radius))) {
+    dp_set_error("the visited grid is not built for this radius: call dp_visited_radius or dp_expand once "
+                 "after the last change of the visited set");
+    return DP_ERR_STATE;
+  }
+  if (c->device < 0 || c->device >= kMaxDevices || !g_scratch_ptr[c->device]) {
+    dp_set_error("scratch address of the device not resolved");
+    return DP_ERR_STATE;
+  }
+  DP_HIP_RET(hipSetDevice(c->device));
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  unsigned char* base = (unsigned char*)g_scratch_ptr[c->device];
+  int32_t* counts = (int32_t*)base;
+  long long* status = (long long*)(base + DP_NOALLOC_COUNTS_BYTES);
+  if (n == 0 || c->nv == 0) {
+    // every list is empty: offsets of zero counts
+    DP_HIP_RET(hipMemsetAsync(d_offsets, 0, sizeof(int64_t) * (n + 1), s));
+    DP_HIP_RET(hipStreamSynchronize(s));
+    return DP_OK;
+  }
+  int rc = launch_radius(c, d_queries, n, radius, counts, d_offsets, nullptr, 0, status, false, s);
+  if (rc) return rc;
+  rc = launch_radius(c, d_queries, n, radius, counts, d_offsets, d_idx, idx_cap, status, true, s);
+  if (rc) return rc;
+  DP_HIP_RET(hipMemcpyAsync(c->h_pinned, status, sizeof(long long) * 2, hipMemcpyDeviceToHost, s));
+  DP_HIP_RET(hipStreamSynchronize(s));
+  const long long total = c->h_pinned[0], longest = c->h_pinned[1];
+  if (total > idx_cap) {
+    dp_set_error("radius query output capacity exceeded (entries beyond idx_cap not written)");
+    return DP_ERR_CAPACITY;
+  }
+  if (longest > DP_RADIUS_SORT_MAX) {
+    dp_set_error("a query has more than 4096 visited states within the radius; its list is not sorted");
     return DP_ERR_CAPACITY;
   }
   return DP_OK;

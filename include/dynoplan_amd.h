@@ -191,6 +191,23 @@ int64_t dp_visited_size(const dp_ctx* ctx);
 int dp_visited_radius(dp_ctx* ctx, const double* queries /*[n][nx]*/, int64_t n, double radius,
                       int64_t* out_offsets, int64_t* out_idx, int64_t idx_cap);
 
+/* dp_visited_radius without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * that must not call hipMalloc, tight serving loops).  Queries, offsets and
+ * indices are DEVICE buffers of the caller ([n][nx] doubles, n+1 and idx_cap
+ * int64).  Its per-query counts live in a fixed per-device scratch array
+ * sized for DP_NOALLOC_MAX_QUERIES queries; larger n is rejected
+ * (DP_ERR_ARG).  The visited grid must already be built for a radius >=
+ * `radius` (by dp_visited_radius or dp_expand after the last change of the
+ * visited set), else DP_ERR_STATE.  Work is enqueued on `stream` (NULL = the
+ * context stream) and the call waits for it.  DP_ERR_CAPACITY when the
+ * total exceeds idx_cap (entries beyond it are not written) or a query has
+ * more than 4096 neighbours (its list is then not sorted).  All calls on one
+ * device share the scratch array: calls that overlap on one device race. */
+#define DP_NOALLOC_MAX_QUERIES 65536
+int dp_visited_radius_noalloc(dp_ctx* ctx, const double* d_queries, int64_t n, double radius,
+                              int64_t* d_offsets, int64_t* d_idx, int64_t idx_cap, void* stream);
+
 /* Pre-size device buffers (optional; dp_expand grows them on demand). */
 int dp_reserve(dp_ctx* ctx, int32_t max_nodes, int64_t max_pairs);
 
