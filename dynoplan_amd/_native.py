@@ -83,7 +83,14 @@ class Counts(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
         "nodes", "pairs", "bounds", "collide", "free_", "surrogate_reject", "novel",
         "dup_visited", "dup_batch", "capped", "goal_hits", "checked", "recheck_fp64",
-        "resolve_rounds")]
+        "resolve_rounds", "conf_pool_used", "fp64_deferred")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+class HealthInfo(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("batches", "wave_giveups", "window_overflows", "host_completions")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -195,9 +202,13 @@ SIGNATURES = {
     "dp_expand_batch": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32,
                                   C.POINTER(ExpandParams), C.c_void_p]),
     "dp_get_counts": (C.c_int, [C.c_void_p, C.POINTER(Counts)]),
+    "dp_health": (C.c_int, [C.c_void_p, C.POINTER(HealthInfo)]),
+    "dp_debug_set_limits": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_int32]),
     "dp_fetch_pairs": (C.c_int, [C.c_void_p, C.POINTER(PairsOut)]),
     "dp_stream": (C.c_void_p, [C.c_void_p]),
     "dp_last_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "dp_last_timing_detail": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "dp_debug_fp32_states": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_double)]),
     "dp_surrogate_get": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "dp_search_options_default": (None, [C.POINTER(SearchOptions)]),
     "dp_search": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(SearchOptions),

@@ -197,9 +197,11 @@ dp_ctx* dp_create(const dp_system_desc* sys, int32_t device) {
   c->desc = *sys;
   c->nbox = sys->system == DP_SYS_CAR_TRAILER ? 2 : 1;
   bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
-  for (int k = 0; k < 5 && ok; ++k) ok = hipEventCreate(&c->ev[k]) == hipSuccess;
+  for (int k = 0; k < DP_EVENTS && ok; ++k) ok = hipEventCreate(&c->ev[k]) == hipSuccess;
   ok = ok && dev_alloc(&c->d_sys, 1) == DP_OK;
   ok = ok && dev_alloc(&c->d_counters, DPC_COUNT) == DP_OK;
+  ok = ok && dev_alloc(&c->d_sticky, DPS_COUNT) == DP_OK;
+  ok = ok && hipMemset(c->d_sticky, 0, sizeof(long long) * DPS_COUNT) == hipSuccess;
   ok = ok && dev_alloc(&c->d_total, 1) == DP_OK;
   ok = ok && hipMemset(c->d_counters, 0, sizeof(long long) * DPC_COUNT) == hipSuccess;
   ok = ok && hipMemset(c->d_total, 0, sizeof(int32_t)) == hipSuccess;
@@ -269,8 +271,9 @@ void dp_destroy(dp_ctx* c) {
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
   c->h_pinned = nullptr;
   dev_free(&c->d_counters);
+  dev_free(&c->d_sticky);
   dev_free(&c->d_total);
-  for (int k = 0; k < 5; ++k)
+  for (int k = 0; k < DP_EVENTS; ++k)
     if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -656,6 +659,7 @@ int dp_expand(dp_ctx* c, const dp_expand_params* p, void* stream) {
   if (rc) return rc;
   rc = dp_launch_expand(c, p, s);
   c->last_pairs = -1;
+  if (!rc) ++c->n_batches;
   return rc;
 }
 
@@ -681,7 +685,22 @@ int dp_get_counts(dp_ctx* c, dp_counts* o) {
   long long h[DPC_COUNT];
   int rc = read_counters(c, h);
   if (rc) return rc;
+  // A quadrotor node whose cap did not fall inside its FP64 window: roll out
+  // the deferred pairs in FP64 and resolve the batch again (no pair is
+  // deferred the second time).
+  const long long deferred = h[DPC_DEFERRED];
+  if (h[DPC_WINDOW_OVF] > 0 && c->N > 0) {
+    ++c->host_completions;
+    if ((rc = dp_launch_quad_deferred(c, c->stream))) return rc;
+    if ((rc = dp_launch_resolve_again(c, c->stream))) return rc;
+    if ((rc = read_counters(c, h))) return rc;
+    if (h[DPC_WINDOW_OVF] > 0) {
+      dp_set_error("FP64 window overflow after the deferred pairs were rolled out");
+      return DP_ERR_NOTCONV;
+    }
+  }
   // Resolution rounds run ahead blind; finish any node still undecided.
+  if (h[DPC_NODES_LEFT] > 0 && c->N > 0) ++c->host_completions;
   for (int extra = 0; h[DPC_NODES_LEFT] > 0 && c->N > 0; ++extra) {
     if (extra > 1000) {
       dp_set_error("duplicate resolution did not converge");
@@ -707,6 +726,8 @@ int dp_get_counts(dp_ctx* c, dp_counts* o) {
   o->checked = h[DPC_CHECKED];
   o->recheck_fp64 = h[DPC_RECHECK];
   o->resolve_rounds = h[DPC_ROUNDS];
+  o->conf_pool_used = h[DPC_CONF];
+  o->fp64_deferred = deferred;
   if (h[DPC_OVERFLOW]) {
     dp_set_error("pair capacity exceeded");
     return DP_ERR_CAPACITY;
@@ -715,6 +736,31 @@ int dp_get_counts(dp_ctx* c, dp_counts* o) {
     dp_set_error("device error flag set");
     return DP_ERR_NOTCONV;
   }
+  return DP_OK;
+}
+
+int dp_health(dp_ctx* c, dp_health_info* o) {
+  ARG_CHECK(c && o, "null argument");
+  HIP_CHECK(hipSetDevice(c->device));
+  HIP_CHECK(hipDeviceSynchronize());
+  long long st[DPS_COUNT];
+  HIP_CHECK(hipMemcpy(st, c->d_sticky, sizeof(st), hipMemcpyDeviceToHost));
+  std::memset(o, 0, sizeof(*o));
+  o->batches = c->n_batches;
+  o->wave_giveups = st[DPS_GIVEUP];
+  o->window_overflows = st[DPS_WINDOW];
+  o->host_completions = c->host_completions;
+  return DP_OK;
+}
+
+int dp_debug_set_limits(dp_ctx* c, int32_t conf_slots, int64_t pool_entries, int32_t fp64_window) {
+  ARG_CHECK(c, "null context");
+  ARG_CHECK(conf_slots >= 0 && conf_slots <= DP_CONF_SLOTS, "conf_slots must be in [0, 64] (0: 64)");
+  ARG_CHECK(pool_entries >= 0 && fp64_window >= 0, "negative limit");
+  c->conf_slots = conf_slots > 0 ? conf_slots : DP_CONF_SLOTS;
+  c->pool_limit = pool_entries;
+  c->fp64_window = fp64_window;
+  dp_bump_epoch();
   return DP_OK;
 }
 
@@ -734,9 +780,17 @@ int dp_fetch_pairs(dp_ctx* c, dp_pairs_out* o) {
   if (o->verdict) HIP_CHECK(hipMemcpy(o->verdict, c->d_verdict, n, hipMemcpyDeviceToHost));
   if (o->first_k) HIP_CHECK(hipMemcpy(o->first_k, c->d_firstk, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
   if (o->goal_index) HIP_CHECK(hipMemcpy(o->goal_index, c->d_goal, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-  if (o->resolution) HIP_CHECK(hipMemcpy(o->resolution, c->d_res, n, hipMemcpyDeviceToHost));
-  std::vector<int8_t> vd((size_t)n);
+  std::vector<int8_t> vd((size_t)n), rs((size_t)n);
   HIP_CHECK(hipMemcpy(vd.data(), c->d_verdict, n, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(rs.data(), c->d_res, n, hipMemcpyDeviceToHost));
+  if (o->resolution) std::memcpy(o->resolution, rs.data(), n);
+  // A CAPPED pair comes after its node's loop ended (dbastar.cpp:916-918):
+  // the reference computes no goal index, final state, g or h for it.
+  for (int64_t q = 0; q < n; ++q)
+    if (rs[q] == DP_R_CAPPED) vd[q] = -1;  // masked below like a non-FREE pair
+  if (o->goal_index)
+    for (int64_t q = 0; q < n; ++q)
+      if (rs[q] == DP_R_CAPPED) o->goal_index[q] = -1;
   const double nan = NAN;
   if (o->final_state) {
     const int nx = c->desc.nx;
@@ -800,6 +854,18 @@ int dp_last_timing(dp_ctx* c, float* ms5) {
   HIP_CHECK(hipEventElapsedTime(&ms5[2], c->ev[2], c->ev[3]));
   HIP_CHECK(hipEventElapsedTime(&ms5[3], c->ev[0], c->ev[3]));
   HIP_CHECK(hipEventElapsedTime(&ms5[4], c->ev[4], c->ev[1]));
+  return DP_OK;
+}
+
+int dp_last_timing_detail(dp_ctx* c, float* ms7) {
+  ARG_CHECK(c && ms7, "null argument");
+  if (!c->timing_valid) {
+    dp_set_error("no expansion has been timed");
+    return DP_ERR_STATE;
+  }
+  HIP_CHECK(hipEventSynchronize(c->ev[3]));
+  const int from[7] = {0, 4, 1, 5, 6, 2, 7}, to[7] = {4, 1, 5, 6, 2, 7, 3};
+  for (int k = 0; k < 7; ++k) HIP_CHECK(hipEventElapsedTime(&ms7[k], c->ev[from[k]], c->ev[to[k]]));
   return DP_OK;
 }
 

@@ -17,6 +17,16 @@
 // Pair flag set by the pre-resolution: every earlier contender is decided
 // and not NOVEL, so the pair is NOVEL unless its node's cap falls before it.
 #define DP_F_NUC 0x10
+// Pair flag (quadrotor): a FREE pair past its node's FP64 window; its FP64
+// rollout, final state, goal index, g and h are computed only if the node's
+// cap does not fall before it (the batch is then resolved again).
+#define DP_F_DEFER 0x20
+// Pair flag (limit_branching_factor == 0): a FREE pair after its node's
+// first FREE pair, CAPPED before the walk.
+#define DP_F_PRECAP 0x40
+// FP64 window of a quadrotor node, in pairs that may be FREE: enough for the
+// cap of limit NOVEL pairs with a wide margin for duplicates.
+#define DP_FP64_WINDOW(limit) (3 * (limit) + 32)
 // Pad added to swept boxes of the broad phase (m): bounds the rounding of
 // "canonical sweep + offset" versus per-state "state + offset" by 1e6x.
 #define DP_SWEEP_PAD 1e-9
@@ -30,6 +40,8 @@
 // Resolution trace, per node: start and end clock of its wave, cycles
 // waiting, evaluating lists and walking chunks, chunks, waits, polls.
 #define DP_TRACE_FIELDS 8
+// HIP events of one expansion (stage and kernel timing)
+#define DP_EVENTS 8
 // LDS per workgroup on gfx950
 #define DP_LDS_MAX 163840
 #define DP_SPIN_LIMIT_WAITING (1 << 10)
@@ -45,8 +57,11 @@ void dp_set_error(const std::string& msg);
 enum {
   DPC_PAIRS = 0, DPC_BOUNDS, DPC_COLLIDE, DPC_FREE, DPC_SURR, DPC_NOVEL, DPC_DUPV, DPC_DUPB,
   DPC_CAPPED, DPC_GOAL, DPC_CHECKED, DPC_RECHECK, DPC_ROUNDS, DPC_ERRFLAG, DPC_OVERFLOW,
-  DPC_NODES_LEFT, DPC_CONF, DPC_TICKET, DPC_CELLS, DPC_COUNT
+  DPC_NODES_LEFT, DPC_CONF, DPC_TICKET, DPC_CELLS, DPC_WINDOW_OVF, DPC_DEFERRED, DPC_COUNT
 };
+// Sticky counters (never reset by a batch): resolutions finished by
+// host-launched rounds after a wave gave up, FP64 window overflows.
+enum { DPS_GIVEUP = 0, DPS_WINDOW = 1, DPS_COUNT = 4 };
 
 // Uniform grid used by the duplicate tests.  Axes 0,1 are the position
 // (x, y); axis 2 is z for 3-D systems and the heading theta (state[2],
@@ -225,14 +240,26 @@ struct dp_ctx {
   int32_t* d_node_nobs = nullptr;   // [N_cap] their count (-1: more than DP_NODE_OBS)
   dp_mlp mlp;
   int32_t round_limit = 0;          // limit_branching_factor of the batch in flight
+  dp_expand_params round_params{};  // parameters of the batch in flight
 
   // counters
   long long* d_counters = nullptr;  // [DPC_COUNT]
+  long long* d_sticky = nullptr;    // [DPS_COUNT] never reset
+  int64_t n_batches = 0;            // dp_expand calls
+  int64_t host_completions = 0;     // batches finished by host-launched work in dp_get_counts
+  // test-only limits (dp_debug_set_limits): inline conflict slots used
+  // (<= DP_CONF_SLOTS), overflow pool capacity, FP64 window (0: defaults)
+  int32_t conf_slots = DP_CONF_SLOTS;
+  int64_t pool_limit = 0;
+  int32_t fp64_window = 0;
   int32_t* d_total = nullptr;       // [1] number of pairs of the batch
   int32_t* d_scan_tmp = nullptr;    // [1024] block partials
 
   // timing
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  // expansion stage events: 0 start, 4 applicability done, 1 pre-filter
+  // done, 5 before the main check kernel, 6 after it, 2 check done, 7 lists
+  // done, 3 end
+  hipEvent_t ev[DP_EVENTS] = {};
   bool timing_valid = false;
   bool no_events = false;           // set while a multi-instance graph is captured
   int32_t poll_cap = 16;            // resolution: longest pause between polls (64-cycle sleeps)
@@ -260,6 +287,14 @@ inline int dp_grid_blocks(int64_t work, int per_block, int max_blocks) {
 }
 
 int dp_ensure_i32(int32_t** p, int64_t* cap, int64_t need);
+
+// Records stage event k of the expansion on s (skipped while a
+// multi-instance graph is captured).
+inline int dp_record_event(dp_ctx* c, int k, hipStream_t s) {
+  if (c->no_events) return DP_OK;
+  DP_HIP_RET(hipEventRecord(c->ev[k], s));
+  return DP_OK;
+}
 int dp_ensure_f64(double** p, int64_t* cap, int64_t need);
 
 // ---- k_util.hip ---------------------------------------------------------
@@ -281,6 +316,9 @@ int dp_launch_visited_rebuild(dp_ctx* c, double radius, hipStream_t s);
 int dp_launch_visited_append_novel(dp_ctx* c, int64_t novel_total, hipStream_t s);
 int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStream_t s);
 int dp_launch_more_rounds(dp_ctx* c, int rounds, hipStream_t s);
+// Resolution from scratch after a window overflow: every FREE pair
+// undecided again, resolution counters cleared, lists and walk rebuilt.
+int dp_launch_resolve_again(dp_ctx* c, hipStream_t s);
 
 // ---- k_radius.hip -------------------------------------------------------
 int dp_noalloc_init_device(int device);
@@ -288,6 +326,11 @@ void dp_radius_free(dp_ctx* c);
 
 // ---- k_quad.hip ---------------------------------------------------------
 int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, double r_dup, hipStream_t s);
+// FP64 rollout of the pairs deferred past their node's FP64 window (after a
+// window overflow), then their visited-set duplicate test.
+int dp_launch_quad_deferred(dp_ctx* c, hipStream_t s);
+// FP64 window of the batch in pairs that may be FREE (INT32_MAX: none).
+int32_t dp_fp64_window(const dp_ctx* c, int32_t limit);
 
 // ---- k_mlp.hip ----------------------------------------------------------
 int dp_mlp_alloc(dp_ctx* c, int32_t P, uint64_t seed);

@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
   int16_t* cand = s_cand + (int64_t)(wave * SEG + seg) * nobs;
   const int32_t total = *a.total;
   const double Tdup = dp_pos_bound(S, a.r_dup);
-  long long c_b = 0, c_c = 0, c_f = 0, c_g = 0, c_k = 0;
+  long long c_b = 0, c_c = 0, c_f = 0, c_k = 0;
 
   for (int64_t q0 = ((int64_t)blockIdx.x * kWaves + wave) * SEG; q0 < total;
        q0 += (int64_t)gridDim.x * kWaves * SEG) {
@@ -349,7 +349,6 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
       c_b += vd == DP_V_BOUNDS;
       c_c += vd == DP_V_COLLIDE;
       c_f += fr;
-      c_g += fr && chosen != -1;
       c_k += vd != DP_V_BOUNDS;
     }
   }
@@ -357,7 +356,6 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
     if (c_b) atomicAdd((unsigned long long*)&a.counters[DPC_BOUNDS], (unsigned long long)c_b);
     if (c_c) atomicAdd((unsigned long long*)&a.counters[DPC_COLLIDE], (unsigned long long)c_c);
     if (c_f) atomicAdd((unsigned long long*)&a.counters[DPC_FREE], (unsigned long long)c_f);
-    if (c_g) atomicAdd((unsigned long long*)&a.counters[DPC_GOAL], (unsigned long long)c_g);
     if (c_k) atomicAdd((unsigned long long*)&a.counters[DPC_CHECKED], (unsigned long long)c_k);
   }
 }
@@ -513,6 +511,7 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
                                                               c->d_node_obs, c->d_node_nobs);
     DP_HIP_RET(hipGetLastError());
   }
+  if ((rc = dp_record_event(c, 5, s))) return rc;
   rc = dispatch_nx(nx, [&](auto NXc) -> int {
     constexpr int NX = decltype(NXc)::value;
     if constexpr (NX == 13) {
@@ -529,6 +528,7 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   });
   if (rc) return rc;
   DP_HIP_RET(hipGetLastError());
+  DP_EV(c->ev[6], s);
   DP_EV(c->ev[2], s);
 
   rc = dp_launch_resolve(c, p, r_dup, s);

@@ -20,27 +20,37 @@
 // what a wave gives up on.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "dp_internal.h"
 
+// Cell sizes: at least r/w0 on the position axes and r/w1 on the heading
+// axis, never narrower than the largest world extent / 2^20 (a radius of 0,
+// or a tiny one, would otherwise give empty cells; coarser cells stay
+// exact).  Counts are formed in double and coarsened until the grid holds at
+// most 2^24 cells.
 dp_grid_geom dp_make_grid(const dp_ctx* c, double r) {
   dp_grid_geom G{};
   const bool planar = c->desc.pos_dim == 2;
+  const int npos = planar ? 2 : 3;
+  double ext_max = 0.0;
+  for (int d = 0; d < npos; ++d) ext_max = std::max(ext_max, c->world_hi[d] - c->world_lo[d]);
+  const double min_cell = ext_max > 0.0 ? ext_max / (double)(1 << 20) : 1.0;
   double cell = (r / c->desc.dist_w[0]) * (1.0 + 1e-9);
+  if (!(cell >= min_cell)) cell = min_cell;
   int32_t nth = 1;
   if (planar && c->desc.dist_w[1] > 0.0) {
-    double bw = (r / c->desc.dist_w[1]) * (1.0 + 1e-6);
-    double k = std::floor(DP_TWO_PI / bw);
+    const double bw = (r / c->desc.dist_w[1]) * (1.0 + 1e-6);
+    const double k = bw > 0.0 ? std::floor(DP_TWO_PI / bw) : 4096.0;
     nth = k >= 3.0 ? (int32_t)std::min(k, 4096.0) : 1;
   }
-  const int64_t max_cells = 1ll << 24;
+  const double max_cells = (double)(1ll << 24);
   for (;;) {
-    int64_t n = 1;
-    for (int d = 0; d < 2 + (planar ? 0 : 1); ++d) {
-      double ext = c->world_hi[d] - c->world_lo[d];
-      int64_t k = (int64_t)(ext / cell) + 3;
-      G.dims[d] = (int32_t)k;
+    double n = 1.0;
+    for (int d = 0; d < npos; ++d) {
+      const double k = std::floor((c->world_hi[d] - c->world_lo[d]) / cell) + 3.0;
+      G.dims[d] = (int32_t)std::min(k, (double)INT32_MAX);
       G.lo[d] = c->world_lo[d] - cell;
       G.inv_cell[d] = 1.0 / cell;
       n *= k;
@@ -53,7 +63,7 @@ dp_grid_geom dp_make_grid(const dp_ctx* c, double r) {
       n *= nth;
     }
     if (n <= max_cells) {
-      G.ncells = n;
+      G.ncells = (int64_t)G.dims[0] * G.dims[1] * G.dims[2];
       break;
     }
     cell *= 1.5;  // coarser cells stay exact, only slower
@@ -101,7 +111,7 @@ __global__ void k_vis_scatter(const double* __restrict__ pts, int64_t stride, in
 
 // ---- contenders --------------------------------------------------------------
 __device__ __forceinline__ bool is_contender(int8_t verdict, int8_t flags) {
-  return verdict == DP_V_FREE && ((flags & 2) || !(flags & 1));
+  return verdict == DP_V_FREE && !(flags & (DP_F_DEFER | DP_F_PRECAP)) && ((flags & 2) || !(flags & 1));
 }
 
 template <int NX>
@@ -162,7 +172,7 @@ __device__ __forceinline__ void scan_earlier(const dp_dev_sys& S, const dp_grid_
   }
 }
 
-// Earlier-conflict lists, stored inline (DP_CONF_SLOTS per pair, pair-major:
+// Earlier-conflict lists, stored inline (up to DP_CONF_SLOTS per pair, pair-major:
 // a list's entries are adjacent for its writer and a chunk's lists for its walk)
 // for contenders: FREE pairs that are neither goal-forced nor duplicates of
 // the visited set (with a window, only those at position < window in their
@@ -221,7 +231,7 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
                                                     const int32_t* __restrict__ cells, double r,
                                                     int16_t* __restrict__ conf_cnt, int32_t* __restrict__ conf,
                                                     int32_t* __restrict__ conf_ov, int32_t* __restrict__ pool,
-                                                    int64_t pool_cap, long long* __restrict__ counters) {
+                                                    int64_t pool_cap, int32_t slots, long long* __restrict__ counters) {
   const dp_dev_sys& S = *Sp;
   const int lane = threadIdx.x % 64;
   const uint64_t ltmask = (1ull << lane) - 1ull;
@@ -372,8 +382,8 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
               }
               const uint64_t mk = __ballot(hit);
               const int pos = cc + __popcll(mk & ltmask);
-              if (hit && !pool_pass && pos < DP_CONF_SLOTS) conf[(int64_t)q * DP_CONF_SLOTS + pos] = o[u];
-              if (hit && pool_pass && pos >= DP_CONF_SLOTS) pool[off + (pos - DP_CONF_SLOTS)] = o[u];
+              if (hit && !pool_pass && pos < slots) conf[(int64_t)q * DP_CONF_SLOTS + pos] = o[u];
+              if (hit && pool_pass && pos >= slots) pool[off + (pos - slots)] = o[u];
               cc += __popcll(mk);
             }
             if (lane == c) cnt_l = cc;
@@ -382,11 +392,10 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
       };
       pass(false, lists);
       // lists longer than the inline slots continue in the overflow pool
-      const bool over = list_l && cnt_l > DP_CONF_SLOTS && cnt_l < DP_CONF_MAX;
+      const bool over = list_l && cnt_l > slots && cnt_l < DP_CONF_MAX;
       if (over) {
-        off_l = (long long)atomicAdd((unsigned long long*)&counters[DPC_CONF],
-                                     (unsigned long long)(cnt_l - DP_CONF_SLOTS));
-        if (off_l + cnt_l - DP_CONF_SLOTS > pool_cap) cnt_l = DP_CONF_MAX;  // pool exhausted: scan when reached
+        off_l = (long long)atomicAdd((unsigned long long*)&counters[DPC_CONF], (unsigned long long)(cnt_l - slots));
+        if (off_l + cnt_l - slots > pool_cap) cnt_l = DP_CONF_MAX;  // pool exhausted: scan when reached
       }
       const uint64_t pooled = __ballot(over && cnt_l < DP_CONF_MAX);
       if (pooled) {
@@ -403,6 +412,34 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
         if (cnt_l == 0) flags[q_l] = (int8_t)(flags[q_l] | DP_F_NUC);
       }
     }
+  }
+}
+
+// limit_branching_factor == 0: the reference breaks out of a node's loop
+// after its first FREE pair whatever that pair's resolution
+// (dbastar.cpp:916-918, size_t >= 0), so every later FREE pair is CAPPED
+// before the contender grid is built.  One wave per node.
+__global__ __launch_bounds__(64) void k_precap_first(int32_t N, const int32_t* __restrict__ app,
+                                                     const int32_t* __restrict__ total,
+                                                     const int8_t* __restrict__ verdict, int8_t* __restrict__ flags,
+                                                     int8_t* __restrict__ res) {
+  const int32_t node = blockIdx.x;
+  if (node >= N) return;
+  const int lane = threadIdx.x;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const int32_t tot = *total;
+  const int32_t b = app[node] < tot ? app[node] : tot;
+  const int32_t e = app[node + 1] < tot ? app[node + 1] : tot;
+  bool seen = false;
+  for (int32_t cb = b; cb < e; cb += 64) {
+    const int32_t p = cb + lane;
+    const bool fr = p < e && verdict[p] == DP_V_FREE;
+    const uint64_t m = __ballot(fr);
+    if (fr && (seen || (m & lt))) {
+      flags[p] = (int8_t)(flags[p] | DP_F_PRECAP);
+      res[p] = DP_R_CAPPED;
+    }
+    seen = seen || m != 0;
   }
 }
 
@@ -496,10 +533,12 @@ struct round_args {
   int32_t* count;
   int32_t* node_novel;
   int32_t limit;
+  int32_t slots;     // inline conflict slots in use (<= DP_CONF_SLOTS)
   int32_t wait;      // polls a lane may spend on one undecided earlier contender
   int32_t poll_cap;  // longest pause between polls, in 64-cycle sleeps
   unsigned long long* trace;  // optional [DP_TRACE_FIELDS * N], fields listed at DP_TRACE_FIELDS
   long long* counters;
+  long long* sticky;  // [DPS_COUNT] never reset
 };
 
 // Pause between two polls of a contender's verdict: grows with the number of
@@ -527,7 +566,8 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
   const int32_t tot = *a.total;
   const int32_t end = a.app[node + 1] < tot ? a.app[node + 1] : tot;
   if (cur >= end) return true;
-  long long n_nov = 0, n_cap = 0, n_dv = 0, n_db = 0;
+  long long n_nov = 0, n_cap = 0, n_dv = 0, n_db = 0, n_goal = 0;
+  bool overflow = false;  // a deferred pair (past the FP64 window) was reached before the cap
   // diagnostics (trace only): cycles evaluating lists, cycles in the
   // in-chunk walk, cycles waiting, chunks, waits, polls
   unsigned long long tr_eval = 0, tr_walk = 0, tr_wait = 0, tr_chunks = 0, tr_waits = 0, tr_polls = 0;
@@ -556,7 +596,10 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
       m_cc = a.conf_cnt[p + 64];
       m_ov = a.conf_ov[p + 64];
     }
-    const bool fr = p < end && vd_p == DP_V_FREE;
+    // FREE pairs capped before the walk (limit 0) are not walked
+    const bool pre = p < end && vd_p == DP_V_FREE && (fl_p & DP_F_PRECAP);
+    const bool fr = p < end && vd_p == DP_V_FREE && !pre;
+    n_cap += __popcll(__ballot(pre));
     if (a.limit > 0 && count >= a.limit) {
       // The reference has left this node's loop: every later FREE pair is
       // capped.  Other nodes may be waiting for any of them, so the whole
@@ -569,7 +612,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
           const int32_t t = t0 + 64 * k + lane;
-          f8[k] = t < end && a.verdict[t] == DP_V_FREE;
+          f8[k] = t < end && a.verdict[t] == DP_V_FREE && !(a.flags[t] & DP_F_PRECAP);
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -584,6 +627,9 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
     const int8_t fl = fr ? fl_p : 0;
     const bool forced = (fl & 2) != 0;
     const bool dupv = (fl & 1) != 0;
+    // past the node's FP64 window: no final state, so it blocks the walk for
+    // good; reaching it before the cap is a window overflow
+    const bool deferred = (fl & DP_F_DEFER) != 0;
     bool ext_dup = false, ext_block = false;
     uint64_t same = 0;
     // undecided earlier-chunk contenders seen by the first evaluation (the
@@ -595,7 +641,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
     uint64_t undec = 0;  // entries (< 64) of a stored list whose contender was undecided
     // entry e of this lane's stored earlier-conflict list
     auto entry = [&](int e) -> int32_t {
-      return e < DP_CONF_SLOTS ? a.conf[(int64_t)p * DP_CONF_SLOTS + e] : a.pool[(int64_t)ov_p + (e - DP_CONF_SLOTS)];
+      return e < a.slots ? a.conf[(int64_t)p * DP_CONF_SLOTS + e] : a.pool[(int64_t)ov_p + (e - a.slots)];
     };
     // Earlier contenders of pair p: those in this chunk go to `same` (the
     // walk below decides them in order); an earlier-chunk contender already
@@ -635,7 +681,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
       });
     };
     const unsigned long long t_eval = a.trace ? (unsigned long long)wall_clock64() : 0ull;
-    const int cnt = fr && !forced && !dupv ? cc_p : 0;
+    const int cnt = fr && !forced && !dupv && !deferred ? cc_p : 0;
     {
       // Stored lists of the chunk's 64 pairs, evaluated by the whole wave:
       // the entries are numbered consecutively over the pairs (exclusive
@@ -652,7 +698,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
       }
       const int total_e = __shfl(incl, 63, 64);
       s_pre[lane] = incl - n_l;
-      s_ov[lane] = n_l > DP_CONF_SLOTS ? ov_p : 0;
+      s_ov[lane] = n_l > a.slots ? ov_p : 0;
       s_same[lane] = 0;
       s_undec[lane] = 0;
       s_flag[lane] = 0;
@@ -673,8 +719,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
             const int e = f - s_pre[lo];  // < DP_CONF_MAX
             oe[k] = (lo << 16) | e;
             const int32_t q = cb + lo;
-            os[k] = e < DP_CONF_SLOTS ? a.conf[(int64_t)q * DP_CONF_SLOTS + e]
-                                      : a.pool[(int64_t)s_ov[lo] + (e - DP_CONF_SLOTS)];
+            os[k] = e < a.slots ? a.conf[(int64_t)q * DP_CONF_SLOTS + e] : a.pool[(int64_t)s_ov[lo] + (e - a.slots)];
           }
         }
 #pragma unroll
@@ -714,7 +759,8 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
     const uint64_t forced_m = __ballot(fr && forced);
     const uint64_t dupv_m = __ballot(fr && dupv && !forced);
     uint64_t dup_m = __ballot(ext_dup);
-    uint64_t block_m = __ballot(ext_block && !ext_dup);
+    const uint64_t def_m = __ballot(fr && deferred);
+    uint64_t block_m = __ballot(ext_block && !ext_dup) | def_m;
     uint64_t novel_m = 0, capped_m = 0, dupb_m = 0, dupvr_m = 0, pub_m = 0;
     int stop_lane = 64;
     // Store the verdicts decided since the last call: other nodes' waves may
@@ -768,6 +814,14 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
       novel_m = nov;
       dupvr_m = dupv_m & below;
       dupb_m = free_m & below & ~nov & ~dupv_m;
+      if (fb < 64 && ((def_m >> fb) & 1ull)) {
+        // Window overflow: the cap does not fall before a pair without FP64
+        // data.  Every later FREE pair of the node is published CAPPED so
+        // that no wave waits on it; the batch is resolved again afterwards.
+        capped_m = free_m & ~below;
+        overflow = true;
+        break;
+      }
       if (fb >= 64) break;
       // Only lane fb's verdict is needed next: wait for its undecided
       // contenders, after publishing what is decided (other nodes' waves
@@ -797,6 +851,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
         } else if (block_m & lane_bit) {
           still = true;  // grid-scanned lane: its wait re-scans
         }
+        still = still || (def_m & lane_bit) != 0;
         dup_m |= __ballot(nov_found);
         const uint64_t nb = __ballot(still);
         if (!(nb & (1ull << fb))) {
@@ -896,6 +951,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
     if (a.trace) tr_walk += (unsigned long long)wall_clock64() - t_walk;
     count += __popcll(novel_m);
     n_nov += __popcll(novel_m);
+    n_goal += __popcll(novel_m & forced_m);  // goal-forced pairs the reference reached
     n_cap += __popcll(capped_m);
     n_dv += __popcll(dupvr_m);
     n_db += __popcll(dupb_m);
@@ -903,6 +959,15 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
       cur = cb + stop_lane;
       stopped = true;
       break;
+    }
+    if (overflow) {
+      // the rest of the node's FREE pairs go out CAPPED, as at the cap
+      count = a.limit > count ? a.limit : count;
+      if (lane == 0) {
+        atomicAdd((unsigned long long*)&a.counters[DPC_WINDOW_OVF], 1ull);
+        if (a.sticky) atomicAdd((unsigned long long*)&a.sticky[DPS_WINDOW], 1ull);
+      }
+      overflow = false;
     }
   }
   if (!stopped) cur = end;
@@ -926,6 +991,7 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
     if (n_cap) atomicAdd((unsigned long long*)&a.counters[DPC_CAPPED], (unsigned long long)n_cap);
     if (n_dv) atomicAdd((unsigned long long*)&a.counters[DPC_DUPV], (unsigned long long)n_dv);
     if (n_db) atomicAdd((unsigned long long*)&a.counters[DPC_DUPB], (unsigned long long)n_db);
+    if (n_goal) atomicAdd((unsigned long long*)&a.counters[DPC_GOAL], (unsigned long long)n_goal);
   }
   return !stopped;
 }
@@ -965,7 +1031,10 @@ __global__ __launch_bounds__(64) void k_resolve_ticketed(round_args a) {
   const int32_t limit = a.wait > 0 ? DP_SPIN_LIMIT_WAITING : DP_SPIN_LIMIT;
   while (!advance_node<NX>(a, node, cur, count)) {
     if (++spins > limit) {
-      if (threadIdx.x == 0) atomicOr((unsigned long long*)&a.counters[DPC_ERRFLAG], 2ull);
+      if (threadIdx.x == 0) {
+        atomicOr((unsigned long long*)&a.counters[DPC_ERRFLAG], 2ull);
+        if (a.sticky) atomicAdd((unsigned long long*)&a.sticky[DPS_GIVEUP], 1ull);
+      }
       break;
     }
     __builtin_amdgcn_s_sleep(2);
@@ -1039,11 +1108,13 @@ static round_args make_round_args(dp_ctx* c) {
   a.cursor = c->d_node_cursor;
   a.count = c->d_node_count;
   a.node_novel = c->d_node_novel;
-  a.limit = c->round_limit;
+  a.limit = c->round_limit > 0 ? c->round_limit : 0;  // 0 (no cap here) also for limit 0: k_precap_first
+  a.slots = c->conf_slots;
   a.wait = 0;
   a.poll_cap = c->poll_cap;
   a.trace = c->d_trace;
   a.counters = c->d_counters;
+  a.sticky = c->d_sticky;
   return a;
 }
 
@@ -1084,14 +1155,16 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
   if (rc) return rc;
   c->round_limit = p->limit_branching_factor;
   c->round_r_dup = r_dup;
-  // Lists are stored for every contender of planar batches.  Quadrotor nodes
-  // reach their cap early (the median cap position is 22 of ~550 pairs) and
-  // its neighbourhoods are large, so only the first 64 pairs of each node get
-  // a list; later pairs scan the grid if the walk reaches them.
-  const int32_t window = p->limit_branching_factor > 0 && c->desc.system == DP_SYS_QUADROTOR ? 64 : 0;
+  c->round_params = *p;
+  // Every contender gets a stored list (window 0).  Quadrotor pairs past
+  // their node's FP64 window are not contenders (k_quad_window).
+  const int32_t window = 0;
   const int pb = dp_grid_blocks(c->pair_cap, 256, 8192);
+  const int64_t pool_cap = c->pool_limit > 0 ? std::min<int64_t>(c->pool_limit, c->conf_pool_cap) : c->conf_pool_cap;
   rc = dispatch_nx(c->desc.nx, [&](auto NXc) -> int {
     constexpr int NX = decltype(NXc)::value;
+    if (p->limit_branching_factor == 0 && N > 0)
+      k_precap_first<<<N, 64, 0, s>>>(N, c->d_app, c->d_total, c->d_verdict, c->d_flags, c->d_res);
     dp_fill_i32(c->d_cont_start, nc + 1, 0, s);
     k_cont_count<NX><<<pb, 256, 0, s>>>(c->grid, c->d_total, c->d_verdict, c->d_flags, c->d_final, c->pair_cap,
                                         c->d_cont_start);
@@ -1104,7 +1177,8 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
     k_cell_list<<<dp_grid_blocks(nc, 4096, 1024), 256, 0, s>>>(nc, c->d_cont_start, c->d_cont_cursor, c->d_counters);
     k_conf_waves<NX><<<(int)std::min<int64_t>((nc + 3) / 4, 8192), 256, 0, s>>>(
         c->d_sys, c->grid, c->d_flags, c->d_pnode, c->d_app, window, c->pair_cap, c->d_cont_start, c->d_cont_idx, c->d_cont_x, c->d_cont_cursor,
-        r_dup, c->d_conf_cnt, c->d_conf, c->d_conf_ov, c->d_conf_pool, c->conf_pool_cap, c->d_counters);
+        r_dup, c->d_conf_cnt, c->d_conf, c->d_conf_ov, c->d_conf_pool, pool_cap, c->conf_slots, c->d_counters);
+    if (dp_record_event(c, 7, s)) return DP_ERR_HIP;
     k_round_init<<<dp_grid_blocks(N, 256, 1024), 256, 0, s>>>(N, c->d_app, c->d_total, c->d_node_cursor,
                                                               c->d_node_count, c->d_node_novel, c->d_counters);
     if (p->limit_branching_factor > 0)
@@ -1118,6 +1192,34 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
   if (rc) return rc;
   DP_HIP_RET(hipGetLastError());
   return DP_OK;
+}
+
+namespace {
+// Resolution state back to "every walked FREE pair undecided" (pairs capped
+// before the walk stay CAPPED), with the resolution counters cleared.
+__global__ void k_reset_resolution(const int32_t* __restrict__ total, const int8_t* __restrict__ verdict,
+                                   int8_t* __restrict__ flags, int8_t* __restrict__ res, long long* __restrict__ counters) {
+  const int32_t tot = *total;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+    if (verdict[q] != DP_V_FREE || (flags[q] & DP_F_PRECAP)) continue;
+    res[q] = DP_R_UNDECIDED;
+    flags[q] = (int8_t)(flags[q] & ~DP_F_NUC);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int slots[] = {DPC_NOVEL, DPC_DUPV, DPC_DUPB, DPC_CAPPED, DPC_GOAL, DPC_NODES_LEFT, DPC_CONF,
+                         DPC_TICKET, DPC_CELLS, DPC_WINDOW_OVF, DPC_ROUNDS};
+    for (int k : slots) counters[k] = 0;
+    counters[DPC_ERRFLAG] &= ~2ll;
+  }
+}
+}  // namespace
+
+int dp_launch_resolve_again(dp_ctx* c, hipStream_t s) {
+  k_reset_resolution<<<dp_grid_blocks(c->pair_cap, 256, 8192), 256, 0, s>>>(c->d_total, c->d_verdict, c->d_flags,
+                                                                           c->d_res, c->d_counters);
+  DP_HIP_RET(hipGetLastError());
+  const dp_expand_params p = c->round_params;
+  return dp_launch_resolve(c, &p, c->round_r_dup, s);
 }
 
 int dp_launch_more_rounds(dp_ctx* c, int rounds, hipStream_t s) {

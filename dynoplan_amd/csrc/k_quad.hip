@@ -5,7 +5,7 @@
 // (check_lazy_trajectory, dbastar.cpp:391-490), then check_goal, g/h and the
 // duplicate test on the FP64 states.
 //
-// Two kernels:
+// Kernels:
 //   k_quad32   one wave per (node, chunk of 64 pairs): FP32 rollout relative
 //              to the node, obstacles of the node's reach box staged in LDS
 //              as FP32 offsets from the node, per-state clearance.  A state
@@ -16,6 +16,9 @@
 //              in the same operation order as the CPU oracle (bit-identical
 //              states), full FP64 re-check for deferred pairs, then goal
 //              check, g, h and the visited-set duplicate test.
+//   k_quad_window  one wave per node, between the two: FREE pairs past the
+//              node's FP64 window are deferred (no FP64 rollout unless the
+//              node's cap does not fall before them).
 // With fp32_rollout = 0, k_quad64 evaluates every pair (exact FP64 mode).
 #include <hip/hip_runtime.h>
 #include <limits.h>
@@ -66,6 +69,7 @@ struct dp_quad_args {
   double goal[DP_MAX_NX];
   double goal_bound, r_dup, cdf;
   int32_t exact;
+  int32_t redo;           // k_quad64: only the pairs deferred past their FP64 window
   long long* counters;
 };
 
@@ -141,12 +145,7 @@ __device__ __forceinline__ void quad_step32(const quad_f32& K, float* s, const f
   s[6] = (float)(nw * inv);
 }
 
-__global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
-  extern __shared__ float qsm[];
-  const dp_dev_sys& S = *a.sys;
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int per_wave = 6 * a.nb + 4 * a.ns;
-  float* cb = qsm + wave * per_wave;  // compacted boxes (6 floats) then spheres (4 floats)
+__device__ __forceinline__ quad_f32 make_quad_f32(const dp_dev_sys& S) {
   quad_f32 K;
   K.g = (float)S.quad[1];
   K.g4 = (float)(S.quad[1] * 0.25);
@@ -162,8 +161,32 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
     K.cosc[n] = (float)S.cos_c[n];
     K.sincc[n] = (float)S.sinc_c[n];
   }
+  return K;
+}
+
+// FP32 start state of pair (node, primitive j), relative to the node:
+// stored p0 and v0 plus the node velocity (the position / velocity offset).
+__device__ __forceinline__ void quad32_start(const dp_quad_args& a, int32_t s0, const double* nvel, float* s) {
+#pragma unroll
+  for (int c = 0; c < 13; ++c) {
+    const double v = a.st[(int64_t)c * a.S + s0];
+    s[c] = (c >= 7 && c <= 9) ? (float)(v + nvel[c - 7]) : (float)v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
+  extern __shared__ float qsm[];
+  const dp_dev_sys& S = *a.sys;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int per_wave = 6 * a.nb + 4 * a.ns;
+  float* cb = qsm + wave * per_wave;  // compacted boxes (6 floats) then spheres (4 floats)
+  const quad_f32 K = make_quad_f32(S);
   const float r = (float)S.radius;
-  const float eps = S.near_eps;
+  // margins under which a verdict is re-checked in FP64: near_eps for
+  // positions (clearance and position bounds), 10x for velocities and 100x
+  // for angular rates, whose FP32 drift is larger (measured on primitives
+  // of 10-50 steps: 1e-6 m, 9e-6 m/s, 2.4e-5 rad/s at most)
+  const float eps = S.near_eps, eps_v = 10.0f * S.near_eps, eps_w = 100.0f * S.near_eps;
   const float r_lo2 = (r - eps) * (r - eps), r_hi2 = (r + eps) * (r + eps), r2 = r * r;
   const int32_t total_chunks = a.chunk_off[a.N];
   long long c_b = 0, c_c = 0, c_k = 0;
@@ -251,12 +274,7 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
       const int32_t L1 = a.off[j + 1] - s0;
       const int32_t a0 = s0 - j;
       float s[13];
-      // start: relative position = stored p0; velocity = stored v0 + node velocity
-#pragma unroll
-      for (int c = 0; c < 13; ++c) {
-        const double v = a.st[(int64_t)c * a.S + s0];
-        s[c] = (c >= 7 && c <= 9) ? (float)(v + nvel[c - 7]) : (float)v;
-      }
+      quad32_start(a, s0, nvel, s);
       float plo[3], phi[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -283,7 +301,8 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
         for (int c = 7; c < 13; ++c) {
           const float lo_ = (float)S.lo[c], hi_ = (float)S.hi[c];
           bad = bad || !(s[c] >= lo_ && s[c] <= hi_);
-          nb_ = nb_ || fabsf(s[c] - lo_) <= eps || fabsf(s[c] - hi_) <= eps;
+          const float e = c < 10 ? eps_v : eps_w;
+          nb_ = nb_ || fabsf(s[c] - lo_) <= e || fabsf(s[c] - hi_) <= e;
         }
         near_bound_any = near_bound_any || nb_;
         if (bad) {
@@ -365,9 +384,10 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
 __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
   const dp_dev_sys& S = *a.sys;
   const int32_t tot = *a.total;
-  long long c_b = 0, c_c = 0, c_f = 0, c_g = 0, c_k = 0, c_r = 0;
+  long long c_b = 0, c_c = 0, c_f = 0, c_k = 0, c_r = 0;
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
     int8_t fl = a.exact ? (int8_t)QF_NEAR : a.flags[q];
+    if (a.redo) fl = (fl & DP_F_DEFER) ? (int8_t)QF_NEED64 : (int8_t)0;
     if (!(fl & (QF_NEAR | QF_NEED64))) continue;
     const bool full = (fl & QF_NEAR) != 0;
     if (full && !a.exact) ++c_r;
@@ -430,9 +450,11 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
     }
     a.verdict[q] = vd;
     a.firstk[q] = fk;
-    c_b += vd == DP_V_BOUNDS;
-    c_c += vd == DP_V_COLLIDE;
-    c_k += vd != DP_V_BOUNDS;
+    if (!a.redo) {  // deferred pairs were counted by k_quad_window
+      c_b += vd == DP_V_BOUNDS;
+      c_c += vd == DP_V_COLLIDE;
+      c_k += vd != DP_V_BOUNDS;
+    }
     if (vd != DP_V_FREE) {
       a.goal_out[q] = -1;
       a.res[q] = DP_R_NONE;
@@ -457,13 +479,11 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
     a.goal_out[q] = chosen;
     a.res[q] = DP_R_UNDECIDED;
     a.flags[q] = (int8_t)(chosen != -1 ? 2 : 0);  // the duplicate bit is set by k_quad_dupv
-    ++c_f;
-    c_g += chosen != -1;
+    c_f += !a.redo;
   }
   if (c_b) atomicAdd((unsigned long long*)&a.counters[DPC_BOUNDS], (unsigned long long)c_b);
   if (c_c) atomicAdd((unsigned long long*)&a.counters[DPC_COLLIDE], (unsigned long long)c_c);
   if (c_f) atomicAdd((unsigned long long*)&a.counters[DPC_FREE], (unsigned long long)c_f);
-  if (c_g) atomicAdd((unsigned long long*)&a.counters[DPC_GOAL], (unsigned long long)c_g);
   if (c_k) atomicAdd((unsigned long long*)&a.counters[DPC_CHECKED], (unsigned long long)c_k);
   if (c_r) atomicAdd((unsigned long long*)&a.counters[DPC_RECHECK], (unsigned long long)c_r);
 }
@@ -476,7 +496,7 @@ __global__ __launch_bounds__(256) void k_quad_dupv(dp_quad_args a) {
   const int32_t tot = *a.total;
   const double T = dp_pos_bound(S, a.r_dup);
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
-    if (a.verdict[q] != DP_V_FREE) continue;
+    if (a.verdict[q] != DP_V_FREE || (a.flags[q] & DP_F_DEFER)) continue;
     double x[13];
 #pragma unroll
     for (int c = 0; c < 13; ++c) x[c] = a.fin[(int64_t)c * a.pcap + q];
@@ -497,9 +517,85 @@ __global__ void k_quad_chunks(int32_t N, const int32_t* __restrict__ app, const 
   }
 }
 
-}  // namespace
+// FP64 window of each node: the pairs that may be FREE after the FP32 pass
+// (FREE, or near contact and re-checked in FP64) are ranked in canonical
+// order, and a FREE pair at rank >= window is deferred: no FP64 rollout, no
+// final state, goal index, g or h unless its node's cap does not fall before
+// it (the resolution then reports a window overflow and the batch is
+// resolved again with every deferred pair rolled out).  The reference
+// never computes a pair after its node's cap (dbastar.cpp:916-918).  A
+// deferred pair's verdict (FREE) is exact from the FP32 pass.
+__global__ __launch_bounds__(64) void k_quad_window(int32_t N, const int32_t* __restrict__ app,
+                                                    const int32_t* __restrict__ total, int8_t* __restrict__ verdict,
+                                                    int32_t* __restrict__ firstk, int32_t* __restrict__ goal_out,
+                                                    int8_t* __restrict__ res, int8_t* __restrict__ flags,
+                                                    int32_t window, long long* __restrict__ counters) {
+  const int32_t node = blockIdx.x;
+  if (node >= N) return;
+  const int lane = threadIdx.x;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const int32_t tot = *total;
+  const int32_t b = app[node] < tot ? app[node] : tot;
+  const int32_t e = app[node + 1] < tot ? app[node + 1] : tot;
+  int32_t rank = 0;
+  long long nd = 0;
+  for (int32_t cb = b; cb < e; cb += 64) {
+    const int32_t p = cb + lane;
+    const int8_t fl = p < e ? flags[p] : (int8_t)0;
+    const uint64_t m = __ballot((fl & (QF_NEAR | QF_NEED64)) != 0);
+    if ((fl & QF_NEED64) && rank + __popcll(m & lt) >= window) {
+      flags[p] = (int8_t)DP_F_DEFER;
+      verdict[p] = DP_V_FREE;
+      firstk[p] = -1;
+      goal_out[p] = -1;
+      res[p] = DP_R_UNDECIDED;
+      ++nd;
+    }
+    rank += __popcll(m);
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) nd += __shfl_xor(nd, d, 64);
+  if (lane == 0 && nd) {
+    atomicAdd((unsigned long long*)&counters[DPC_DEFERRED], (unsigned long long)nd);
+    atomicAdd((unsigned long long*)&counters[DPC_FREE], (unsigned long long)nd);
+    atomicAdd((unsigned long long*)&counters[DPC_CHECKED], (unsigned long long)nd);
+  }
+}
 
-int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, double r_dup, hipStream_t s) {
+// Every FP32 state of the listed pairs of the last batch, as k_quad32 forms
+// them (the same step function), position made absolute in FP64: out
+// [n][maxlen][13], rows past a primitive's length untouched.  One thread per
+// pair; error reports only.
+__global__ void k_quad32_trace(dp_quad_args a, const int32_t* __restrict__ pairs, int32_t n, int32_t maxlen,
+                               double* __restrict__ out) {
+  const dp_dev_sys& S = *a.sys;
+  const quad_f32 K = make_quad_f32(S);
+  for (int32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const int32_t q = pairs[t];
+    const int32_t node = a.pnode[q], j = a.pprim[q];
+    const int32_t s0 = a.off[j];
+    const int32_t L1 = a.off[j + 1] - s0;
+    const int32_t a0 = s0 - j;
+    double np[3], nvel[3];
+    for (int c = 0; c < 3; ++c) {
+      np[c] = a.nodes[(int64_t)c * a.Ncap + node];
+      nvel[c] = a.nodes[(int64_t)(7 + c) * a.Ncap + node];
+    }
+    float s[13];
+    quad32_start(a, s0, nvel, s);
+    for (int k = 0; k < L1 && k < maxlen; ++k) {
+      if (k > 0) {
+        const float4 uv = reinterpret_cast<const float4*>(a.act32)[a0 + k - 1];
+        const float u[4] = {uv.x, uv.y, uv.z, uv.w};
+        quad_step32(K, s, u);
+      }
+      double* o = out + ((int64_t)t * maxlen + k) * 13;
+      for (int c = 0; c < 13; ++c) o[c] = c < 3 ? np[c] + (double)s[c] : (double)s[c];
+    }
+  }
+}
+
+dp_quad_args make_quad_args(dp_ctx* c, const dp_expand_params* p, double goal_bound, double r_dup) {
   dp_quad_args a{};
   a.sys = c->d_sys;
   a.nodes = c->d_nodes;
@@ -546,7 +642,64 @@ int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, doub
   a.r_dup = r_dup;
   a.cdf = p->cost_delta_factor;
   a.exact = p->fp32_rollout ? 0 : 1;
+  a.redo = 0;
   a.counters = c->d_counters;
+  return a;
+}
+
+}  // namespace
+
+int32_t dp_fp64_window(const dp_ctx* c, int32_t limit) {
+  if (c->fp64_window > 0) return c->fp64_window;
+  return limit >= 0 ? DP_FP64_WINDOW(limit) : INT32_MAX;
+}
+
+int dp_launch_quad_deferred(dp_ctx* c, hipStream_t s) {
+  const dp_expand_params* p = &c->round_params;
+  dp_quad_args a = make_quad_args(c, p, p->delta_factor_goal * (double)p->delta, c->round_r_dup);
+  a.redo = 1;
+  const int t64 = c->tiles.quad64_threads;
+  k_quad64<<<dp_grid_blocks(c->pair_cap, t64, c->tiles.quad64_blocks), t64, 0, s>>>(a);
+  a.redo = 0;
+  if (c->nv > 0) k_quad_dupv<<<dp_grid_blocks(c->pair_cap, 256, 8192), 256, 0, s>>>(a);
+  DP_HIP_RET(hipGetLastError());
+  return DP_OK;
+}
+
+extern "C" int dp_debug_fp32_states(dp_ctx* c, const int32_t* pairs, int32_t n, double* out) {
+  if (!c || (n > 0 && (!pairs || !out)) || n < 0) {
+    dp_set_error("invalid argument");
+    return DP_ERR_ARG;
+  }
+  if (c->desc.system != DP_SYS_QUADROTOR || c->N <= 0) {
+    dp_set_error("no quadrotor batch expanded");
+    return DP_ERR_STATE;
+  }
+  if (n == 0) return DP_OK;
+  DP_HIP_RET(hipSetDevice(c->device));
+  DP_HIP_RET(hipDeviceSynchronize());
+  int32_t* d_pairs = nullptr;
+  double* d_out = nullptr;
+  const size_t out_bytes = sizeof(double) * (size_t)n * c->maxlen * 13;
+  DP_HIP_RET(hipMalloc(&d_pairs, sizeof(int32_t) * n));
+  hipError_t e = hipMalloc(&d_out, out_bytes);
+  if (e == hipSuccess) e = hipMemcpy(d_pairs, pairs, sizeof(int32_t) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const dp_expand_params p = c->round_params;
+    dp_quad_args a = make_quad_args(c, &p, 0.0, 0.0);
+    k_quad32_trace<<<dp_grid_blocks(n, 256, 1024), 256, 0, c->stream>>>(a, d_pairs, n, c->maxlen, d_out);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(out, d_out, out_bytes, hipMemcpyDeviceToHost);
+  (void)hipFree(d_pairs);
+  (void)hipFree(d_out);
+  DP_HIP_RET(e);
+  return DP_OK;
+}
+
+int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, double r_dup, hipStream_t s) {
+  dp_quad_args a = make_quad_args(c, p, goal_bound, r_dup);
   if (p->fp32_rollout) {
     const size_t lds = sizeof(float) * kQWaves * (6 * (size_t)c->nb + 4 * (size_t)c->ns + 1);
     if (lds > 64 * 1024) {
@@ -556,7 +709,18 @@ int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, doub
     k_quad_chunks<<<dp_grid_blocks(c->N, 256, 1024), 256, 0, s>>>(c->N, c->d_app, c->d_total, c->d_chunk);
     dp_scan_small(c->d_chunk, c->N, nullptr, INT64_MAX, c->d_counters, -1, s);
     const int64_t max_chunks = (int64_t)c->N * ((c->P + 63) / 64);
+    int rc = dp_record_event(c, 5, s);
+    if (rc) return rc;
     k_quad32<<<dp_grid_blocks(max_chunks, kQWaves, c->tiles.quad32_blocks), 256, lds, s>>>(a);
+    if ((rc = dp_record_event(c, 6, s))) return rc;
+    const int32_t window = dp_fp64_window(c, p->limit_branching_factor);
+    if (window < INT32_MAX && c->N > 0)
+      k_quad_window<<<c->N, 64, 0, s>>>(c->N, c->d_app, c->d_total, c->d_verdict, c->d_firstk, c->d_goal, c->d_res,
+                                        c->d_flags, window, c->d_counters);
+  } else {
+    int rc = dp_record_event(c, 5, s);
+    if (!rc) rc = dp_record_event(c, 6, s);
+    if (rc) return rc;
   }
   const int t64 = c->tiles.quad64_threads;
   k_quad64<<<dp_grid_blocks(c->pair_cap, t64, c->tiles.quad64_blocks), t64, 0, s>>>(a);

@@ -222,7 +222,10 @@ int dp_search(dp_ctx* c, const double* start, const double* goal, const dp_searc
   open.nodes = &nodes;
   node_grid grid;
   grid.dims = S.system == DP_SYS_QUADROTOR ? 3 : 2;
-  grid.inv = 1.0 / ((r_dup / S.w[0]) * (1.0 + 1e-9));
+  // cells at least r_dup / w0 wide (the 3x3(x3) block then holds every
+  // neighbour); never narrower than 1 um, so that delta = 0 keeps cell
+  // coordinates finite
+  grid.inv = 1.0 / std::max((r_dup / S.w[0]) * (1.0 + 1e-9), 1e-6);
 
   auto h_of = [&](const double* x) { return opt->heuristic == 0 ? dp_lower_bound_time(S, x, goal) : 0.0; };
   {

@@ -133,10 +133,27 @@ class GpuExpander:
         N.check(self.lib.dp_get_counts(self.ctx, C.byref(c)))
         return c.as_dict()
 
+    def health(self):
+        """Never-reset totals: batches, resolution waves that gave up, FP64
+        window overflows, batches completed by host-launched work."""
+        h = N.HealthInfo()
+        N.check(self.lib.dp_health(self.ctx, C.byref(h)))
+        return h.as_dict()
+
+    def debug_set_limits(self, conf_slots=0, pool_entries=0, fp64_window=0):
+        N.check(self.lib.dp_debug_set_limits(self.ctx, int(conf_slots), int(pool_entries), int(fp64_window)))
+
     def timing(self):
         ms = (C.c_float * 5)()
         N.check(self.lib.dp_last_timing(self.ctx, ms))
         return {"applicability": ms[0], "check": ms[1], "resolve": ms[2], "total": ms[3], "surrogate": ms[4]}
+
+    def timing_detail(self):
+        """Device ms of the last expansion per kernel group (HIP events)."""
+        ms = (C.c_float * 7)()
+        N.check(self.lib.dp_last_timing_detail(self.ctx, ms))
+        keys = ("applicability", "surrogate", "check_pre", "check_main", "check_post", "lists", "walk")
+        return dict(zip(keys, [float(x) for x in ms]))
 
     def tiles(self):
         t = N.TileConfig()

@@ -359,7 +359,14 @@ def quad_step_np(desc, s, u):
     return out
 
 
-def quadrotor_primitives(rng, P, lmin, lmax, desc=None, tilt_max=0.25, w_sd=0.7, base=(0.9, 1.1), diff=0.02):
+def quadrotor_primitives(rng, P, lmin, lmax, desc=None, tilt_max=0.25, w_sd=0.7, thrust="uniform", base=(0.9, 1.1),
+                         diff=0.02):
+    """P primitives of L ~ U{lmin..lmax} steps: 1 to 3 pieces of constant
+    rotor thrusts (hover = 1), rolled out in FP64 from a random attitude and
+    angular rate at the origin.  thrust="uniform": each rotor U[0, 1.4] per
+    piece (the thrust range of quadrotor_v0); most such primitives leave the
+    angular-rate bounds within a few steps.  thrust="balanced": U[base] plus
+    U[-diff, diff] per rotor, primitives that stay near hover."""
     desc = desc or default_desc_static(N.SYS_QUADROTOR)
     L = rng.integers(lmin, lmax + 1, size=P)
     Lmax = int(L.max())
@@ -369,7 +376,10 @@ def quadrotor_primitives(rng, P, lmin, lmax, desc=None, tilt_max=0.25, w_sd=0.7,
         cuts = np.sort(rng.integers(1, Lmax, size=pieces[j] - 1))
         bnd = np.concatenate([[0], cuts, [Lmax]])
         for a, b in zip(bnd[:-1], bnd[1:]):
-            u[j, a:b] = np.clip(rng.uniform(*base) + rng.uniform(-diff, diff, 4), 0.0, 1.4)
+            if thrust == "uniform":
+                u[j, a:b] = rng.uniform(0.0, 1.4, 4)
+            else:
+                u[j, a:b] = np.clip(rng.uniform(*base) + rng.uniform(-diff, diff, 4), 0.0, 1.4)
     x = np.zeros((P, Lmax + 1, 13))
     x[:, 0, 3:7] = random_attitudes(rng, P, tilt_max)
     x[:, 0, 10:13] = rng.normal(0, w_sd, (P, 3))
@@ -389,10 +399,13 @@ def sphere_collides_np(desc, boxes, spheres, p):
 
 
 def quadrotor(seed=3, P=10000, lmin=10, lmax=50, n_boxes=100, n_spheres=50, N_nodes=2048, n_visited=20000,
-              limit=20, tilt_max=0.25, w_sd=0.7):
+              limit=20, tilt_max=0.25, w_sd=0.7, thrust="uniform"):
+    """quadrotor_v0 batch: P primitives of 10-50 steps (dt 0.01), a 10x10x5 m
+    world with boxes (sides U[0.5, 1.5]) and spheres (r U[0.2, 0.8]), a 0.1 m
+    sphere robot, N frontier nodes and V visited states in free space."""
     rng = np.random.default_rng(seed)
     desc = default_desc_static(N.SYS_QUADROTOR)
-    offsets, states, actions = quadrotor_primitives(rng, P, lmin, lmax, desc, tilt_max, w_sd)
+    offsets, states, actions = quadrotor_primitives(rng, P, lmin, lmax, desc, tilt_max, w_sd, thrust)
     lo = np.array([0.0, 0.0, 0.0])
     hi = np.array([10.0, 10.0, 5.0])
     boxes = random_boxes(rng, n_boxes, lo, hi, 0.5, 1.5, dim=3)
@@ -414,7 +427,8 @@ def quadrotor(seed=3, P=10000, lmin=10, lmax=50, n_boxes=100, n_spheres=50, N_no
     visited = states_in_free_space(n_visited)
     goal = states_in_free_space(1)[0]
     params = make_params(goal=goal, limit_branching_factor=limit, fp32_rollout=True)
-    return Workload(name=f"quadrotor_v0 N={N_nodes} P={P} O={n_boxes}+{n_spheres} V={n_visited}", desc=desc,
+    tag = "" if thrust == "uniform" else f" thrust={thrust}"
+    return Workload(name=f"quadrotor_v0 N={N_nodes} P={P} O={n_boxes}+{n_spheres} V={n_visited}{tag}", desc=desc,
                     offsets=offsets, states=states, actions=actions, lo=lo, hi=hi, boxes=boxes, spheres=spheres,
                     nodes=nodes, g_parent=rng.uniform(0, 10, N_nodes), visited=visited, params=params, seed=seed)
 
@@ -443,7 +457,7 @@ def multi_instance(seed=6, n_instances=128, p_range=(1000, 10000), o_range=(10, 
         elif s == N.SYS_UNICYCLE2:
             libs[s] = unicycle2_primitives(lr, pmax, 5, 20)
         elif s == N.SYS_QUADROTOR:
-            libs[s] = quadrotor_primitives(lr, pmax, 10, 50)
+            libs[s] = quadrotor_primitives(lr, pmax, 10, 50, thrust="balanced")
         else:
             libs[s] = car_trailer_primitives(lr, pmax, 10, 40)
     out = []

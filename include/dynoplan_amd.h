@@ -21,10 +21,12 @@
  * duplicate decision of a FREE pair is taken against the visited set plus
  * every NOVEL pair that precedes it in canonical order, exactly as the
  * reference inserts each novel node into T_n at once (dbastar.cpp:875-876).
- * A FREE pair of node i is CAPPED when limit_branching_factor NOVEL pairs of
- * node i precede it (the reference's loop break).  The reference shuffles
- * the applicable list (dbastar.hpp:302-303); the canonical order is the
- * identity shuffle.
+ * The reference leaves a node's loop after a FREE pair once its NOVEL count
+ * reaches limit_branching_factor (dbastar.cpp:916-918): every later FREE
+ * pair of the node is CAPPED, and, as the reference never reaches it, has no
+ * goal index, final state, g or h (dp_fetch_pairs gives -1 / NaN).  The
+ * reference shuffles the applicable list (dbastar.hpp:302-303); the
+ * canonical order is the identity shuffle.
  *
  * Conventions: every entry point returns DP_OK (0) or a negative DP_ERR_*;
  * dp_last_error() gives a thread-local message.  No exceptions cross the
@@ -88,7 +90,9 @@ typedef struct dp_system_desc {
   double hitch;                /* car+trailer: hitch length; trailer centre at p - hitch*(cos th1, sin th1) */
   double wheelbase;            /* car+trailer: tractor wheelbase */
   double quad[8];              /* quadrotor: mass, g, arm, t2t, Jx, Jy, Jz, -          */
-  double near_eps;             /* FP32 path: margin (m) under which a state is re-checked in FP64 */
+  double near_eps;             /* FP32 path: margin under which a verdict is re-checked in FP64: near_eps
+                                  (m) for clearances and position bounds, 10 x for velocity and 100 x for
+                                  angular-rate bounds */
 } dp_system_desc;
 
 /* Motion primitives (dynoplan::Motion, ompl/robots.h:206-235), row-major as
@@ -118,7 +122,9 @@ typedef struct dp_expand_params {
   float alpha;                     /* in (0,1) */
   double delta_factor_goal;        /* goal radius = delta_factor_goal * delta */
   double cost_delta_factor;
-  int32_t limit_branching_factor;  /* <= 0: no cap */
+  int32_t limit_branching_factor;  /* the reference's break (dbastar.cpp:916-918), a size_t NOVEL count
+                                      compared with this int: 0 decides only the first FREE pair of a
+                                      node, a negative value never caps */
   int32_t num_check_goal;          /* intermediate goal checks (reference: 4) */
   int32_t check_intermediate_goal; /* reference: true */
   int32_t use_surrogate;           /* learned pre-filter (needs a trained surrogate) */
@@ -137,6 +143,9 @@ typedef struct dp_counts {
   int64_t checked;          /* pairs that reached the collision check (num_col_motions) */
   int64_t recheck_fp64;     /* FP32 pairs re-evaluated in FP64 near contact */
   int64_t resolve_rounds;   /* diagnostic: rounds of the batch duplicate resolution */
+  int64_t conf_pool_used;   /* diagnostic: earlier-conflict entries past the inline slots */
+  int64_t fp64_deferred;    /* quadrotor: FREE pairs past their node's FP64 window (rolled out in FP64
+                               only if the node's cap did not fall before them) */
 } dp_counts;
 
 /* Caller-owned host buffers for dp_fetch_pairs (any pointer may be NULL). */
@@ -223,8 +232,32 @@ int dp_expand(dp_ctx* ctx, const dp_expand_params* params, void* stream);
 int dp_expand_batch(dp_ctx* ctx, const double* nodes, const double* g_parent, int32_t N,
                     const dp_expand_params* params, void* stream);
 
-/* Counters of the last dp_expand (synchronises the stream). */
+/* Counters of the last dp_expand (synchronises the stream).  Finishes the
+ * batch first if its device pipeline left work (a resolution wave that ran
+ * out of its wait bound, or a quadrotor node whose cap fell past its FP64
+ * window): dp_health counts such batches. */
 int dp_get_counts(dp_ctx* ctx, dp_counts* out);
+
+/* Totals over the context's life, never reset (no counterpart in the
+ * reference): expansions issued, resolution waves that gave up waiting
+ * (their nodes finished by host-launched rounds), FP64 window overflows,
+ * and batches completed by host-launched work in dp_get_counts.  A caller
+ * timing dp_expand calls checks that the last three did not move. */
+typedef struct dp_health_info {
+  int64_t batches;
+  int64_t wave_giveups;
+  int64_t window_overflows;
+  int64_t host_completions;
+} dp_health_info;
+int dp_health(dp_ctx* ctx, dp_health_info* out);
+
+/* Test-only limits (no counterpart in the reference) that force the rare
+ * paths of the resolution: inline earlier-conflict slots used per pair
+ * (1..64; 0 restores 64), overflow pool entries (0: the full pool; a list
+ * that does not fit is scanned from the grid when reached) and the quadrotor
+ * FP64 window in pairs that may be FREE (0: 3 * limit + 32).  Results do not
+ * depend on them. */
+int dp_debug_set_limits(dp_ctx* ctx, int32_t conf_slots, int64_t pool_entries, int32_t fp64_window);
 
 /* Copy the per-pair results of the last dp_expand into caller buffers, in
  * canonical (node, primitive) order.  Returns DP_ERR_CAPACITY if the batch
@@ -238,6 +271,12 @@ void* dp_stream(dp_ctx* ctx);
  * (HIP events on the expansion stream), ms5[5]: 0 applicability, 1 check,
  * 2 duplicate resolution, 3 total, 4 learned pre-filter (0 without it). */
 int dp_last_timing(dp_ctx* ctx, float* ms5);
+/* The same split per kernel group, ms7[7]: 0 applicability, 1 learned
+ * pre-filter, 2 before the main check kernel (node obstacle lists; quadrotor
+ * chunk offsets), 3 the main check kernel (k_check_planar; quadrotor
+ * k_quad32), 4 after it (quadrotor: FP64 window, FP64 pass, visited
+ * duplicates), 5 contender grid and conflict lists, 6 the ordered walk. */
+int dp_last_timing_detail(dp_ctx* ctx, float* ms7);
 
 /* Launch shapes of the check stage.  Every configuration gives identical
  * results; dp_autotune picks the fastest for the uploaded batch. */
@@ -354,6 +393,13 @@ int dp_debug_gemm_bf16(dp_ctx* ctx, const float* A, const float* B, int32_t M, i
  * dp_expand (other entries unspecified).  No counterpart in the reference;
  * used to report the FP32-vs-FP64 rollout error per system. */
 int dp_debug_fp32_positions(dp_ctx* ctx, float* out, int64_t n);
+
+/* Quadrotor FP32 rollout error report, every state: the FP32 states of the
+ * listed pairs (indices in the last dp_expand's canonical pair order) as
+ * the FP32 pass forms them, positions made absolute in FP64, into
+ * out[n][L][13] with L the longest primitive's state count (rows past a
+ * pair's own length untouched).  No counterpart in the reference. */
+int dp_debug_fp32_states(dp_ctx* ctx, const int32_t* pairs, int32_t n, double* out);
 
 /* Diagnostics of the duplicate resolution (no counterpart in the reference):
  * enable > 0 records, for each node of later batches, its wave's timeline;

@@ -534,6 +534,15 @@ static void swept_candidates(const dp_system_desc* sys, const dp_world* world, c
   *ncs = ns;
 }
 
+/* The break of dbastar.cpp:916-918, taken after each FREE pair:
+ * num_expansion_best_node (size_t, the NOVEL pairs so far) >=
+ * limit_branching_factor (int, converted to size_t).  A limit of 0 breaks
+ * after the first FREE pair whatever its resolution; a negative limit
+ * converts to a huge value and never breaks. */
+static int cap_reached(const dp_expand_params* par, int32_t novel_in_node) {
+  return par->limit_branching_factor >= 0 && novel_in_node >= par->limit_branching_factor;
+}
+
 /* T_n->nearestR(x, radius) (nigh_custom_spaces.hpp:236-247), the rewiring
  * query of dbastar.cpp:883-912, by brute force: for each query, every
  * visited index v with distance <= radius, ascending (CSR).  Returns the
@@ -596,6 +605,7 @@ int64_t oracle_expand(const dp_system_desc* sys, const dp_prims* prims, int32_t 
     const double* node = nodes + (int64_t)i * nx;
     canonical_state(sys, node, canon);
     int32_t novel_in_node = 0;
+    int broke = 0; /* the reference's loop over this node's pairs has ended */
     for (int32_t j = 0; j < P; ++j) {
       const double* start = prims->states + (int64_t)prims->offsets[j] * nx;
       if (!(oracle_distance(sys, canon, start) <= r_app)) continue; /* nearestR, <= */
@@ -647,14 +657,18 @@ int64_t oracle_expand(const dp_system_desc* sys, const dp_prims* prims, int32_t 
             }
           }
         }
-        if (chosen != -1) counts->goal_hits++;
         hq = oracle_lower_bound_time(sys, x, par->goal);
         double cost_motion = chosen != -1 ? chosen * sys->dt : (L1 - 1) * sys->dt;
         gq = g_parent[i] + cost_motion + par->cost_delta_factor * oracle_lower_bound_time(sys, node, traj);
-        if (par->limit_branching_factor > 0 && novel_in_node >= par->limit_branching_factor) {
-          res = DP_R_CAPPED; /* the reference broke out of the loop, dbastar.cpp:916-918 */
+        if (broke) {
+          /* the reference left the loop (dbastar.cpp:916-918) before this
+           * pair: it has no goal index, final state, g or h */
+          res = DP_R_CAPPED;
           counts->capped++;
+          chosen = -1;
+          gq = hq = NAN;
         } else {
+          if (chosen != -1) counts->goal_hits++;
           int dv = nn_any_within(sys, &vis, x, r_dup);
           int db = dv ? 0 : nn_any_within(sys, &acc, x, r_dup);
           if ((!dv && !db) || chosen != -1) { /* dbastar.cpp:856 */
@@ -669,6 +683,7 @@ int64_t oracle_expand(const dp_system_desc* sys, const dp_prims* prims, int32_t 
             res = DP_R_DUP_BATCH;
             counts->dup_batch++;
           }
+          broke = cap_reached(par, novel_in_node);
         }
       }
       if (wr) {
@@ -679,7 +694,8 @@ int64_t oracle_expand(const dp_system_desc* sys, const dp_prims* prims, int32_t 
         if (out->goal_index) out->goal_index[q] = chosen;
         if (out->resolution) out->resolution[q] = res;
         if (out->final_state)
-          for (int c = 0; c < nx; ++c) out->final_state[q * nx + c] = verdict == DP_V_FREE ? x[c] : NAN;
+          for (int c = 0; c < nx; ++c)
+            out->final_state[q * nx + c] = verdict == DP_V_FREE && res != DP_R_CAPPED ? x[c] : NAN;
         if (out->g) out->g[q] = gq;
         if (out->h) out->h[q] = hq;
       }
@@ -831,6 +847,7 @@ int64_t oracle_expand_mt(const dp_system_desc* sys, const dp_prims* prims, int32
   int overflow = 0;
   for (int32_t i = 0; i < N; ++i) {
     int32_t novel_in_node = 0;
+    int broke = 0; /* the reference's loop over this node's pairs has ended */
     for (int64_t q = first[i]; q < first[i + 1]; ++q) {
       int8_t res = DP_R_NONE;
       if (vd[q] == DP_V_BOUNDS) counts->bounds++;
@@ -839,11 +856,11 @@ int64_t oracle_expand_mt(const dp_system_desc* sys, const dp_prims* prims, int32
       if (vd[q] == DP_V_FREE) {
         const double* x = xf + q * nx;
         counts->free_++;
-        if (ch[q] != -1) counts->goal_hits++;
-        if (par->limit_branching_factor > 0 && novel_in_node >= par->limit_branching_factor) {
+        if (broke) {
           res = DP_R_CAPPED;
           counts->capped++;
         } else {
+          if (ch[q] != -1) counts->goal_hits++;
           int db = dv[q] ? 0 : nn_any_within(sys, &acc, x, r_dup);
           if ((!dv[q] && !db) || ch[q] != -1) {
             res = DP_R_NOVEL;
@@ -857,6 +874,7 @@ int64_t oracle_expand_mt(const dp_system_desc* sys, const dp_prims* prims, int32
             res = DP_R_DUP_BATCH;
             counts->dup_batch++;
           }
+          broke = cap_reached(par, novel_in_node);
         }
       }
       if (out && q >= out->capacity) overflow = 1;
@@ -865,12 +883,13 @@ int64_t oracle_expand_mt(const dp_system_desc* sys, const dp_prims* prims, int32
         if (out->prim) out->prim[q] = pj[q];
         if (out->verdict) out->verdict[q] = vd[q];
         if (out->first_k) out->first_k[q] = fk[q];
-        if (out->goal_index) out->goal_index[q] = ch[q];
+        const int capped = res == DP_R_CAPPED;
+        if (out->goal_index) out->goal_index[q] = capped ? -1 : ch[q];
         if (out->resolution) out->resolution[q] = res;
         if (out->final_state)
-          for (int c = 0; c < nx; ++c) out->final_state[q * nx + c] = xf[q * nx + c];
-        if (out->g) out->g[q] = gg[q];
-        if (out->h) out->h[q] = hh[q];
+          for (int c = 0; c < nx; ++c) out->final_state[q * nx + c] = capped ? NAN : xf[q * nx + c];
+        if (out->g) out->g[q] = capped ? NAN : gg[q];
+        if (out->h) out->h[q] = capped ? NAN : hh[q];
       }
     }
   }

@@ -20,7 +20,11 @@ def assert_parity(gpu, ora, label=""):
         assert bad.size == 0, (f"{label}: field {k} differs at {bad.size} pairs, first at {bad[0]}: "
                                f"gpu={a[bad[0]]} oracle={b[bad[0]]} (node {ora['node'][bad[0]]}, "
                                f"prim {ora['prim'][bad[0]]}, verdict {ora['verdict'][bad[0]]})")
-    free = ora["verdict"] == N.V_FREE
+    # CAPPED pairs come after their node's loop ended: no final state, g, h
+    capped = ora["resolution"] == N.R_CAPPED
+    for k in ("final_state", "g", "h"):
+        assert np.isnan(gpu[k][capped]).all(), f"{label}: {k} of a CAPPED pair is not NaN"
+    free = (ora["verdict"] == N.V_FREE) & ~capped
     if not free.any():
         return
     for k in ("final_state", "g", "h"):

@@ -20,10 +20,10 @@ def instances():
     wls = [
         synth.unicycle1_small(seed=21, N_nodes=64, P=400),
         synth.unicycle2_small(seed=22, N_nodes=300, P=700),
-        synth.quadrotor(seed=23, P=600, N_nodes=128, n_visited=2000),
+        synth.quadrotor(seed=23, P=600, N_nodes=128, n_visited=2000, thrust="balanced"),
         synth.car_trailer(seed=24, P=500, N_nodes=200, n_boxes=80, extent=12.0, n_visited=3000),
         synth.unicycle1_scale(seed=25, N_nodes=512, P=1500, n_visited=20000),
-        synth.quadrotor(seed=26, P=300, N_nodes=64, n_visited=500, n_boxes=20, n_spheres=10),
+        synth.quadrotor(seed=26, P=300, N_nodes=64, n_visited=500, n_boxes=20, n_spheres=10, thrust="balanced"),
     ]
     for w in wls:
         if w.desc.system == N.SYS_QUADROTOR:

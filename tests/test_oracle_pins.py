@@ -521,27 +521,35 @@ def test_oracle_applicability_brute_force(lib):
     assert got == expect
 
 
-@pytest.mark.parametrize("limit", [0, 3, 20])
+@pytest.mark.parametrize("limit", [-1, 0, 3, 20])
 def test_oracle_sequential_duplicates_restated(lib, limit):
     """Pure-Python restatement of dbastar.cpp:850-918 over the oracle's own
-    verdicts and final states: same resolutions."""
+    verdicts and final states: same resolutions.  The loop breaks after a
+    FREE pair once the node's NOVEL count >= limit (size_t against int: 0
+    breaks after the first FREE pair, a negative limit never); pairs after
+    the break are CAPPED and carry no goal index, final state, g or h."""
     wl = small(N.SYS_UNICYCLE1, seed=10)
     p = synth.make_params(goal=wl.nodes[3], delta=0.5, limit_branching_factor=limit)
     out, c = O.expand(wl, params=p, nn_mode=O.NN_BRUTE)
+    p_free = synth.make_params(goal=wl.nodes[3], delta=0.5, limit_branching_factor=-1)
+    full, _ = O.expand(wl, params=p_free, nn_mode=O.NN_BRUTE)  # every FREE pair decided
     d = wl.desc
     r = (1.0 - float(np.float32(p.alpha))) * float(np.float32(p.delta))
     tree = [v for v in wl.visited]
     n_vis = len(tree)
     novel_in_node = {}
+    broke = set()
     for q in range(out["node"].size):
         if out["verdict"][q] != N.V_FREE:
             assert out["resolution"][q] == N.R_NONE
             continue
         i = out["node"][q]
-        if limit > 0 and novel_in_node.get(i, 0) >= limit:
+        if i in broke:
             assert out["resolution"][q] == N.R_CAPPED
+            assert out["goal_index"][q] == -1 and np.isnan(out["g"][q]) and np.isnan(out["final_state"][q]).all()
             continue
-        x = out["final_state"][q]
+        x = full["final_state"][q]
+        assert np.array_equal(out["final_state"][q], x) and out["goal_index"][q] == full["goal_index"][q]
         near = [k for k, t in enumerate(tree) if dist(lib, d, x, t) <= r]
         if not near or out["goal_index"][q] != -1:
             assert out["resolution"][q] == N.R_NOVEL
@@ -551,6 +559,16 @@ def test_oracle_sequential_duplicates_restated(lib, limit):
             assert out["resolution"][q] == N.R_DUP_VISITED
         else:
             assert out["resolution"][q] == N.R_DUP_BATCH
+        if limit >= 0 and novel_in_node.get(i, 0) >= limit:
+            broke.add(i)
+    if limit == 0:
+        # exactly one decided FREE pair per node that has FREE pairs
+        dec = (out["verdict"] == N.V_FREE) & (out["resolution"] != N.R_CAPPED)
+        has_free = np.unique(out["node"][out["verdict"] == N.V_FREE])
+        assert np.array_equal(np.unique(out["node"][dec]), has_free)
+        assert dec.sum() == has_free.size
+    if limit < 0:
+        assert c["capped"] == 0
     assert c["novel"] == len(tree) - n_vis
 
 

@@ -42,14 +42,41 @@ def test_unicycle1_small_full(need_gpu):
         assert cg[k] == co[k], k
 
 
-@pytest.mark.parametrize("limit", [0, 1, 5])
+@pytest.mark.parametrize("limit", [-1, 0, 1, 5])
 def test_unicycle1_branching_cap(need_gpu, limit):
+    """The reference's break (dbastar.cpp:916-918): a size_t NOVEL count >=
+    the int limit after each FREE pair.  -1 never caps; 0 decides only each
+    node's first FREE pair."""
     wl = synth.unicycle1_small(seed=11)
     p = synth.make_params(goal=wl.params.goal[:3], limit_branching_factor=limit)
     gpu, cg, ora, co = run_both(wl, params=p)
     assert_parity(gpu, ora, f"limit={limit}")
-    if limit == 0:
+    counts_consistent(cg)
+    assert cg["capped"] == co["capped"] and cg["goal_hits"] == co["goal_hits"]
+    if limit < 0:
         assert cg["capped"] == 0
+    if limit == 0:
+        dec = (gpu["verdict"] == N.V_FREE) & (gpu["resolution"] != N.R_CAPPED)
+        with_free = np.unique(gpu["node"][gpu["verdict"] == N.V_FREE])
+        assert dec.sum() == with_free.size and np.array_equal(np.unique(gpu["node"][dec]), with_free)
+
+
+def test_unicycle1_rare_list_paths(need_gpu):
+    """Conflict lists forced out of their inline slots: 2 inline entries per
+    pair, then a pool too small for every long list, so the walk reads pool
+    entries and scans the grid for lists that did not fit.  Results are the
+    oracle's."""
+    wl = synth.unicycle1_small(seed=13, N_nodes=256, n_visited=2000)
+    p = synth.make_params(goal=wl.params.goal[:3], delta=0.6, limit_branching_factor=-1)
+    ora, co = O.expand(wl, params=p)
+    for slots, pool in ((2, 0), (2, 300), (1, 1)):
+        with expander_for(wl) as ex:
+            ex.debug_set_limits(conf_slots=slots, pool_entries=pool)
+            cg = ex.expand(wl.nodes, wl.g_parent, p)
+            gpu = ex.fetch()
+        print("slots", slots, "pool", pool, "pool entries used", cg["conf_pool_used"], "dup_batch", cg["dup_batch"])
+        assert cg["conf_pool_used"] > 0
+        assert_parity(gpu, ora, f"slots={slots} pool={pool}")
 
 
 def test_unicycle1_goal_hits(need_gpu):
@@ -57,7 +84,7 @@ def test_unicycle1_goal_hits(need_gpu):
     checks fire and forced-novel pairs appear."""
     wl = synth.unicycle1_small(seed=12)
     goal = wl.nodes[7].copy()
-    p = synth.make_params(goal=goal, delta=0.6, limit_branching_factor=0)
+    p = synth.make_params(goal=goal, delta=0.6, limit_branching_factor=-1)
     gpu, cg, ora, co = run_both(wl, params=p)
     assert co["goal_hits"] > 0
     assert_parity(gpu, ora, "goal")

@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def wl():
-    return synth.quadrotor(N_nodes=512, n_visited=20000)
+    return synth.quadrotor(N_nodes=512, n_visited=20000, thrust="balanced")
 
 
 def params(wl, fp32, **kw):
@@ -43,22 +43,49 @@ def test_quadrotor_parity(need_gpu, wl, fp32):
     print("counts", c)
 
 
-def test_quadrotor_fp32_error_and_rechecks(need_gpu, wl):
-    """FP32 rollout vs the FP64 states of the same pairs."""
+@pytest.mark.parametrize("thrust", ["balanced", "uniform"])
+def test_quadrotor_fp32_error_and_rechecks(need_gpu, wl, thrust):
+    """FP32 rollout against the FP64 oracle states of the same pairs, every
+    state and every component.  The FP32 verdicts are exact only while the
+    error stays below the re-check margins (near_eps for positions, 10x for
+    velocities, 100x for angular rates): the test bounds it by a tenth of
+    each."""
     import ctypes as C
+    if thrust == "uniform":
+        wl = synth.quadrotor(seed=42, N_nodes=256, n_visited=5000)
     p = params(wl, 1, check_intermediate_goal=False)
+    rng = np.random.default_rng(0)
     with expander_for(wl) as ex:
         c = ex.expand(wl.nodes, wl.g_parent, p)
         out = ex.fetch()
         n = c["pairs"]
-        dbg = np.zeros((3, n), np.float32)
-        lib = ex.lib
-        assert lib.dp_debug_fp32_positions(ex.ctx, dbg.ctypes.data_as(C.POINTER(C.c_float)), n) == 0
-    free = (out["verdict"] == N.V_FREE) & np.isfinite(dbg).all(axis=0)
-    err = np.abs(dbg.T[free].astype(np.float64) - out["final_state"][free, :3]).max(axis=1)
-    print(f"FP32 vs FP64 final position error over {free.sum()} FREE pairs: max {err.max():.3e} m, "
-          f"mean {err.mean():.3e} m; re-checked in FP64: {c['recheck_fp64']} of {n}")
-    assert err.max() < wl.desc.near_eps / 10
+        pick = np.sort(rng.choice(n, 3000, replace=False)).astype(np.int32)
+        L = int(np.diff(wl.offsets).max())
+        st = np.full((pick.size, L, 13), np.nan)
+        N.check(ex.lib.dp_debug_fp32_states(ex.ctx, pick.ctypes.data_as(C.POINTER(C.c_int32)), pick.size,
+                                            st.ctypes.data_as(C.POINTER(C.c_double))))
+    lib = O.load()
+    pr = O.prims_struct(np.ascontiguousarray(wl.offsets, np.int32), np.ascontiguousarray(wl.states),
+                        np.ascontiguousarray(wl.actions))
+    ref = np.zeros((L, 13))
+    groups = {"position": slice(0, 3), "quaternion": slice(3, 7), "velocity": slice(7, 10), "rate": slice(10, 13)}
+    worst = {k: 0.0 for k in groups}
+    means = {k: [] for k in groups}
+    for t, q in enumerate(pick):
+        node = np.ascontiguousarray(wl.nodes[out["node"][q]])
+        n1 = lib.oracle_transform_primitive(C.byref(wl.desc), C.byref(pr), int(out["prim"][q]), O.dptr(node),
+                                            O.dptr(ref))
+        upto = n1 if out["verdict"][q] != N.V_BOUNDS else out["first_k"][q] + 1
+        for k, sl in groups.items():
+            e = np.abs(st[t, :upto, sl] - ref[:upto, sl]).max()
+            worst[k] = max(worst[k], float(e))
+            means[k].append(float(e))
+    print("FP32 vs FP64 over every state of", pick.size, "pairs: max", {k: f"{v:.2e}" for k, v in worst.items()},
+          "mean of per-pair max", {k: f"{np.mean(v):.2e}" for k, v in means.items()},
+          f"; re-checked in FP64: {c['recheck_fp64']} of {n}")
+    margin = {"position": 1.0, "velocity": 10.0, "rate": 100.0}
+    for k, m in margin.items():
+        assert worst[k] < m * wl.desc.near_eps / 10, k
     assert c["recheck_fp64"] < 0.2 * n
 
 
@@ -76,3 +103,44 @@ def test_quadrotor_goal_and_cap(need_gpu, wl):
         ora, co = O.expand(wl, nodes=wl.nodes[:64], g_parent=wl.g_parent[:64], params=p)
         assert co["goal_hits"] > 0
         assert_parity(gpu, ora, f"quad goal fp32={fp32}")
+
+
+@pytest.mark.parametrize("window,limit,overflow", [(0, 20, False), (2, 20, True), (3, 5, True), (1, 0, None)])
+def test_quadrotor_fp64_window(need_gpu, wl, window, limit, overflow):
+    """FREE pairs past a node's FP64 window skip the FP64 rollout: the
+    default window (3 * limit + 32 pairs that may be FREE) holds every cap of
+    this batch; tiny forced windows overflow, and the batch is completed by a
+    second resolution with every deferred pair rolled out.  Either way the
+    results are the oracle's."""
+    n = 128
+    p = params(wl, 1, limit_branching_factor=limit)
+    with expander_for(wl) as ex:
+        ex.debug_set_limits(fp64_window=window)
+        c = ex.expand(wl.nodes[:n], wl.g_parent[:n], p)
+        gpu = ex.fetch()
+        h = ex.health()
+    ora, co = O.expand(wl, nodes=wl.nodes[:n], g_parent=wl.g_parent[:n], params=p)
+    print("window", window, "limit", limit, "deferred", c["fp64_deferred"], "health", h)
+    assert_parity(gpu, ora, f"quad window={window} limit={limit}")
+    counts_consistent(c)
+    assert c["fp64_deferred"] > 0
+    if overflow is False:
+        assert h["window_overflows"] == 0 and h["host_completions"] == 0
+    elif overflow:
+        assert h["window_overflows"] > 0 and h["host_completions"] > 0
+
+
+def test_quadrotor_uniform_thrust_parity(need_gpu):
+    """The stated thrust distribution (each rotor U[0, 1.4] x hover per
+    piece): most pairs leave the angular-rate bounds early; BOUNDS indices,
+    collisions, the few FREE pairs and their resolution match the oracle."""
+    wl = synth.quadrotor(seed=41, P=3000, N_nodes=256, n_visited=5000)
+    p = params(wl, 1)
+    with expander_for(wl) as ex:
+        c = ex.expand(wl.nodes, wl.g_parent, p)
+        gpu = ex.fetch()
+    ora, co = O.expand(wl, params=p)
+    print("counts", c)
+    assert_parity(gpu, ora, "quad uniform thrust")
+    counts_consistent(c)
+    assert co["bounds"] > co["pairs"] // 2 and co["free_"] > 0 and co["collide"] > 0

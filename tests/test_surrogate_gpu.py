@@ -201,7 +201,7 @@ def test_train_calibrate_recall_and_gain(need_gpu):
     assert np.array_equal(sur["verdict"][keep], plain["verdict"][keep])
     assert np.array_equal(sur["first_k"][keep], plain["first_k"][keep])
     for k in ("final_state", "g", "h"):
-        assert np.array_equal(sur[k][free], plain[k][free])
+        assert np.array_equal(sur[k][free], plain[k][free], equal_nan=True)  # NaN: CAPPED
     print(f"exact checks skipped by the surrogate: {rej.sum()} of {rej.size} pairs ({100 * rej.mean():.2f}%), "
           f"{100 * rej.sum() / max(1, (~free).sum()):.2f}% of infeasible pairs; recall 1.0")
     assert c_sur["surrogate_reject"] == rej.sum()

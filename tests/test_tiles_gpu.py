@@ -38,7 +38,7 @@ def test_planar_shapes_and_autotune(need_gpu):
 
 
 def test_quadrotor_shapes_and_autotune(need_gpu):
-    wl = synth.quadrotor(seed=32, P=800, N_nodes=128, n_visited=2000)
+    wl = synth.quadrotor(seed=32, P=800, N_nodes=128, n_visited=2000, thrust="balanced")
     wl.params.fp32_rollout = 1
     with expander_for(wl) as ex:
         ex.expand(wl.nodes, wl.g_parent, wl.params)
