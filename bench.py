@@ -24,6 +24,8 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
+FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32, public spec
+FP64_PEAK_TFLOPS = 78.6   # MI355X vector FP64, public spec
 # HBM bytes per launch of the check kernel, measured with rocprofv3 --pmc
 # FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy (profiles/ holds the run).
 MEASURED_TRAFFIC = {
@@ -44,12 +46,25 @@ def progress(msg):
 def workloads():
     from dynoplan_amd import synth
     return {
+        "quadrotor": synth.quadrotor,  # thrusts U[0, 1.4] x hover per rotor and piece
+        "quadrotor_balanced": lambda: synth.quadrotor(thrust="balanced"),
         "unicycle1_scale": synth.unicycle1_scale,
         "unicycle1_small": synth.unicycle1_small,
         "car_trailer": synth.car_trailer,
-        "quadrotor": synth.quadrotor,
         "multi": synth.multi_instance,
     }
+
+
+# Kernel groups timed by dp_last_timing_detail (HIP events on the expansion
+# stream), per system family.
+KERNEL_GROUPS = {
+    "check_main": {"quad": "k_quad32", "planar": "k_check_planar"},
+    "check_post": {"quad": "k_quad_window + k_quad64 + k_quad_dupv", "planar": "(none)"},
+    "lists": {"quad": "k_cont_count + k_cont_scatter + k_cell_list + k_conf_waves (+ fills, scans)",
+              "planar": "k_cont_count + k_cont_scatter + k_cell_list + k_conf_waves (+ fills, scans)"},
+    "walk": {"quad": "k_round_init + k_pre_nodes + k_resolve_ticketed",
+             "planar": "k_round_init + k_pre_nodes + k_resolve_ticketed"},
+}
 
 
 def algorithmic_bytes(wl, prim):
@@ -74,12 +89,30 @@ def cpu_model():
     return "unknown"
 
 
+def host_cpus():
+    """CPUs this process may use: the affinity mask, capped by the cgroup
+    CPU quota when one is set; with os.cpu_count() for the record."""
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    usable = affinity if quota is None else max(1, min(affinity, int(quota)))
+    return {"os_cpu_count": os.cpu_count(), "affinity": affinity, "cgroup_quota": quota, "usable": usable,
+            "model": cpu_model()}
+
+
 def cpu_baseline(wl, n_nodes):
     """The FP64 oracle on the host cores: single-threaded (as the reference's
     loop) and with OpenMP over frontier nodes; ``value`` is the threaded one."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    cpus = host_cpus()
+    threads = cpus["usable"]
     nodes = np.ascontiguousarray(wl.nodes[:n_nodes])
     gp = np.ascontiguousarray(wl.g_parent[:n_nodes])
     O.expand(wl, nodes=nodes[:4], g_parent=gp[:4], want_out=False)  # warm-up (load, page in)
@@ -98,10 +131,10 @@ def cpu_baseline(wl, n_nodes):
     _, tn = timed(5, threads)
     progress(f"cpu oracle {threads} threads: {pairs} pairs in {tn:.3f} s")
     return {"value": pairs / tn, "unit": "pairs/s", "cores": threads, "kind": "port",
-            "single_thread_value": pairs / t1, "cpu_model": cpu_model(),
+            "single_thread_value": pairs / t1, "cpu_model": cpus["model"], "host_cpus": cpus,
             "sample": f"first {n_nodes} of {wl.nodes.shape[0]} frontier nodes ({pairs} pairs); FP64 oracle "
-                      f"(grid NN, swept-box cull) with OpenMP over nodes on {threads} threads, median of 5; "
-                      f"single-threaded median of 3 in single_thread_value"}
+                      f"(grid NN, swept-box cull) with OpenMP over nodes on all {threads} usable host CPUs, "
+                      f"median of 5; single-threaded median of 3 in single_thread_value"}
 
 
 def plain_gpu(ex, wl, n_nodes):
@@ -282,13 +315,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="unicycle1_scale", choices=sorted(workloads()))
+    ap.add_argument("--workload", default="quadrotor", choices=sorted(workloads()))
     ap.add_argument("--cpu-nodes", type=int, default=0,
-                    help="frontier nodes in the CPU baseline sample (0: 4096 planar, 512 quadrotor)")
+                    help="frontier nodes in the CPU baseline sample (0: 4096 planar, 2048 quadrotor)")
     ap.add_argument("--plain-nodes", type=int, default=256, help="nodes expanded one per call (plain path)")
     ap.add_argument("--label-pairs", type=int, default=2_000_000, help="surrogate training labels (car_trailer)")
     ap.add_argument("--no-surrogate", action="store_true", help="car_trailer without the learned pre-filter")
     ap.add_argument("--no-baselines", action="store_true", help="skip CPU and plain-GPU legs (profiling runs)")
+    ap.add_argument("--no-variants", action="store_true", help="skip the balanced-thrust quadrotor figure")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (scale reference for HBM byte counters)")
@@ -339,25 +373,46 @@ def main():
         ex.expand_async(params)
         torch.cuda.synchronize()
         progress(f"warm-up step done: {ex.timing()}")
+    health0 = ex.health()
     elapsed = time_steps(ex, params, args.steps)
+    health1 = ex.health()
+    # every timed batch must have finished on the device: no resolution wave
+    # gave up waiting and no FP64 window overflowed (either leaves work to a
+    # later dp_get_counts, outside the timed region)
+    integrity = {"batches": health1["batches"] - health0["batches"],
+                 "wave_giveups": health1["wave_giveups"] - health0["wave_giveups"],
+                 "window_overflows": health1["window_overflows"] - health0["window_overflows"]}
+    integrity["device_complete"] = integrity["wave_giveups"] == 0 and integrity["window_overflows"] == 0
+    progress(f"timed {args.steps} steps in {elapsed:.3f} s; integrity {integrity}")
     counts = ex.counts()
     pairs = counts["pairs"]
     value = pairs * args.steps / elapsed
-    progress(f"timed {args.steps} steps in {elapsed:.3f} s: {pairs} pairs per step")
 
-    # Per-stage device time from HIP events recorded on the expansion stream.
+    # Per-stage and per-kernel-group device time from HIP events recorded on
+    # the expansion stream, over as many further steps.
     stage = {"applicability": [], "surrogate": [], "check": [], "resolve": [], "total": []}
+    detail = {k: [] for k in ("applicability", "surrogate", "check_pre", "check_main", "check_post", "lists", "walk")}
     for _ in range(args.steps):
         ex.expand_async(params)
         t = ex.timing()
         for k in stage:
             stage[k].append(t[k])
+        d = ex.timing_detail()
+        for k in detail:
+            detail[k].append(d[k])
     stage_ms = {k: float(np.mean(v)) for k, v in stage.items()}
-    progress(f"stage ms {stage_ms}")
+    detail_ms = {k: float(np.mean(v)) for k, v in detail.items()}
+    progress(f"stage ms {stage_ms}; kernel groups {detail_ms}")
     out = ex.fetch()
     checked = out["verdict"] != N.V_SURROGATE_REJECT
     abytes = algorithmic_bytes(wl, out["prim"][checked])
-    achieved = abytes / (stage_ms["check"] * 1e-3) / 1e9
+    fam = "quad" if is_quad else "planar"
+    groups = {k: {"kernels": KERNEL_GROUPS[k][fam], "ms": detail_ms[k],
+                  "hbm_frac": abytes / (detail_ms[k] * 1e-3) / 1e9 / HBM_PEAK_GBS if detail_ms[k] > 0 else None}
+              for k in KERNEL_GROUPS}
+    dominant = max(groups, key=lambda k: groups[k]["ms"])
+    dom_ms = groups[dominant]["ms"]
+    achieved = abytes / (dom_ms * 1e-3) / 1e9
 
     line = {
         "metric": METRIC,
@@ -378,22 +433,21 @@ def main():
                    "limit_branching_factor": int(wl.params.limit_branching_factor), "seed": wl.seed,
                    "surrogate": use_sur, "pairs_per_step": int(pairs)},
         "counts": counts,
+        "integrity": integrity,
         "stage_ms": stage_ms,
-        "dominant_stage": max((k for k in stage_ms if k != "total"), key=lambda k: stage_ms[k]),
+        "kernel_groups": groups,
+        "dominant_kernel": KERNEL_GROUPS[dominant][fam],
         "tiles": tiles,
-        "roofline": {"bound": "hbm", "kernel": "k_quad32+k_quad64" if is_quad else "k_check_planar",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes_per_launch": abytes},
+        "roofline": {"bound": "hbm", "kernel": KERNEL_GROUPS[dominant][fam], "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": abytes, "launch_ms": dom_ms},
     }
-    if args.workload in MEASURED_TRAFFIC and not use_sur:
-        line["roofline"]["traffic"], line["roofline"]["traffic_source"] = MEASURED_TRAFFIC[args.workload]
-    if args.workload in MEASURED_RESOLVE_TRAFFIC and not use_sur:
-        rb, src = MEASURED_RESOLVE_TRAFFIC[args.workload]
-        line["resolution_bound"] = {
-            "kernels": "k_conf_waves + k_resolve_ticketed", "bound": "dependency chain (latency)",
-            "stage_ms": stage_ms["resolve"], "traffic": rb, "traffic_source": src,
-            "hbm_gbs": rb / (stage_ms["resolve"] * 1e-3) / 1e9,
-            "hbm_frac": rb / (stage_ms["resolve"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    key = args.workload
+    if key in MEASURED_TRAFFIC and not use_sur:
+        line["roofline"]["traffic"], line["roofline"]["traffic_source"] = MEASURED_TRAFFIC[key]
+    if is_quad:
+        line["fp32_vs_fp64"] = rollout_error(ex, out)
+        progress(f"FP32 vs FP64 rollout error: {line['fp32_vs_fp64']}")
     if use_sur:
         # Untimed validation: the exact path on the same batch decides every
         # pair; the surrogate run must keep every feasible edge and give the
@@ -415,17 +469,74 @@ def main():
                           "speedup_vs_batched_exact": t_exact / elapsed})
         progress(f"surrogate validation: {surrogate}")
         line["surrogate"] = surrogate
+    if args.workload == "quadrotor" and not args.no_variants:
+        # the same path on primitives that stay near hover (few pairs fail the bounds)
+        line["variants"] = {"quadrotor_balanced": variant_line(workloads()["quadrotor_balanced"](), args)}
     if not args.no_baselines:
         pg = plain_gpu(ex, wl, args.plain_nodes)
         progress(f"plain GPU path: {pg}")
         line["plain_gpu"] = pg
         line["speedup_vs_plain_gpu"] = value / pg["value"]
-        cpu_nodes = args.cpu_nodes or (512 if is_quad else 4096)
+        cpu_nodes = args.cpu_nodes or (2048 if is_quad else 4096)
         cb = cpu_baseline(wl, cpu_nodes)
         line["cpu_baseline"] = cb
         line["speedup_vs_cpu"] = value / cb["value"]
     ex.close()
     print(json.dumps(line))
+    if not integrity["device_complete"]:
+        print("bench: a timed batch needed host-launched completion; the line is not a device-only measurement",
+              file=sys.stderr)
+        sys.exit(3)
+
+
+def rollout_error(ex, out, n_sample=2000, seed=0):
+    """FP32 rollout against the library's FP64 rollout of the same pairs,
+    every state (up to the first out-of-bounds state) and every component
+    group: max and mean of the per-pair maxima."""
+    from dynoplan_amd import _native as N
+    rng = np.random.default_rng(seed)
+    n = out["prim"].size
+    pick = np.sort(rng.choice(n, min(n_sample, n), replace=False)).astype(np.int32)
+    s32 = ex.rollout_states(pick, 32)
+    s64 = ex.rollout_states(pick, 64)
+    upto = np.where(out["verdict"][pick] == N.V_BOUNDS, out["first_k"][pick] + 1, s32.shape[1])
+    mask = np.arange(s32.shape[1])[None, :] < upto[:, None]
+    err = np.abs(s32 - s64)
+    err = np.where(mask[:, :, None] & np.isfinite(err), err, 0.0)
+    res = {"pairs": int(pick.size)}
+    for name, sl in (("position_m", slice(0, 3)), ("quaternion", slice(3, 7)), ("velocity_mps", slice(7, 10)),
+                     ("rate_radps", slice(10, 13))):
+        per_pair = err[:, :, sl].max(axis=(1, 2))
+        res[name] = {"max": float(per_pair.max()), "mean": float(per_pair.mean())}
+    return res
+
+
+def variant_line(wl, args):
+    """Batched throughput of another workload in the same run (no baselines)."""
+    import copy
+    import torch
+    from dynoplan_amd.expander import expander_for
+    with expander_for(wl, device=0) as ex:
+        params = copy.copy(wl.params)
+        ex.upload_nodes(wl.nodes, wl.g_parent)
+        ex.reserve(wl.nodes.shape[0], wl.nodes.shape[0] * (wl.offsets.size - 1))
+        tiles, _ = ex.autotune(params)
+        for _ in range(args.warmup):
+            ex.expand_async(params)
+            torch.cuda.synchronize()
+        h0 = ex.health()
+        el = time_steps(ex, params, args.steps)
+        h1 = ex.health()
+        c = ex.counts()
+        ex.expand_async(params)
+        d = ex.timing_detail()
+        res = {"workload": wl.name, "value": c["pairs"] * args.steps / el, "ms_per_step": 1e3 * el / args.steps,
+               "pairs_per_step": c["pairs"], "bounds": c["bounds"], "collide": c["collide"], "free_": c["free_"],
+               "capped": c["capped"], "fp64_deferred": c["fp64_deferred"], "kernel_ms": d, "tiles": tiles,
+               "device_complete": h1["wave_giveups"] == h0["wave_giveups"]
+               and h1["window_overflows"] == h0["window_overflows"]}
+    progress(f"variant {wl.name}: {res}")
+    return res
 
 
 if __name__ == "__main__":

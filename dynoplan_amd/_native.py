@@ -208,7 +208,8 @@ SIGNATURES = {
     "dp_stream": (C.c_void_p, [C.c_void_p]),
     "dp_last_timing": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "dp_last_timing_detail": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
-    "dp_debug_fp32_states": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_double)]),
+    "dp_debug_rollout_states": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
+                                          C.POINTER(C.c_double)]),
     "dp_surrogate_get": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "dp_search_options_default": (None, [C.POINTER(SearchOptions)]),
     "dp_search": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(SearchOptions),

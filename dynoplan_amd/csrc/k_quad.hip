@@ -562,10 +562,38 @@ __global__ __launch_bounds__(64) void k_quad_window(int32_t N, const int32_t* __
   }
 }
 
-// Every FP32 state of the listed pairs of the last batch, as k_quad32 forms
-// them (the same step function), position made absolute in FP64: out
-// [n][maxlen][13], rows past a primitive's length untouched.  One thread per
-// pair; error reports only.
+// Every state of the listed pairs of the last batch: FP32 as k_quad32 forms
+// them (the same step function, position made absolute in FP64), or FP64 as
+// k_quad64 forms them; out [n][maxlen][13], rows past a primitive's length
+// untouched.  One thread per pair; error reports only.
+__global__ void k_quad64_trace(dp_quad_args a, const int32_t* __restrict__ pairs, int32_t n, int32_t maxlen,
+                               double* __restrict__ out) {
+  const dp_dev_sys& S = *a.sys;
+  for (int32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const int32_t q = pairs[t];
+    const int32_t node = a.pnode[q], j = a.pprim[q];
+    const int32_t s0 = a.off[j];
+    const int32_t L1 = a.off[j + 1] - s0;
+    const int32_t a0 = s0 - j;
+    double s[13];
+    for (int c = 0; c < 13; ++c) {
+      const double v = a.st[(int64_t)c * a.S + s0];
+      s[c] = dp_is_offset_dim(S, c) ? v + a.nodes[(int64_t)c * a.Ncap + node] : v;
+    }
+    for (int k = 0; k < L1 && k < maxlen; ++k) {
+      if (k > 0) {
+        const double4 uv = reinterpret_cast<const double4*>(a.act)[a0 + k - 1];
+        const double u[4] = {uv.x, uv.y, uv.z, uv.w};
+        double o[13];
+        dp_quad_step64(S, s, u, o);
+        for (int c = 0; c < 13; ++c) s[c] = o[c];
+      }
+      double* w = out + ((int64_t)t * maxlen + k) * 13;
+      for (int c = 0; c < 13; ++c) w[c] = s[c];
+    }
+  }
+}
+
 __global__ void k_quad32_trace(dp_quad_args a, const int32_t* __restrict__ pairs, int32_t n, int32_t maxlen,
                                double* __restrict__ out) {
   const dp_dev_sys& S = *a.sys;
@@ -666,8 +694,8 @@ int dp_launch_quad_deferred(dp_ctx* c, hipStream_t s) {
   return DP_OK;
 }
 
-extern "C" int dp_debug_fp32_states(dp_ctx* c, const int32_t* pairs, int32_t n, double* out) {
-  if (!c || (n > 0 && (!pairs || !out)) || n < 0) {
+extern "C" int dp_debug_rollout_states(dp_ctx* c, const int32_t* pairs, int32_t n, int32_t bits, double* out) {
+  if (!c || (n > 0 && (!pairs || !out)) || n < 0 || (bits != 32 && bits != 64)) {
     dp_set_error("invalid argument");
     return DP_ERR_ARG;
   }
@@ -687,7 +715,8 @@ extern "C" int dp_debug_fp32_states(dp_ctx* c, const int32_t* pairs, int32_t n, 
   if (e == hipSuccess) {
     const dp_expand_params p = c->round_params;
     dp_quad_args a = make_quad_args(c, &p, 0.0, 0.0);
-    k_quad32_trace<<<dp_grid_blocks(n, 256, 1024), 256, 0, c->stream>>>(a, d_pairs, n, c->maxlen, d_out);
+    if (bits == 32) k_quad32_trace<<<dp_grid_blocks(n, 256, 1024), 256, 0, c->stream>>>(a, d_pairs, n, c->maxlen, d_out);
+    else k_quad64_trace<<<dp_grid_blocks(n, 256, 1024), 256, 0, c->stream>>>(a, d_pairs, n, c->maxlen, d_out);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

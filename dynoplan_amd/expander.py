@@ -59,6 +59,8 @@ class GpuExpander:
         p.nu = actions.shape[1] if actions is not None and actions.ndim == 2 else 0
         p.offsets = _iptr(offsets)
         p.states = _dptr(states)
+        P = offsets.size - 1 if max_motions <= 0 else min(int(max_motions), offsets.size - 1)
+        self._maxlen = int(np.diff(offsets[: P + 1]).max()) if P > 0 else 0
         if actions is not None:
             actions = np.ascontiguousarray(actions, np.float64)
             p.actions = _dptr(actions)
@@ -154,6 +156,15 @@ class GpuExpander:
         N.check(self.lib.dp_last_timing_detail(self.ctx, ms))
         keys = ("applicability", "surrogate", "check_pre", "check_main", "check_post", "lists", "walk")
         return dict(zip(keys, [float(x) for x in ms]))
+
+    def rollout_states(self, pairs, bits):
+        """Every rollout state of the listed pairs of the last quadrotor batch,
+        FP32 (bits=32) or FP64 (bits=64): [n][L][13], NaN past each length."""
+        pairs = np.ascontiguousarray(pairs, np.int32)
+        L = int(self._maxlen)
+        out = np.full((pairs.size, L, 13), np.nan)
+        N.check(self.lib.dp_debug_rollout_states(self.ctx, _iptr(pairs), pairs.size, int(bits), _dptr(out)))
+        return out
 
     def tiles(self):
         t = N.TileConfig()

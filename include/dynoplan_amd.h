@@ -394,12 +394,13 @@ int dp_debug_gemm_bf16(dp_ctx* ctx, const float* A, const float* B, int32_t M, i
  * used to report the FP32-vs-FP64 rollout error per system. */
 int dp_debug_fp32_positions(dp_ctx* ctx, float* out, int64_t n);
 
-/* Quadrotor FP32 rollout error report, every state: the FP32 states of the
- * listed pairs (indices in the last dp_expand's canonical pair order) as
- * the FP32 pass forms them, positions made absolute in FP64, into
- * out[n][L][13] with L the longest primitive's state count (rows past a
- * pair's own length untouched).  No counterpart in the reference. */
-int dp_debug_fp32_states(dp_ctx* ctx, const int32_t* pairs, int32_t n, double* out);
+/* Quadrotor rollout error report, every state: the states of the listed
+ * pairs (indices in the last dp_expand's canonical pair order) as the FP32
+ * pass forms them (bits = 32, positions made absolute in FP64) or as the
+ * FP64 pass forms them (bits = 64), into out[n][L][13] with L the longest
+ * primitive's state count (rows past a pair's own length untouched).  No
+ * counterpart in the reference. */
+int dp_debug_rollout_states(dp_ctx* ctx, const int32_t* pairs, int32_t n, int32_t bits, double* out);
 
 /* Diagnostics of the duplicate resolution (no counterpart in the reference):
  * enable > 0 records, for each node of later batches, its wave's timeline;

@@ -61,9 +61,8 @@ def test_quadrotor_fp32_error_and_rechecks(need_gpu, wl, thrust):
         n = c["pairs"]
         pick = np.sort(rng.choice(n, 3000, replace=False)).astype(np.int32)
         L = int(np.diff(wl.offsets).max())
-        st = np.full((pick.size, L, 13), np.nan)
-        N.check(ex.lib.dp_debug_fp32_states(ex.ctx, pick.ctypes.data_as(C.POINTER(C.c_int32)), pick.size,
-                                            st.ctypes.data_as(C.POINTER(C.c_double))))
+        st = ex.rollout_states(pick, 32)
+        st64 = ex.rollout_states(pick, 64)
     lib = O.load()
     pr = O.prims_struct(np.ascontiguousarray(wl.offsets, np.int32), np.ascontiguousarray(wl.states),
                         np.ascontiguousarray(wl.actions))
@@ -75,6 +74,8 @@ def test_quadrotor_fp32_error_and_rechecks(need_gpu, wl, thrust):
         node = np.ascontiguousarray(wl.nodes[out["node"][q]])
         n1 = lib.oracle_transform_primitive(C.byref(wl.desc), C.byref(pr), int(out["prim"][q]), O.dptr(node),
                                             O.dptr(ref))
+        # the library's FP64 rollout is the oracle's, bit for bit
+        assert np.array_equal(st64[t, :n1].view(np.int64), ref[:n1].view(np.int64)), t
         upto = n1 if out["verdict"][q] != N.V_BOUNDS else out["first_k"][q] + 1
         for k, sl in groups.items():
             e = np.abs(st[t, :upto, sl] - ref[:upto, sl]).max()
