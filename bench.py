@@ -533,8 +533,9 @@ def variant_line(wl, args):
         res = {"workload": wl.name, "value": c["pairs"] * args.steps / el, "ms_per_step": 1e3 * el / args.steps,
                "pairs_per_step": c["pairs"], "bounds": c["bounds"], "collide": c["collide"], "free_": c["free_"],
                "capped": c["capped"], "fp64_deferred": c["fp64_deferred"], "kernel_ms": d, "tiles": tiles,
-               "device_complete": h1["wave_giveups"] == h0["wave_giveups"]
-               and h1["window_overflows"] == h0["window_overflows"]}
+               "integrity": {k: h1[k] - h0[k] for k in ("batches", "wave_giveups", "window_overflows")}}
+        res["integrity"]["device_complete"] = (res["integrity"]["wave_giveups"] == 0
+                                               and res["integrity"]["window_overflows"] == 0)
     progress(f"variant {wl.name}: {res}")
     return res
 

@@ -268,6 +268,7 @@ void dp_destroy(dp_ctx* c) {
   dev_free(&c->d_node_count);
   dev_free(&c->d_chunk);
   dev_free(&c->d_dbg32);
+  dev_free(&c->d_q64);
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
   c->h_pinned = nullptr;
   dev_free(&c->d_counters);
@@ -575,6 +576,7 @@ int dp_reserve(dp_ctx* c, int32_t max_nodes, int64_t max_pairs) {
     c->conf_pool_cap = 2 * max_pairs;  // about 1 in 10 lists overflows, by ~15 entries, on dense batches
     if ((rc = dev_alloc(&c->d_conf_pool, c->conf_pool_cap))) return rc;
     if (c->desc.system == DP_SYS_QUADROTOR && (rc = dev_alloc(&c->d_dbg32, 3 * max_pairs))) return rc;
+    if (c->desc.system == DP_SYS_QUADROTOR && (rc = dev_alloc(&c->d_q64, max_pairs))) return rc;
     c->pair_cap = max_pairs;
   }
   return DP_OK;

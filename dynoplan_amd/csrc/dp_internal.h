@@ -24,7 +24,7 @@
 // Pair flag (limit_branching_factor == 0): a FREE pair after its node's
 // first FREE pair, CAPPED before the walk.
 #define DP_F_PRECAP 0x40
-// FP64 window of a quadrotor node, in pairs that may be FREE: enough for the
+// FP64 window of a quadrotor node, in FREE pairs: enough for the
 // cap of limit NOVEL pairs with a wide margin for duplicates.
 #define DP_FP64_WINDOW(limit) (3 * (limit) + 32)
 // Pad added to swept boxes of the broad phase (m): bounds the rounding of
@@ -57,7 +57,7 @@ void dp_set_error(const std::string& msg);
 enum {
   DPC_PAIRS = 0, DPC_BOUNDS, DPC_COLLIDE, DPC_FREE, DPC_SURR, DPC_NOVEL, DPC_DUPV, DPC_DUPB,
   DPC_CAPPED, DPC_GOAL, DPC_CHECKED, DPC_RECHECK, DPC_ROUNDS, DPC_ERRFLAG, DPC_OVERFLOW,
-  DPC_NODES_LEFT, DPC_CONF, DPC_TICKET, DPC_CELLS, DPC_WINDOW_OVF, DPC_DEFERRED, DPC_COUNT
+  DPC_NODES_LEFT, DPC_CONF, DPC_TICKET, DPC_CELLS, DPC_WINDOW_OVF, DPC_DEFERRED, DPC_Q64, DPC_COUNT
 };
 // Sticky counters (never reset by a batch): resolutions finished by
 // host-launched rounds after a wave gave up, FP64 window overflows.
@@ -234,6 +234,7 @@ struct dp_ctx {
   int32_t rounds = 32;              // resolution rounds launched per batch
   int32_t* d_chunk = nullptr;       // [N_cap+1] quadrotor: 64-pair chunks per node, scanned
   float* d_dbg32 = nullptr;         // [3][pair_cap] quadrotor: FP32 final positions (error report)
+  int32_t* d_q64 = nullptr;         // [pair_cap] quadrotor: pairs the FP64 pass rolls out (DPC_Q64 of them)
   double prim_reach = 0.0;          // max over primitives of max_k |p_k - p_0|_inf
   double reach_box[4] = {0, 0, 0, 0};  // planar: union of the canonical swept boxes (lo x, hi x, lo y, hi y)
   int16_t* d_node_obs = nullptr;    // [N_cap][DP_NODE_OBS] obstacles within a node's reach box
@@ -329,7 +330,7 @@ int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, doub
 // FP64 rollout of the pairs deferred past their node's FP64 window (after a
 // window overflow), then their visited-set duplicate test.
 int dp_launch_quad_deferred(dp_ctx* c, hipStream_t s);
-// FP64 window of the batch in pairs that may be FREE (INT32_MAX: none).
+// FP64 window of the batch in FREE pairs (INT32_MAX: none).
 int32_t dp_fp64_window(const dp_ctx* c, int32_t limit);
 
 // ---- k_mlp.hip ----------------------------------------------------------

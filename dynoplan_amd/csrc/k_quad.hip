@@ -70,6 +70,7 @@ struct dp_quad_args {
   double goal_bound, r_dup, cdf;
   int32_t exact;
   int32_t redo;           // k_quad64: only the pairs deferred past their FP64 window
+  int32_t* q64;           // FP32 mode: the pairs of the FP64 pass (counters[DPC_Q64] of them)
   long long* counters;
 };
 
@@ -381,11 +382,21 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
 }
 
 // ---- FP64 pass -------------------------------------------------------------------
+// Pairs of the FP64 pass: in FP32 mode the compact list k_quad_window built
+// (every lane of a wave busy), else every pair of the batch.
+__device__ __forceinline__ int64_t quad64_count(const dp_quad_args& a) {
+  return a.exact || a.redo ? (int64_t)*a.total : (int64_t)a.counters[DPC_Q64];
+}
+__device__ __forceinline__ int64_t quad64_pair(const dp_quad_args& a, int64_t t) {
+  return a.exact || a.redo ? t : (int64_t)a.q64[t];
+}
+
 __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
   const dp_dev_sys& S = *a.sys;
-  const int32_t tot = *a.total;
+  const int64_t n64 = quad64_count(a);
   long long c_b = 0, c_c = 0, c_f = 0, c_k = 0, c_r = 0;
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n64; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = quad64_pair(a, t);
     int8_t fl = a.exact ? (int8_t)QF_NEAR : a.flags[q];
     if (a.redo) fl = (fl & DP_F_DEFER) ? (int8_t)QF_NEED64 : (int8_t)0;
     if (!(fl & (QF_NEAR | QF_NEED64))) continue;
@@ -493,9 +504,10 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
 // rollout so that neither kernel's registers limit the other's occupancy.
 __global__ __launch_bounds__(256) void k_quad_dupv(dp_quad_args a) {
   const dp_dev_sys& S = *a.sys;
-  const int32_t tot = *a.total;
+  const int64_t n64 = quad64_count(a);
   const double T = dp_pos_bound(S, a.r_dup);
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n64; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t q = quad64_pair(a, t);
     if (a.verdict[q] != DP_V_FREE || (a.flags[q] & DP_F_DEFER)) continue;
     double x[13];
 #pragma unroll
@@ -517,9 +529,10 @@ __global__ void k_quad_chunks(int32_t N, const int32_t* __restrict__ app, const 
   }
 }
 
-// FP64 window of each node: the pairs that may be FREE after the FP32 pass
-// (FREE, or near contact and re-checked in FP64) are ranked in canonical
-// order, and a FREE pair at rank >= window is deferred: no FP64 rollout, no
+// FP64 window of each node: the pairs the FP32 pass found FREE are ranked in
+// canonical order (near-contact pairs, re-checked in FP64 whatever their
+// place, are not counted), so a pair's rank never exceeds its rank among
+// the node's FREE pairs; a FREE pair at rank >= window is deferred: no FP64 rollout, no
 // final state, goal index, g or h unless its node's cap does not fall before
 // it (the resolution then reports a window overflow and the batch is
 // resolved again with every deferred pair rolled out).  The reference
@@ -529,7 +542,8 @@ __global__ __launch_bounds__(64) void k_quad_window(int32_t N, const int32_t* __
                                                     const int32_t* __restrict__ total, int8_t* __restrict__ verdict,
                                                     int32_t* __restrict__ firstk, int32_t* __restrict__ goal_out,
                                                     int8_t* __restrict__ res, int8_t* __restrict__ flags,
-                                                    int32_t window, long long* __restrict__ counters) {
+                                                    int32_t window, int32_t* __restrict__ q64,
+                                                    long long* __restrict__ counters) {
   const int32_t node = blockIdx.x;
   if (node >= N) return;
   const int lane = threadIdx.x;
@@ -542,14 +556,24 @@ __global__ __launch_bounds__(64) void k_quad_window(int32_t N, const int32_t* __
   for (int32_t cb = b; cb < e; cb += 64) {
     const int32_t p = cb + lane;
     const int8_t fl = p < e ? flags[p] : (int8_t)0;
-    const uint64_t m = __ballot((fl & (QF_NEAR | QF_NEED64)) != 0);
-    if ((fl & QF_NEED64) && rank + __popcll(m & lt) >= window) {
+    const uint64_t m = __ballot((fl & QF_NEED64) != 0);
+    const bool defer = (fl & QF_NEED64) && rank + __popcll(m & lt) >= window;
+    if (defer) {
       flags[p] = (int8_t)DP_F_DEFER;
       verdict[p] = DP_V_FREE;
       firstk[p] = -1;
       goal_out[p] = -1;
       res[p] = DP_R_UNDECIDED;
       ++nd;
+    }
+    // the others go to the FP64 pass's compact list
+    const bool need = (fl & (QF_NEAR | QF_NEED64)) != 0 && !defer;
+    const uint64_t nm = __ballot(need);
+    if (nm) {
+      int32_t base = 0;
+      if (lane == 0) base = (int32_t)atomicAdd((unsigned long long*)&counters[DPC_Q64], (unsigned long long)__popcll(nm));
+      base = __shfl(base, 0, 64);
+      if (need) q64[base + __popcll(nm & lt)] = p;
     }
     rank += __popcll(m);
   }
@@ -671,6 +695,7 @@ dp_quad_args make_quad_args(dp_ctx* c, const dp_expand_params* p, double goal_bo
   a.cdf = p->cost_delta_factor;
   a.exact = p->fp32_rollout ? 0 : 1;
   a.redo = 0;
+  a.q64 = c->d_q64;
   a.counters = c->d_counters;
   return a;
 }
@@ -685,10 +710,9 @@ int32_t dp_fp64_window(const dp_ctx* c, int32_t limit) {
 int dp_launch_quad_deferred(dp_ctx* c, hipStream_t s) {
   const dp_expand_params* p = &c->round_params;
   dp_quad_args a = make_quad_args(c, p, p->delta_factor_goal * (double)p->delta, c->round_r_dup);
-  a.redo = 1;
+  a.redo = 1;  // both kernels over every pair of the batch
   const int t64 = c->tiles.quad64_threads;
   k_quad64<<<dp_grid_blocks(c->pair_cap, t64, c->tiles.quad64_blocks), t64, 0, s>>>(a);
-  a.redo = 0;
   if (c->nv > 0) k_quad_dupv<<<dp_grid_blocks(c->pair_cap, 256, 8192), 256, 0, s>>>(a);
   DP_HIP_RET(hipGetLastError());
   return DP_OK;
@@ -743,9 +767,9 @@ int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, doub
     k_quad32<<<dp_grid_blocks(max_chunks, kQWaves, c->tiles.quad32_blocks), 256, lds, s>>>(a);
     if ((rc = dp_record_event(c, 6, s))) return rc;
     const int32_t window = dp_fp64_window(c, p->limit_branching_factor);
-    if (window < INT32_MAX && c->N > 0)
+    if (c->N > 0)
       k_quad_window<<<c->N, 64, 0, s>>>(c->N, c->d_app, c->d_total, c->d_verdict, c->d_firstk, c->d_goal, c->d_res,
-                                        c->d_flags, window, c->d_counters);
+                                        c->d_flags, window, c->d_q64, c->d_counters);
   } else {
     int rc = dp_record_event(c, 5, s);
     if (!rc) rc = dp_record_event(c, 6, s);

@@ -109,7 +109,7 @@ def test_quadrotor_goal_and_cap(need_gpu, wl):
 @pytest.mark.parametrize("window,limit,overflow", [(0, 20, False), (2, 20, True), (3, 5, True), (1, 0, None)])
 def test_quadrotor_fp64_window(need_gpu, wl, window, limit, overflow):
     """FREE pairs past a node's FP64 window skip the FP64 rollout: the
-    default window (3 * limit + 32 pairs that may be FREE) holds every cap of
+    default window (3 * limit + 32 FREE pairs) holds every cap of
     this batch; tiny forced windows overflow, and the batch is completed by a
     second resolution with every deferred pair rolled out.  Either way the
     results are the oracle's."""
@@ -145,3 +145,25 @@ def test_quadrotor_uniform_thrust_parity(need_gpu):
     assert_parity(gpu, ora, "quad uniform thrust")
     counts_consistent(c)
     assert co["bounds"] > co["pairs"] // 2 and co["free_"] > 0 and co["collide"] > 0
+
+
+@pytest.mark.parametrize("thrust", ["uniform", "balanced"])
+def test_quadrotor_full_size_batch(need_gpu, thrust):
+    """The quadrotor batch at full size (N=2048, P=10k, 100 boxes + 50
+    spheres, V=20k): every pair against the oracle, and the device pipeline
+    finishes every batch by itself (no FP64 window overflow, no resolution
+    wave giving up) over repeated launches."""
+    wl = synth.quadrotor(thrust=thrust)
+    with expander_for(wl) as ex:
+        ex.upload_nodes(wl.nodes, wl.g_parent)
+        h0 = ex.health()
+        for _ in range(5):
+            ex.expand_async(wl.params)
+        h1 = ex.health()
+        c = ex.counts()
+        gpu = ex.fetch()
+    print(thrust, "counts", c, "health", h0, h1)
+    assert h1["window_overflows"] == h0["window_overflows"] and h1["wave_giveups"] == h0["wave_giveups"]
+    ora, co = O.expand(wl, threads=16)
+    assert_parity(gpu, ora, f"quad full {thrust}")
+    counts_consistent(c)
