@@ -244,6 +244,7 @@ void dp_destroy(dp_ctx* c) {
   dev_free(&c->d_nodes);
   dev_free(&c->d_gpar);
   dev_free(&c->d_app);
+  dev_free(&c->d_app_mask);
   dev_free(&c->d_node_novel);
   dev_free(&c->d_pnode);
   dev_free(&c->d_pprim);
@@ -269,6 +270,12 @@ void dp_destroy(dp_ctx* c) {
   dev_free(&c->d_chunk);
   dev_free(&c->d_dbg32);
   dev_free(&c->d_q64);
+  dev_free(&c->d_rot64);
+  dev_free(&c->d_dv64);
+  dev_free(&c->d_dv32);
+  dev_free(&c->d_rbad64);
+  dev_free(&c->d_rbad32);
+  dev_free(&c->d_rnear32);
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
   c->h_pinned = nullptr;
   dev_free(&c->d_counters);
@@ -442,6 +449,17 @@ int dp_upload_primitives(dp_ctx* c, const dp_prims* pr, int32_t max_motions) {
   c->maxlen = maxlen;
   c->h_off = off;
   c->goal_num = -1;
+  if (quad) {
+    // attitude / rate trajectories shared by every pair of a primitive
+    if ((rc = dev_alloc(&c->d_rot64, (int64_t)7 * S))) return rc;
+    if ((rc = dev_alloc(&c->d_dv64, (int64_t)3 * S))) return rc;
+    if ((rc = dev_alloc(&c->d_dv32, (int64_t)3 * S))) return rc;
+    if ((rc = dev_alloc(&c->d_rbad64, P))) return rc;
+    if ((rc = dev_alloc(&c->d_rbad32, P))) return rc;
+    if ((rc = dev_alloc(&c->d_rnear32, P))) return rc;
+    if ((rc = dp_launch_quad_rot(c, c->stream))) return rc;
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  }
   return DP_OK;
 }
 
@@ -628,6 +646,7 @@ int dp_size_pairs(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   const int64_t kOutright = 1ll << 22;
   int rc = dp_reserve(c, c->N, std::min(worst, kOutright));
   if (rc) return rc;
+  if ((rc = dp_ensure_u64(&c->d_app_mask, &c->app_mask_cap, (int64_t)c->N * ((c->P + 63) / 64)))) return rc;
   if (worst <= c->pair_cap) return DP_OK;
   if ((rc = dp_launch_count(c, p, s))) return rc;
   long long total = 0;

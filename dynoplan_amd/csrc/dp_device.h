@@ -296,12 +296,17 @@ DP_HD void dp_quad_cos_sinc(const dp_dev_sys& S, double z, double* c, double* si
   *sinc = ps;
 }
 
-DP_HD void dp_quad_step64(const dp_dev_sys& S, const double* s, const double* u, double* o) {
+// Attitude and angular-rate part of the step, and the velocity increment
+// dt * (thrust acceleration) it implies.  It reads only the quaternion and
+// the angular rate (s[3..6], s[10..12]) and the rotors, so for a primitive
+// rolled out from any translated start it is the same sequence: it is
+// evaluated once per primitive and shared by every pair.
+// Writes o[3..6], o[10..12] and dv[3] (o[7 + c] = s[7 + c] + dv[c]).
+DP_HD void dp_quad_rot_step64(const dp_dev_sys& S, const double* s, const double* u, double* o, double* dv) {
   const double m = S.quad[0], g = S.quad[1], arm = S.quad[2], t2t = S.quad[3];
   const double Jx = S.quad[4], Jy = S.quad[5], Jz = S.quad[6];
   const double dt = S.dt;
   const double qx = s[3], qy = s[4], qz = s[5], qw = s[6];
-  const double vx = s[7], vy = s[8], vz = s[9];
   const double wx = s[10], wy = s[11], wz = s[12];
   double g4 = g * 0.25;
   double usum = ((u[0] + u[1]) + u[2]) + u[3];
@@ -323,12 +328,9 @@ DP_HD void dp_quad_step64(const dp_dev_sys& S, const double* s, const double* u,
   double dwx = (tx - cx) / Jx;
   double dwy = (ty - cy) / Jy;
   double dwz = (tz - cz) / Jz;
-  o[0] = s[0] + dt * vx;
-  o[1] = s[1] + dt * vy;
-  o[2] = s[2] + dt * vz;
-  o[7] = vx + dt * ax;
-  o[8] = vy + dt * ay;
-  o[9] = vz + dt * az;
+  dv[0] = dt * ax;
+  dv[1] = dt * ay;
+  dv[2] = dt * az;
   o[10] = wx + dt * dwx;
   o[11] = wy + dt * dwy;
   o[12] = wz + dt * dwz;
@@ -348,6 +350,23 @@ DP_HD void dp_quad_step64(const dp_dev_sys& S, const double* s, const double* u,
   o[4] = ny * inv;
   o[5] = nz * inv;
   o[6] = nw * inv;
+}
+
+// Translation part of the step: explicit Euler with the old velocity, then
+// the velocity increment of dp_quad_rot_step64.
+DP_HD void dp_quad_trans_step64(double dt, const double* s, const double* dv, double* o) {
+  o[0] = s[0] + dt * s[7];
+  o[1] = s[1] + dt * s[8];
+  o[2] = s[2] + dt * s[9];
+  o[7] = s[7] + dv[0];
+  o[8] = s[8] + dv[1];
+  o[9] = s[9] + dv[2];
+}
+
+DP_HD void dp_quad_step64(const dp_dev_sys& S, const double* s, const double* u, double* o) {
+  double dv[3];
+  dp_quad_rot_step64(S, s, u, o, dv);
+  dp_quad_trans_step64(S.dt, s, dv, o);
 }
 
 __device__ __forceinline__ uint64_t dp_ballot(bool p) { return __ballot(p); }

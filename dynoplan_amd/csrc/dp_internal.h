@@ -199,6 +199,8 @@ struct dp_ctx {
   double* d_nodes = nullptr;  // [nx][N_cap]
   double* d_gpar = nullptr;   // [N_cap]
   int32_t* d_app = nullptr;   // [N_cap+1] counts, then exclusive offsets
+  uint64_t* d_app_mask = nullptr;  // [N][ceil(P/64)] applicable primitives per node
+  int64_t app_mask_cap = 0;
 
   // pairs
   int64_t pair_cap = 0;
@@ -235,6 +237,13 @@ struct dp_ctx {
   int32_t* d_chunk = nullptr;       // [N_cap+1] quadrotor: 64-pair chunks per node, scanned
   float* d_dbg32 = nullptr;         // [3][pair_cap] quadrotor: FP32 final positions (error report)
   int32_t* d_q64 = nullptr;         // [pair_cap] quadrotor: pairs the FP64 pass rolls out (DPC_Q64 of them)
+  // quadrotor, per primitive state: attitude / rate trajectory and velocity
+  // increments shared by every pair (k_quad_rot); per primitive: first state
+  // whose attitude / rate leaves the bounds (FP64, FP32) or the FP32 margin
+  double* d_rot64 = nullptr;        // [7][S]
+  double* d_dv64 = nullptr;         // [3][S]
+  float* d_dv32 = nullptr;          // [3][S]
+  int32_t *d_rbad64 = nullptr, *d_rbad32 = nullptr, *d_rnear32 = nullptr;  // [P]
   double prim_reach = 0.0;          // max over primitives of max_k |p_k - p_0|_inf
   double reach_box[4] = {0, 0, 0, 0};  // planar: union of the canonical swept boxes (lo x, hi x, lo y, hi y)
   int16_t* d_node_obs = nullptr;    // [N_cap][DP_NODE_OBS] obstacles within a node's reach box
@@ -297,6 +306,7 @@ inline int dp_record_event(dp_ctx* c, int k, hipStream_t s) {
   return DP_OK;
 }
 int dp_ensure_f64(double** p, int64_t* cap, int64_t need);
+int dp_ensure_u64(uint64_t** p, int64_t* cap, int64_t need);
 
 // ---- k_util.hip ---------------------------------------------------------
 void dp_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
@@ -330,6 +340,8 @@ int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, doub
 // FP64 rollout of the pairs deferred past their node's FP64 window (after a
 // window overflow), then their visited-set duplicate test.
 int dp_launch_quad_deferred(dp_ctx* c, hipStream_t s);
+// Per-primitive attitude / rate trajectories (after a primitive upload).
+int dp_launch_quad_rot(dp_ctx* c, hipStream_t s);
 // FP64 window of the batch in FREE pairs (INT32_MAX: none).
 int32_t dp_fp64_window(const dp_ctx* c, int32_t limit);
 

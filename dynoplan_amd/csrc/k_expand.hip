@@ -1,11 +1,12 @@
 // k_expand.hip -- gfx950 kernels of the batched db-A* expansion.
 //
 // Pipeline of one dp_expand (all on one stream, no host round trip):
-//   k_app<count>      one wave per node: applicability query nearestR(c, alpha*delta)
+//   k_app_count       one wave per node: applicability query nearestR(c, alpha*delta)
 //                     over primitive start states (Expander::expand_lazy,
-//                     dbastar.hpp:270-312), brute force, ballot-counted
+//                     dbastar.hpp:270-312), brute force with exact early
+//                     rejections, one bit mask per 64 primitives, counted
 //   k_scan            exclusive scan -> canonical pair offsets per node
-//   k_app<fill>       same scan, ballot-compacted (node, primitive) pair list
+//   k_app_fill        the masks as the (node, primitive) pair list
 //   k_check_planar    LPP lanes per pair, one lane per primitive state:
 //                     translate (transform_primitive), is_state_valid on every
 //                     state, swept-box cull of LDS-staged obstacles, SAT
@@ -17,6 +18,8 @@
 //                     based duplicate / branching-cap resolution (:856-918)
 #include <hip/hip_runtime.h>
 #include <limits.h>
+
+#include <cmath>
 
 #include "dp_internal.h"
 
@@ -33,40 +36,90 @@ __device__ __forceinline__ void wave_lds_sync() {
 // ------------------------------------------------------------------------
 // applicability (T_m->nearestR over primitive start states)
 // ------------------------------------------------------------------------
-template <int NX, bool FILL>
-__global__ __launch_bounds__(64) void k_app(const dp_dev_sys* __restrict__ Sp, const double* __restrict__ nodes,
-                                            int32_t Ncap, int32_t N, const double* __restrict__ start, int32_t P,
-                                            double r_app, int32_t* __restrict__ app, int32_t* __restrict__ pnode,
-                                            int32_t* __restrict__ pprim, int64_t pair_cap) {
+// Exact rejections ahead of dp_within for the applicability scan: each test
+// shows d(a, b) > r from one metric term (the terms are non-negative and
+// IEEE addition is monotone), reading a few components only.
+struct app_bounds {
+  double T_p, T_v, T_w;  // squared-sum bounds of dp_pos_bound's form for the position,
+                         // velocity and rate terms (quadrotor; INFINITY when unused)
+  double c_q;            // quadrotor: |q_a . q_b| < c_q implies w1 * angle > r
+};
+
+template <int NX>
+__device__ __forceinline__ bool app_reject(const dp_dev_sys& S, const app_bounds& B, const double* a,
+                                           const double* start, int32_t P, int32_t j, double r) {
+  if constexpr (NX == 13) {
+    double dot = a[3] * start[(int64_t)3 * P + j];
+    dot = dot + a[4] * start[(int64_t)4 * P + j];
+    dot = dot + a[5] * start[(int64_t)5 * P + j];
+    dot = dot + a[6] * start[(int64_t)6 * P + j];
+    if (fabs(dot) < B.c_q) return true;
+    double s = 0.0;
+#pragma unroll
+    for (int c = 10; c < 13; ++c) {
+      const double d = a[c] - start[(int64_t)c * P + j];
+      s = c == 10 ? d * d : s + d * d;
+    }
+    return s > B.T_w;
+  } else {
+    // the heading term (unicycles, car): w1 * so2 alone
+    return S.w[1] * dp_so2(a[2], start[(int64_t)2 * P + j]) > r;
+  }
+}
+
+// One wave per node, 64 primitives per step: the applicable ones are
+// recorded as a bit mask per 64 primitives (mask[i * words + j / 64]) and
+// counted; the fill pass turns the masks into the (node, primitive) pair
+// list without evaluating the metric again.
+template <int NX>
+__global__ __launch_bounds__(64) void k_app_count(const dp_dev_sys* __restrict__ Sp, app_bounds B,
+                                                  const double* __restrict__ nodes, int32_t Ncap, int32_t N,
+                                                  const double* __restrict__ start, int32_t P, double r_app,
+                                                  int32_t* __restrict__ app, uint64_t* __restrict__ mask) {
   const dp_dev_sys& S = *Sp;
   const int32_t i = blockIdx.x;
   if (i >= N) return;
   const int lane = threadIdx.x;
+  const int32_t words = (P + 63) / 64;
   double a[NX];
 #pragma unroll
   for (int c = 0; c < NX; ++c) a[c] = dp_is_offset_dim(S, c) ? 0.0 : nodes[(int64_t)c * Ncap + i];
-  int64_t base = FILL ? (int64_t)app[i] : 0;
   int32_t cnt = 0;
   for (int32_t j0 = 0; j0 < P; j0 += 64) {
     const int32_t j = j0 + lane;
     bool hit = false;
-    if (j < P) {
+    if (j < P && !app_reject<NX>(S, B, a, start, P, j, r_app)) {
       double b[NX];
 #pragma unroll
       for (int c = 0; c < NX; ++c) b[c] = start[(int64_t)c * P + j];
       hit = dp_within(S, a, b, r_app);
     }
     const uint64_t m = __ballot(hit);
-    if (FILL && hit) {
-      const int64_t q = base + cnt + __popcll(m & ((1ull << lane) - 1ull));
-      if (q < pair_cap) {
-        pnode[q] = i;
-        pprim[q] = j;
-      }
-    }
+    if (lane == 0) mask[(int64_t)i * words + j0 / 64] = m;
     cnt += __popcll(m);
   }
-  if (!FILL && lane == 0) app[i] = cnt;
+  if (lane == 0) app[i] = cnt;
+}
+
+__global__ __launch_bounds__(64) void k_app_fill(int32_t N, int32_t P, const int32_t* __restrict__ app,
+                                                 const uint64_t* __restrict__ mask, int32_t* __restrict__ pnode,
+                                                 int32_t* __restrict__ pprim, int64_t pair_cap) {
+  const int32_t i = blockIdx.x;
+  if (i >= N) return;
+  const int lane = threadIdx.x;
+  const int32_t words = (P + 63) / 64;
+  int64_t q0 = app[i];
+  for (int32_t w = 0; w < words; ++w) {
+    const uint64_t m = mask[(int64_t)i * words + w];
+    if ((m >> lane) & 1ull) {
+      const int64_t q = q0 + __popcll(m & ((1ull << lane) - 1ull));
+      if (q < pair_cap) {
+        pnode[q] = i;
+        pprim[q] = w * 64 + lane;
+      }
+    }
+    q0 += __popcll(m);
+  }
 }
 
 }  // namespace
@@ -380,13 +433,34 @@ static int dispatch_nx(int nx, F&& f) {
 
 // Applicability count only: the exact number of pairs of the uploaded nodes
 // lands in counters[DPC_PAIRS] (buffers are sized from it).
+// Exact rejection bounds of the applicability scan for radius r.
+static app_bounds make_app_bounds(const dp_ctx* c, double r) {
+  app_bounds B{INFINITY, INFINITY, INFINITY, -1.0};
+  const dp_system_desc& d = c->desc;
+  auto sq = [&](double w) {
+    if (!(w > 0.0)) return (double)INFINITY;
+    const double t = r / w;
+    return t * t * (1.0 + 1e-9);
+  };
+  B.T_p = sq(d.dist_w[0]);
+  if (d.system == DP_SYS_QUADROTOR) {
+    B.T_v = sq(d.dist_w[2]);
+    B.T_w = sq(d.dist_w[3]);
+    // w1 * acos(|dot|) > r when |dot| < cos(r / w1); the 1e-9 margin covers
+    // libm's cos and the few ulps of the device's acos series
+    if (d.dist_w[1] > 0.0 && r / d.dist_w[1] < 1.5) B.c_q = std::cos(r / d.dist_w[1]) - 1e-9;
+  }
+  return B;
+}
+
 int dp_launch_count(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   const double r_app = (double)(p->alpha * p->delta);  // dbastar.cpp:585-586 (float product)
   const int32_t N = c->N;
+  const app_bounds B = make_app_bounds(c, r_app);
   return dispatch_nx(c->desc.nx, [&](auto NXc) -> int {
     constexpr int NX = decltype(NXc)::value;
-    k_app<NX, false><<<N, 64, 0, s>>>(c->d_sys, c->d_nodes, c->N_cap, N, c->d_start, c->P, r_app, c->d_app,
-                                     nullptr, nullptr, 0);
+    k_app_count<NX><<<N, 64, 0, s>>>(c->d_sys, B, c->d_nodes, c->N_cap, N, c->d_start, c->P, r_app, c->d_app,
+                                     c->d_app_mask);
     dp_scan_small(c->d_app, N, c->d_total, c->pair_cap, c->d_counters, DPC_PAIRS, s);
     DP_HIP_RET(hipGetLastError());
     return DP_OK;
@@ -413,13 +487,13 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
 
   DP_EV(c->ev[0], s);
   dp_zero_counters(c->d_counters, s);
+  const app_bounds B = make_app_bounds(c, r_app);
   int rc = dispatch_nx(nx, [&](auto NXc) -> int {
     constexpr int NX = decltype(NXc)::value;
-    k_app<NX, false><<<N, 64, 0, s>>>(c->d_sys, c->d_nodes, c->N_cap, N, c->d_start, c->P, r_app, c->d_app,
-                                     nullptr, nullptr, 0);
+    k_app_count<NX><<<N, 64, 0, s>>>(c->d_sys, B, c->d_nodes, c->N_cap, N, c->d_start, c->P, r_app, c->d_app,
+                                     c->d_app_mask);
     dp_scan_small(c->d_app, N, c->d_total, c->pair_cap, c->d_counters, DPC_PAIRS, s);
-    k_app<NX, true><<<N, 64, 0, s>>>(c->d_sys, c->d_nodes, c->N_cap, N, c->d_start, c->P, r_app, c->d_app,
-                                    c->d_pnode, c->d_pprim, c->pair_cap);
+    k_app_fill<<<N, 64, 0, s>>>(N, c->P, c->d_app, c->d_app_mask, c->d_pnode, c->d_pprim, c->pair_cap);
     return DP_OK;
   });
   if (rc) return rc;

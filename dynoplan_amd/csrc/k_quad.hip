@@ -71,19 +71,22 @@ struct dp_quad_args {
   int32_t exact;
   int32_t redo;           // k_quad64: only the pairs deferred past their FP64 window
   int32_t* q64;           // FP32 mode: the pairs of the FP64 pass (counters[DPC_Q64] of them)
+  // per primitive state s: attitude / rate trajectory from the stored start
+  // (k_quad_rot), shared by every pair of the primitive
+  const double* rot64;    // [7][S] qx qy qz qw wx wy wz
+  const double* dv64;     // [3][S] velocity increment of the step into state s
+  const float* dv32;      // [3][S] the same in FP32
+  const int32_t* rbad64;  // [P] first state whose attitude / rate leaves the bounds (INT_MAX: none)
+  const int32_t* rbad32;  // [P] the same in FP32
+  const int32_t* rnear32; // [P] first FP32 state within the re-check margin of an attitude / rate bound
   long long* counters;
 };
 
 namespace {
 
 constexpr int kQWaves = 4;
-
-__device__ __forceinline__ bool state_valid64(const dp_dev_sys& S, const double* s) {
-  bool ok = true;
-#pragma unroll
-  for (int c = 0; c < 13; ++c) ok = ok && (s[c] >= S.lo[c] && s[c] <= S.hi[c]);
-  return ok;
-}
+// obstacles listed per lane for the FP64 re-check (int16 in LDS)
+constexpr int kQ64Cand = 128;
 
 __device__ __forceinline__ bool collides64(const dp_dev_sys& S, const double* obs, int nb, int ns, const double* s) {
   const double r = S.radius;
@@ -101,9 +104,12 @@ struct quad_f32 {
   float cosc[6], sincc[6];
 };
 
-__device__ __forceinline__ void quad_step32(const quad_f32& K, float* s, const float* u) {
+// FP32 attitude / rate part of the step (quaternion renormalised in FP64)
+// and the FP32 velocity increment dv (the state's velocity becomes v + dv).
+// It reads only s[3..6], s[10..12] and the rotors, so it is evaluated once
+// per primitive and shared by every pair (k_quad_rot).
+__device__ __forceinline__ void quad_rot_step32(const quad_f32& K, float* s, const float* u, float* dv) {
   const float qx = s[3], qy = s[4], qz = s[5], qw = s[6];
-  const float vx = s[7], vy = s[8], vz = s[9];
   const float wx = s[10], wy = s[11], wz = s[12];
   const float f = K.g4 * (((u[0] + u[1]) + u[2]) + u[3]);
   const float tx = K.armF * (((u[2] + u[3]) - u[0]) - u[1]);
@@ -116,12 +122,9 @@ __device__ __forceinline__ void quad_step32(const quad_f32& K, float* s, const f
   const float dwx = (tx - (wy * jwz - wz * jwy)) / K.Jx;
   const float dwy = (ty - (wz * jwx - wx * jwz)) / K.Jy;
   const float dwz = (tz - (wx * jwy - wy * jwx)) / K.Jz;
-  s[0] = s[0] + K.dt * vx;
-  s[1] = s[1] + K.dt * vy;
-  s[2] = s[2] + K.dt * vz;
-  s[7] = vx + K.dt * (f * rzx);
-  s[8] = vy + K.dt * (f * rzy);
-  s[9] = vz + K.dt * (f * rzz - K.g);
+  dv[0] = K.dt * (f * rzx);
+  dv[1] = K.dt * (f * rzy);
+  dv[2] = K.dt * (f * rzz - K.g);
   s[10] = wx + K.dt * dwx;
   s[11] = wy + K.dt * dwy;
   s[12] = wz + K.dt * dwz;
@@ -163,6 +166,84 @@ __device__ __forceinline__ quad_f32 make_quad_f32(const dp_dev_sys& S) {
     K.sincc[n] = (float)S.sinc_c[n];
   }
   return K;
+}
+
+// Per primitive, once per upload: the attitude / rate rollout from the
+// stored start and the velocity increments of its steps, in FP64 (the FP64
+// step's operations, dp_quad_rot_step64) and in FP32 (quad_rot_step32), plus
+// the first state whose attitude or rate leaves the bounds (FP64 exact;
+// FP32 with the re-check margins, also the first state within them).  The
+// position / velocity offset of a pair does not enter these components, so
+// every pair of the primitive shares them.  One thread per primitive.
+__global__ void k_quad_rot(const dp_dev_sys* __restrict__ Sp, int32_t P, const int32_t* __restrict__ off,
+                           const double* __restrict__ st, int32_t Sn, const double* __restrict__ act,
+                           const float* __restrict__ act32, double* __restrict__ rot64, double* __restrict__ dv64,
+                           float* __restrict__ dv32, int32_t* __restrict__ rbad64, int32_t* __restrict__ rbad32,
+                           int32_t* __restrict__ rnear32) {
+  const dp_dev_sys& S = *Sp;
+  const quad_f32 K = make_quad_f32(S);
+  const float eps_q = 100.0f * S.near_eps, eps_w = 100.0f * S.near_eps;
+  for (int32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < P; j += gridDim.x * blockDim.x) {
+    const int32_t s0 = off[j];
+    const int32_t L1 = off[j + 1] - s0;
+    const int32_t a0 = s0 - j;
+    double x[13];
+    float y[13];
+#pragma unroll
+    for (int c = 0; c < 13; ++c) {
+      x[c] = st[(int64_t)c * Sn + s0];
+      y[c] = (float)x[c];
+    }
+    int32_t b64 = INT_MAX, b32 = INT_MAX, n32 = INT_MAX;
+    for (int k = 0; k < L1; ++k) {
+      if (k > 0) {
+        double dv[3];
+        const double4 uv = reinterpret_cast<const double4*>(act)[a0 + k - 1];
+        const double u[4] = {uv.x, uv.y, uv.z, uv.w};
+        double o[13];
+        dp_quad_rot_step64(S, x, u, o, dv);
+#pragma unroll
+        for (int c = 3; c < 7; ++c) x[c] = o[c];
+#pragma unroll
+        for (int c = 10; c < 13; ++c) x[c] = o[c];
+        float dvf[3];
+        const float4 uf = reinterpret_cast<const float4*>(act32)[a0 + k - 1];
+        const float u32[4] = {uf.x, uf.y, uf.z, uf.w};
+        quad_rot_step32(K, y, u32, dvf);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          dv64[(int64_t)c * Sn + s0 + k] = dv[c];
+          dv32[(int64_t)c * Sn + s0 + k] = dvf[c];
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          dv64[(int64_t)c * Sn + s0] = 0.0;
+          dv32[(int64_t)c * Sn + s0] = 0.0f;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) rot64[(int64_t)c * Sn + s0 + k] = x[3 + c];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) rot64[(int64_t)(4 + c) * Sn + s0 + k] = x[10 + c];
+      bool bad64 = false, bad32 = false, near32 = false;
+#pragma unroll
+      for (int c = 3; c < 13; ++c) {
+        if (c >= 7 && c < 10) continue;  // velocities depend on the pair
+        bad64 = bad64 || !(x[c] >= S.lo[c] && x[c] <= S.hi[c]);
+        const float lo_ = (float)S.lo[c], hi_ = (float)S.hi[c];
+        const float e = c < 7 ? eps_q : eps_w;
+        bad32 = bad32 || !(y[c] >= lo_ && y[c] <= hi_);
+        near32 = near32 || fabsf(y[c] - lo_) <= e || fabsf(y[c] - hi_) <= e;
+      }
+      if (bad64 && b64 == INT_MAX) b64 = k;
+      if (bad32 && b32 == INT_MAX) b32 = k;
+      if (near32 && n32 == INT_MAX) n32 = k;
+    }
+    rbad64[j] = b64;
+    rbad32[j] = b32;
+    rnear32[j] = n32;
+  }
 }
 
 // FP32 start state of pair (node, primitive j), relative to the node:
@@ -274,36 +355,43 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
       const int32_t s0 = a.off[j];
       const int32_t L1 = a.off[j + 1] - s0;
       const int32_t a0 = s0 - j;
-      float s[13];
-      quad32_start(a, s0, nvel, s);
-      float plo[3], phi[3];
+      // position (relative to the node) and velocity; the attitude / rate
+      // part of every step is the primitive's (k_quad_rot)
+      float s[6];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        s[c] = (float)a.st[(int64_t)c * a.S + s0];
+        s[3 + c] = (float)(a.st[(int64_t)(7 + c) * a.S + s0] + nvel[c]);
+      }
+      const int32_t rbad = a.rbad32[j], rnear = a.rnear32[j];
+      float plo[3], phi[3], vlo[3], vhi[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         plo[c] = (float)(S.lo[c] - np[c]);
         phi[c] = (float)(S.hi[c] - np[c]);
+        vlo[c] = (float)S.lo[7 + c];
+        vhi[c] = (float)S.hi[7 + c];
       }
       int first_bad = INT_MAX, first_col = INT_MAX;
       bool near_bound_any = false, near_col_before = false, near = false;
       for (int k = 0; k < L1; ++k) {
         if (k > 0) {
-          const float4 uv = reinterpret_cast<const float4*>(a.act32)[a0 + k - 1];
-          const float u[4] = {uv.x, uv.y, uv.z, uv.w};
-          quad_step32(K, s, u);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            s[c] = s[c] + K.dt * s[3 + c];
+            s[3 + c] = s[3 + c] + a.dv32[(int64_t)c * a.S + s0 + k];
+          }
         }
-        // bounds with margin
-        bool bad = false, nb_ = false;
+        // bounds with margin (attitude and rate: the primitive's first
+        // offending / near state)
+        bool bad = k >= rbad, nb_ = k >= rnear;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           bad = bad || !(s[c] >= plo[c] && s[c] <= phi[c]);
           nb_ = nb_ || fabsf(s[c] - plo[c]) <= eps || fabsf(s[c] - phi[c]) <= eps;
           nb_ = nb_ || fabsf(s[c]) > freach;  // left the staged reach box
-        }
-#pragma unroll
-        for (int c = 7; c < 13; ++c) {
-          const float lo_ = (float)S.lo[c], hi_ = (float)S.hi[c];
-          bad = bad || !(s[c] >= lo_ && s[c] <= hi_);
-          const float e = c < 10 ? eps_v : eps_w;
-          nb_ = nb_ || fabsf(s[c] - lo_) <= e || fabsf(s[c] - hi_) <= e;
+          bad = bad || !(s[3 + c] >= vlo[c] && s[3 + c] <= vhi[c]);
+          nb_ = nb_ || fabsf(s[3 + c] - vlo[c]) <= eps_v || fabsf(s[3 + c] - vhi[c]) <= eps_v;
         }
         near_bound_any = near_bound_any || nb_;
         if (bad) {
@@ -391,9 +479,63 @@ __device__ __forceinline__ int64_t quad64_pair(const dp_quad_args& a, int64_t t)
   return a.exact || a.redo ? t : (int64_t)a.q64[t];
 }
 
+// Start state of pair (node i, primitive j) in FP64: stored state plus the
+// node's position / velocity (the offset dimensions).
+__device__ __forceinline__ void quad64_start(const dp_dev_sys& S, const dp_quad_args& a, int32_t i, int32_t s0,
+                                             double* s) {
+#pragma unroll
+  for (int c = 0; c < 13; ++c) {
+    const double v = a.st[(int64_t)c * a.S + s0];
+    s[c] = dp_is_offset_dim(S, c) ? v + a.nodes[(int64_t)c * a.Ncap + i] : v;
+  }
+}
+
+// State k of the pair from state k-1: translation with the primitive's
+// velocity increment (dp_quad_trans_step64); s[3..6], s[10..12] untouched.
+__device__ __forceinline__ void quad64_advance(const dp_quad_args& a, int32_t s0, int k, double* s) {
+  double dv[3], o[13];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) dv[c] = a.dv64[(int64_t)c * a.S + s0 + k];
+  dp_quad_trans_step64(a.sys->dt, s, dv, o);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    s[c] = o[c];
+    s[7 + c] = o[7 + c];
+  }
+}
+
+// Attitude and rate of primitive state st into s[3..6], s[10..12].
+__device__ __forceinline__ void quad64_rot(const dp_quad_args& a, int64_t st, double* s) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) s[3 + c] = a.rot64[(int64_t)c * a.S + st];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) s[10 + c] = a.rot64[(int64_t)(4 + c) * a.S + st];
+}
+
+// Bounds of the position and velocity components (is_state_valid on them).
+__device__ __forceinline__ bool trans_valid64(const dp_dev_sys& S, const double* s) {
+  bool ok = true;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    ok = ok && (s[c] >= S.lo[c] && s[c] <= S.hi[c]);
+    ok = ok && (s[7 + c] >= S.lo[7 + c] && s[7 + c] <= S.hi[7 + c]);
+  }
+  return ok;
+}
+
+// FP64 pass, one lane per pair.  A pair that needs the full re-check is
+// rolled out twice: first for the bounds over every state (the reference
+// rejects on bounds before it tests collisions, dbastar.cpp:440-446) and the
+// box of its positions; if in bounds, the obstacles meeting that box
+// (inflated by the robot radius and 1e-9 m) are listed in the lane's LDS
+// slice, and the second rollout tests each state against them only.  Both
+// rollouts form the same states.  Other pairs are rolled out once, for the
+// goal checks, the final state, g and h.
 __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
+  extern __shared__ int16_t q64_cand[];  // [blockDim.x][kQ64Cand]
   const dp_dev_sys& S = *a.sys;
   const int64_t n64 = quad64_count(a);
+  int16_t* cand = q64_cand + (int64_t)threadIdx.x * kQ64Cand;
   long long c_b = 0, c_c = 0, c_f = 0, c_k = 0, c_r = 0;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n64; t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t q = quad64_pair(a, t);
@@ -406,41 +548,78 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
     const int32_t s0 = a.off[j];
     const int32_t L1 = a.off[j + 1] - s0;
     const int32_t a0 = s0 - j;
-    // The node and the first translated state only enter g: its
-    // lower-bound term is formed here, so neither stays live in registers
-    // through the rollout.
+    // s[0..2] position, s[7..9] velocity; attitude and rate come from the
+    // primitive's trajectory (k_quad_rot) when a full state is needed
     double s[13];
+    int first_bad = INT_MAX, first_col = INT_MAX;
+    const int32_t rbad = a.rbad64[j];
+    int ncand = 0;  // obstacles listed for the second rollout (-1: more than kQ64Cand, test all)
+    if (full) {
+      quad64_start(S, a, i, s0, s);
+      double bl[3] = {s[0], s[1], s[2]}, bh[3] = {s[0], s[1], s[2]};
+      for (int k = 0; k < L1; ++k) {
+        if (k > 0) quad64_advance(a, s0, k, s);
+        if (k >= rbad || !trans_valid64(S, s)) {
+          first_bad = k;
+          break;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          bl[c] = fmin(bl[c], s[c]);
+          bh[c] = fmax(bh[c], s[c]);
+        }
+      }
+      if (first_bad == INT_MAX) {
+        // a sphere of radius r at p meets a box only if p lies in the box
+        // grown by r on every axis; pad 1e-9 m against rounding
+        const double rr = S.radius + 1e-9;
+        for (int o = 0; o < a.nb + a.ns && ncand >= 0; ++o) {
+          const bool box = o < a.nb;
+          const double* b = box ? a.obs + 6 * o : a.obs + 6 * a.nb + 4 * (o - a.nb);
+          bool meet = true;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const double h = (box ? b[3 + c] : b[3]) + rr;
+            meet = meet && b[c] - h <= bh[c] && b[c] + h >= bl[c];
+          }
+          if (meet) {
+            if (ncand < kQ64Cand) cand[ncand++] = (int16_t)o;
+            else ncand = -1;
+          }
+        }
+      }
+    }
+    // The node and the first translated state only enter g: its lower-bound
+    // term is formed here, so neither stays live through the rollout.
     double lb0;
     {
       double node[13];
 #pragma unroll
       for (int c = 0; c < 13; ++c) node[c] = a.nodes[(int64_t)c * a.Ncap + i];
-#pragma unroll
-      for (int c = 0; c < 13; ++c) {
-        const double v = a.st[(int64_t)c * a.S + s0];
-        s[c] = dp_is_offset_dim(S, c) ? v + node[c] : v;
-      }
+      quad64_start(S, a, i, s0, s);
       lb0 = dp_lower_bound_time(S, node, s);
     }
-    int first_bad = INT_MAX, first_col = INT_MAX;
     int chosen = -1, nn = 0;
     const int nng = a.check_goal ? a.num_goal : 0;
-    for (int k = 0; k < L1; ++k) {
-      if (k > 0) {
-        const double4 uv = reinterpret_cast<const double4*>(a.act)[a0 + k - 1];
-        const double u[4] = {uv.x, uv.y, uv.z, uv.w};
-        double o[13];
-        dp_quad_step64(S, s, u, o);
-#pragma unroll
-        for (int c = 0; c < 13; ++c) s[c] = o[c];
-      }
-      if (full) {
-        if (first_bad == INT_MAX && !state_valid64(S, s)) first_bad = k;
-        if (first_bad != INT_MAX) break;
-        if (first_col == INT_MAX && collides64(S, a.obs, a.nb, a.ns, s)) first_col = k;
+    for (int k = 0; k < L1 && first_bad == INT_MAX; ++k) {
+      if (k > 0) quad64_advance(a, s0, k, s);
+      if (full && first_col == INT_MAX) {
+        bool col;
+        if (ncand < 0) {
+          col = collides64(S, a.obs, a.nb, a.ns, s);
+        } else {
+          col = false;
+          for (int e = 0; e < ncand && !col; ++e) {
+            const int o = cand[e];
+            col = o < a.nb ? dp_sphere_box(s[0], s[1], s[2], S.radius, a.obs + 6 * o)
+                           : dp_sphere_sphere(s[0], s[1], s[2], S.radius, a.obs + 6 * a.nb + 4 * (o - a.nb));
+          }
+        }
+        if (col) first_col = k;
       }
       // check_goal at the float-rule indices, in nn order (dbastar.cpp:492-512)
       while (chosen == -1 && nn < nng && a.goalidx[(int64_t)nn * a.P + j] == k) {
+        quad64_rot(a, s0 + k, s);
         if (dp_within(S, s, a.goal, a.goal_bound)) {
           chosen = k;
           // the final state is the goal state: stored now, read back below
@@ -450,6 +629,7 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
         ++nn;
       }
     }
+    if (first_bad == INT_MAX) quad64_rot(a, s0 + L1 - 1, s);  // the full final state
     int8_t vd = DP_V_FREE;
     int32_t fk = -1;
     if (first_bad != INT_MAX) {
@@ -551,7 +731,9 @@ __global__ __launch_bounds__(64) void k_quad_window(int32_t N, const int32_t* __
   const int32_t tot = *total;
   const int32_t b = app[node] < tot ? app[node] : tot;
   const int32_t e = app[node + 1] < tot ? app[node + 1] : tot;
-  int32_t rank = 0;
+  // first pass: deferrals, and the node's count of pairs for the FP64 pass
+  // (one atomic per node reserves their place in the compact list)
+  int32_t rank = 0, need_n = 0;
   long long nd = 0;
   for (int32_t cb = b; cb < e; cb += 64) {
     const int32_t p = cb + lane;
@@ -566,16 +748,22 @@ __global__ __launch_bounds__(64) void k_quad_window(int32_t N, const int32_t* __
       res[p] = DP_R_UNDECIDED;
       ++nd;
     }
-    // the others go to the FP64 pass's compact list
-    const bool need = (fl & (QF_NEAR | QF_NEED64)) != 0 && !defer;
-    const uint64_t nm = __ballot(need);
-    if (nm) {
-      int32_t base = 0;
-      if (lane == 0) base = (int32_t)atomicAdd((unsigned long long*)&counters[DPC_Q64], (unsigned long long)__popcll(nm));
-      base = __shfl(base, 0, 64);
-      if (need) q64[base + __popcll(nm & lt)] = p;
-    }
+    need_n += __popcll(__ballot((fl & (QF_NEAR | QF_NEED64)) != 0 && !defer));
     rank += __popcll(m);
+  }
+  if (need_n > 0) {
+    int32_t base = 0;
+    if (lane == 0) base = (int32_t)atomicAdd((unsigned long long*)&counters[DPC_Q64], (unsigned long long)need_n);
+    base = __shfl(base, 0, 64);
+    // second pass: the pairs in canonical order (deferred ones now carry DP_F_DEFER)
+    for (int32_t cb = b; cb < e; cb += 64) {
+      const int32_t p = cb + lane;
+      const int8_t fl = p < e ? flags[p] : (int8_t)0;
+      const bool need = (fl & (QF_NEAR | QF_NEED64)) != 0;
+      const uint64_t nm = __ballot(need);
+      if (need) q64[base + __popcll(nm & lt)] = p;
+      base += __popcll(nm);
+    }
   }
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) nd += __shfl_xor(nd, d, 64);
@@ -598,26 +786,19 @@ __global__ void k_quad64_trace(dp_quad_args a, const int32_t* __restrict__ pairs
     const int32_t node = a.pnode[q], j = a.pprim[q];
     const int32_t s0 = a.off[j];
     const int32_t L1 = a.off[j + 1] - s0;
-    const int32_t a0 = s0 - j;
     double s[13];
-    for (int c = 0; c < 13; ++c) {
-      const double v = a.st[(int64_t)c * a.S + s0];
-      s[c] = dp_is_offset_dim(S, c) ? v + a.nodes[(int64_t)c * a.Ncap + node] : v;
-    }
+    quad64_start(S, a, node, s0, s);
     for (int k = 0; k < L1 && k < maxlen; ++k) {
-      if (k > 0) {
-        const double4 uv = reinterpret_cast<const double4*>(a.act)[a0 + k - 1];
-        const double u[4] = {uv.x, uv.y, uv.z, uv.w};
-        double o[13];
-        dp_quad_step64(S, s, u, o);
-        for (int c = 0; c < 13; ++c) s[c] = o[c];
-      }
+      if (k > 0) quad64_advance(a, s0, k, s);
+      quad64_rot(a, s0 + k, s);
       double* w = out + ((int64_t)t * maxlen + k) * 13;
       for (int c = 0; c < 13; ++c) w[c] = s[c];
     }
   }
 }
 
+// FP32: the same step functions as k_quad_rot and k_quad32 (attitude and
+// rate rolled out per pair here, bit-identical to the shared trajectory).
 __global__ void k_quad32_trace(dp_quad_args a, const int32_t* __restrict__ pairs, int32_t n, int32_t maxlen,
                                double* __restrict__ out) {
   const dp_dev_sys& S = *a.sys;
@@ -639,7 +820,12 @@ __global__ void k_quad32_trace(dp_quad_args a, const int32_t* __restrict__ pairs
       if (k > 0) {
         const float4 uv = reinterpret_cast<const float4*>(a.act32)[a0 + k - 1];
         const float u[4] = {uv.x, uv.y, uv.z, uv.w};
-        quad_step32(K, s, u);
+        float dv[3];
+        quad_rot_step32(K, s, u, dv);
+        for (int c = 0; c < 3; ++c) {
+          s[c] = s[c] + K.dt * s[7 + c];
+          s[7 + c] = s[7 + c] + dv[c];
+        }
       }
       double* o = out + ((int64_t)t * maxlen + k) * 13;
       for (int c = 0; c < 13; ++c) o[c] = c < 3 ? np[c] + (double)s[c] : (double)s[c];
@@ -696,11 +882,26 @@ dp_quad_args make_quad_args(dp_ctx* c, const dp_expand_params* p, double goal_bo
   a.exact = p->fp32_rollout ? 0 : 1;
   a.redo = 0;
   a.q64 = c->d_q64;
+  a.rot64 = c->d_rot64;
+  a.dv64 = c->d_dv64;
+  a.dv32 = c->d_dv32;
+  a.rbad64 = c->d_rbad64;
+  a.rbad32 = c->d_rbad32;
+  a.rnear32 = c->d_rnear32;
   a.counters = c->d_counters;
   return a;
 }
 
 }  // namespace
+
+int dp_launch_quad_rot(dp_ctx* c, hipStream_t s) {
+  if (c->P <= 0) return DP_OK;
+  k_quad_rot<<<dp_grid_blocks(c->P, 64, 4096), 64, 0, s>>>(c->d_sys, c->P, c->d_off, c->d_st, c->S, c->d_act,
+                                                           c->d_act32, c->d_rot64, c->d_dv64, c->d_dv32, c->d_rbad64,
+                                                           c->d_rbad32, c->d_rnear32);
+  DP_HIP_RET(hipGetLastError());
+  return DP_OK;
+}
 
 int32_t dp_fp64_window(const dp_ctx* c, int32_t limit) {
   if (c->fp64_window > 0) return c->fp64_window;
@@ -712,7 +913,7 @@ int dp_launch_quad_deferred(dp_ctx* c, hipStream_t s) {
   dp_quad_args a = make_quad_args(c, p, p->delta_factor_goal * (double)p->delta, c->round_r_dup);
   a.redo = 1;  // both kernels over every pair of the batch
   const int t64 = c->tiles.quad64_threads;
-  k_quad64<<<dp_grid_blocks(c->pair_cap, t64, c->tiles.quad64_blocks), t64, 0, s>>>(a);
+  k_quad64<<<dp_grid_blocks(c->pair_cap, t64, c->tiles.quad64_blocks), t64, sizeof(int16_t) * t64 * kQ64Cand, s>>>(a);
   if (c->nv > 0) k_quad_dupv<<<dp_grid_blocks(c->pair_cap, 256, 8192), 256, 0, s>>>(a);
   DP_HIP_RET(hipGetLastError());
   return DP_OK;
@@ -776,7 +977,7 @@ int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, doub
     if (rc) return rc;
   }
   const int t64 = c->tiles.quad64_threads;
-  k_quad64<<<dp_grid_blocks(c->pair_cap, t64, c->tiles.quad64_blocks), t64, 0, s>>>(a);
+  k_quad64<<<dp_grid_blocks(c->pair_cap, t64, c->tiles.quad64_blocks), t64, sizeof(int16_t) * t64 * kQ64Cand, s>>>(a);
   if (c->nv > 0) k_quad_dupv<<<dp_grid_blocks(c->pair_cap, 256, 8192), 256, 0, s>>>(a);
   DP_HIP_RET(hipGetLastError());
   return DP_OK;

@@ -168,3 +168,15 @@ int dp_ensure_f64(double** p, int64_t* cap, int64_t need) {
   *cap = need;
   return DP_OK;
 }
+
+int dp_ensure_u64(uint64_t** p, int64_t* cap, int64_t need) {
+  if (*cap >= need && *p) return DP_OK;
+  dp_bump_epoch();
+  DP_HIP_RET(hipDeviceSynchronize());
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  DP_HIP_RET(hipMalloc(p, sizeof(uint64_t) * (size_t)(need > 0 ? need : 1)));
+  *cap = need;
+  return DP_OK;
+}
