@@ -270,6 +270,8 @@ void dp_destroy(dp_ctx* c) {
   dev_free(&c->d_chunk);
   dev_free(&c->d_dbg32);
   dev_free(&c->d_q64);
+  dev_free(&c->d_qperm);
+  dev_free(&c->d_nshort);
   dev_free(&c->d_rot64);
   dev_free(&c->d_dv64);
   dev_free(&c->d_dv32);
@@ -569,6 +571,7 @@ int dp_reserve(dp_ctx* c, int32_t max_nodes, int64_t max_pairs) {
     if ((rc = dev_alloc(&c->d_node_cursor, max_nodes))) return rc;
     if ((rc = dev_alloc(&c->d_node_count, max_nodes))) return rc;
     if ((rc = dev_alloc(&c->d_chunk, (int64_t)max_nodes + 1))) return rc;
+    if ((rc = dev_alloc(&c->d_nshort, max_nodes))) return rc;
     if ((rc = dev_alloc(&c->d_node_obs, (int64_t)DP_NODE_OBS * max_nodes))) return rc;
     if ((rc = dev_alloc(&c->d_node_nobs, max_nodes))) return rc;
     c->N_cap = max_nodes;
@@ -595,6 +598,7 @@ int dp_reserve(dp_ctx* c, int32_t max_nodes, int64_t max_pairs) {
     if ((rc = dev_alloc(&c->d_conf_pool, c->conf_pool_cap))) return rc;
     if (c->desc.system == DP_SYS_QUADROTOR && (rc = dev_alloc(&c->d_dbg32, 3 * max_pairs))) return rc;
     if (c->desc.system == DP_SYS_QUADROTOR && (rc = dev_alloc(&c->d_q64, max_pairs))) return rc;
+    if (c->desc.system == DP_SYS_QUADROTOR && (rc = dev_alloc(&c->d_qperm, max_pairs))) return rc;
     c->pair_cap = max_pairs;
   }
   return DP_OK;

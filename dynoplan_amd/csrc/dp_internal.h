@@ -235,6 +235,8 @@ struct dp_ctx {
   long long* h_pinned = nullptr;    // pinned host words for device totals
   int32_t rounds = 32;              // resolution rounds launched per batch
   int32_t* d_chunk = nullptr;       // [N_cap+1] quadrotor: 64-pair chunks per node, scanned
+  int32_t* d_nshort = nullptr;      // [N_cap] quadrotor: pairs whose primitive leaves the rate bounds
+  int32_t* d_qperm = nullptr;       // [pair_cap] quadrotor: pairs per node, those first
   float* d_dbg32 = nullptr;         // [3][pair_cap] quadrotor: FP32 final positions (error report)
   int32_t* d_q64 = nullptr;         // [pair_cap] quadrotor: pairs the FP64 pass rolls out (DPC_Q64 of them)
   // quadrotor, per primitive state: attitude / rate trajectory and velocity

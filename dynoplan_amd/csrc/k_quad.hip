@@ -53,6 +53,8 @@ struct dp_quad_args {
   dp_grid_geom grid;
   const int32_t* app;     // pair offsets per node [N+1]
   const int32_t* chunk_off;  // [N+1] wave chunks per node, scanned
+  const int32_t* qperm;   // [pairs] each node's pairs, those the primitive's rate bounds reject first
+  const int32_t* nshort;  // [N] how many of them
   const int32_t* pnode;
   const int32_t* pprim;
   const int32_t* total;
@@ -285,8 +287,16 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
     const int32_t tot = *a.total;
     const int32_t pbeg = a.app[node] < tot ? a.app[node] : tot;
     const int32_t pend = a.app[node + 1] < tot ? a.app[node + 1] : tot;
-    const int32_t q = pbeg + (w - a.chunk_off[node]) * 64 + lane;
-    const bool vp = q < pend;
+    // the node's chunks: first those of pairs whose primitive leaves the
+    // rate bounds (BOUNDS whatever the translation: no collision tests), then
+    // the others
+    const int32_t ns = a.nshort[node];
+    const int32_t cshort = (ns + 63) / 64;
+    const int32_t cw = w - a.chunk_off[node];
+    const bool short_chunk = cw < cshort;
+    const int32_t idx = short_chunk ? pbeg + cw * 64 + lane : pbeg + ns + (cw - cshort) * 64 + lane;
+    const bool vp = short_chunk ? idx < pbeg + ns : idx < pend;
+    const int32_t q = vp ? a.qperm[idx] : 0;
     double np[3], nvel[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -299,7 +309,7 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
     const float freach = (float)reach;
     // stage candidate obstacles relative to the node, ballot-compacted
     int nbc = 0, nsc = 0;
-    for (int o0 = 0; o0 < a.nb; o0 += 64) {
+    for (int o0 = 0; o0 < (short_chunk ? 0 : a.nb); o0 += 64) {
       const int o = o0 + lane;
       bool keep = false;
       float rel[6];
@@ -323,7 +333,7 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
       nbc += __popcll(m);
     }
     float* cs = cb + 6 * a.nb;
-    for (int o0 = 0; o0 < a.ns; o0 += 64) {
+    for (int o0 = 0; o0 < (short_chunk ? 0 : a.ns); o0 += 64) {
       const int o = o0 + lane;
       bool keep = false;
       float rel[4];
@@ -398,7 +408,7 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
           first_bad = k;
           break;  // later states cannot change a bounds verdict
         }
-        if (first_col == INT_MAX) {
+        if (first_col == INT_MAX && !short_chunk) {
           bool col = false, nc = false;
           for (int t = 0; t < nbc; ++t) {
             const float* b = cb + 6 * t;
@@ -699,13 +709,49 @@ __global__ __launch_bounds__(256) void k_quad_dupv(dp_quad_args a) {
   }
 }
 
-__global__ void k_quad_chunks(int32_t N, const int32_t* __restrict__ app, const int32_t* __restrict__ total,
-                              int32_t* __restrict__ chunks) {
+// Per node: its pairs reordered for the FP32 pass, those whose primitive
+// leaves the attitude / rate bounds within its length first (their verdict
+// is BOUNDS whatever the translation, so their rollouts are short and need
+// no collision test), then the rest, each group in canonical order; and the
+// node's wave chunks (64 pairs, one group per chunk).  One wave per node.
+__global__ __launch_bounds__(64) void k_quad_split(int32_t N, const int32_t* __restrict__ app,
+                                                   const int32_t* __restrict__ total, const int32_t* __restrict__ pprim,
+                                                   const int32_t* __restrict__ off, const int32_t* __restrict__ rbad32,
+                                                   int32_t* __restrict__ qperm, int32_t* __restrict__ nshort,
+                                                   int32_t* __restrict__ chunks) {
+  const int32_t node = blockIdx.x;
+  if (node >= N) return;
+  const int lane = threadIdx.x;
+  const uint64_t lt = (1ull << lane) - 1ull;
   const int32_t tot = *total;
-  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-    const int32_t b = app[i] < tot ? app[i] : tot;
-    const int32_t e = app[i + 1] < tot ? app[i + 1] : tot;
-    chunks[i] = (e - b + 63) / 64;
+  const int32_t b = app[node] < tot ? app[node] : tot;
+  const int32_t e = app[node + 1] < tot ? app[node + 1] : tot;
+  int32_t ns = 0;
+  for (int32_t cb = b; cb < e; cb += 64) {
+    const int32_t p = cb + lane;
+    bool sh = false;
+    if (p < e) {
+      const int32_t j = pprim[p];
+      sh = rbad32[j] < off[j + 1] - off[j];
+    }
+    ns += __popcll(__ballot(sh));
+  }
+  int32_t ws = b, wl = b + ns;
+  for (int32_t cb = b; cb < e; cb += 64) {
+    const int32_t p = cb + lane;
+    bool sh = false;
+    if (p < e) {
+      const int32_t j = pprim[p];
+      sh = rbad32[j] < off[j + 1] - off[j];
+    }
+    const uint64_t ms = __ballot(sh), ml = __ballot(p < e && !sh);
+    if (p < e) qperm[sh ? ws + __popcll(ms & lt) : wl + __popcll(ml & lt)] = p;
+    ws += __popcll(ms);
+    wl += __popcll(ml);
+  }
+  if (lane == 0) {
+    nshort[node] = ns;
+    chunks[node] = (ns + 63) / 64 + (e - b - ns + 63) / 64;
   }
 }
 
@@ -862,6 +908,8 @@ dp_quad_args make_quad_args(dp_ctx* c, const dp_expand_params* p, double goal_bo
   a.grid = c->grid;
   a.app = c->d_app;
   a.chunk_off = c->d_chunk;
+  a.qperm = c->d_qperm;
+  a.nshort = c->d_nshort;
   a.pnode = c->d_pnode;
   a.pprim = c->d_pprim;
   a.total = c->d_total;
@@ -960,7 +1008,8 @@ int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, doub
       dp_set_error("too many obstacles for the FP32 quadrotor staging (max 64 KiB LDS per workgroup)");
       return DP_ERR_ARG;
     }
-    k_quad_chunks<<<dp_grid_blocks(c->N, 256, 1024), 256, 0, s>>>(c->N, c->d_app, c->d_total, c->d_chunk);
+    k_quad_split<<<c->N, 64, 0, s>>>(c->N, c->d_app, c->d_total, c->d_pprim, c->d_off, c->d_rbad32, c->d_qperm,
+                                     c->d_nshort, c->d_chunk);
     dp_scan_small(c->d_chunk, c->N, nullptr, INT64_MAX, c->d_counters, -1, s);
     const int64_t max_chunks = (int64_t)c->N * ((c->P + 63) / 64);
     int rc = dp_record_event(c, 5, s);
