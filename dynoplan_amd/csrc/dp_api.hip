@@ -271,8 +271,10 @@ void dp_destroy(dp_ctx* c) {
   dev_free(&c->d_dbg32);
   dev_free(&c->d_q64);
   dev_free(&c->d_qperm);
+  dev_free(&c->d_chunk_node);
   dev_free(&c->d_nshort);
   dev_free(&c->d_rot64);
+  dev_free(&c->d_preach);
   dev_free(&c->d_dv64);
   dev_free(&c->d_dv32);
   dev_free(&c->d_rbad64);
@@ -427,11 +429,18 @@ int dp_upload_primitives(dp_ctx* c, const dp_prims* pr, int32_t max_motions) {
     HIP_CHECK(hipMemcpy(c->d_act, asoa.data(), sizeof(double) * asoa.size(), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(c->d_act32, asoa32.data(), sizeof(float) * asoa32.size(), hipMemcpyHostToDevice));
     double reach = 0.0;
-    for (int32_t j = 0; j < P; ++j)
+    std::vector<float> preach((size_t)P);
+    for (int32_t j = 0; j < P; ++j) {
+      double rj = 0.0;
       for (int32_t s = off[j]; s < off[j + 1]; ++s)
         for (int k = 0; k < 3; ++k)
-          reach = std::max(reach, std::fabs(st[(int64_t)s * nx + k] - st[(int64_t)off[j] * nx + k]));
+          rj = std::max(rj, std::fabs(st[(int64_t)s * nx + k] - st[(int64_t)off[j] * nx + k]));
+      preach[j] = std::nextafter((float)rj, INFINITY);  // never below the FP64 value
+      reach = std::max(reach, rj);
+    }
     c->prim_reach = reach;
+    if ((rc = dev_alloc(&c->d_preach, P))) return rc;
+    HIP_CHECK(hipMemcpy(c->d_preach, preach.data(), sizeof(float) * P, hipMemcpyHostToDevice));
   }
   dp_bump_epoch();
   c->h_st.assign(st, st + (size_t)S * nx);
@@ -454,8 +463,8 @@ int dp_upload_primitives(dp_ctx* c, const dp_prims* pr, int32_t max_motions) {
   if (quad) {
     // attitude / rate trajectories shared by every pair of a primitive
     if ((rc = dev_alloc(&c->d_rot64, (int64_t)7 * S))) return rc;
-    if ((rc = dev_alloc(&c->d_dv64, (int64_t)3 * S))) return rc;
-    if ((rc = dev_alloc(&c->d_dv32, (int64_t)3 * S))) return rc;
+    if ((rc = dev_alloc(&c->d_dv64, (int64_t)4 * S))) return rc;
+    if ((rc = dev_alloc(&c->d_dv32, (int64_t)4 * S))) return rc;
     if ((rc = dev_alloc(&c->d_rbad64, P))) return rc;
     if ((rc = dev_alloc(&c->d_rbad32, P))) return rc;
     if ((rc = dev_alloc(&c->d_rnear32, P))) return rc;
@@ -651,6 +660,9 @@ int dp_size_pairs(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   int rc = dp_reserve(c, c->N, std::min(worst, kOutright));
   if (rc) return rc;
   if ((rc = dp_ensure_u64(&c->d_app_mask, &c->app_mask_cap, (int64_t)c->N * ((c->P + 63) / 64)))) return rc;
+  if (c->desc.system == DP_SYS_QUADROTOR &&
+      (rc = dp_ensure_i32(&c->d_chunk_node, &c->chunk_node_cap, (int64_t)c->N * ((c->P + 63) / 64 + 1))))
+    return rc;
   if (worst <= c->pair_cap) return DP_OK;
   if ((rc = dp_launch_count(c, p, s))) return rc;
   long long total = 0;
@@ -920,7 +932,7 @@ int dp_autotune(dp_ctx* c, const dp_expand_params* p, dp_tile_config* best, dp_t
   std::vector<dp_tile_config> cand;
   const dp_tile_config base = c->tiles;
   if (c->desc.system == DP_SYS_QUADROTOR) {
-    for (int q32 : {2048, 4096, 8192})
+    for (int q32 : {512, 1024, 2048, 4096})
       for (int t64 : {64, 128, 256})
         for (int b64 : {4096, 8192, 16384}) {
           dp_tile_config t = base;

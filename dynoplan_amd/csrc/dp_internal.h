@@ -237,16 +237,19 @@ struct dp_ctx {
   int32_t* d_chunk = nullptr;       // [N_cap+1] quadrotor: 64-pair chunks per node, scanned
   int32_t* d_nshort = nullptr;      // [N_cap] quadrotor: pairs whose primitive leaves the rate bounds
   int32_t* d_qperm = nullptr;       // [pair_cap] quadrotor: pairs per node, those first
+  int32_t* d_chunk_node = nullptr;  // quadrotor: node of each FP32 wave chunk
+  int64_t chunk_node_cap = 0;
   float* d_dbg32 = nullptr;         // [3][pair_cap] quadrotor: FP32 final positions (error report)
   int32_t* d_q64 = nullptr;         // [pair_cap] quadrotor: pairs the FP64 pass rolls out (DPC_Q64 of them)
   // quadrotor, per primitive state: attitude / rate trajectory and velocity
   // increments shared by every pair (k_quad_rot); per primitive: first state
   // whose attitude / rate leaves the bounds (FP64, FP32) or the FP32 margin
   double* d_rot64 = nullptr;        // [7][S]
-  double* d_dv64 = nullptr;         // [3][S]
-  float* d_dv32 = nullptr;          // [3][S]
+  double* d_dv64 = nullptr;         // [S][4] (x y z, pad)
+  float* d_dv32 = nullptr;          // [S][4]
   int32_t *d_rbad64 = nullptr, *d_rbad32 = nullptr, *d_rnear32 = nullptr;  // [P]
   double prim_reach = 0.0;          // max over primitives of max_k |p_k - p_0|_inf
+  float* d_preach = nullptr;        // [P] quadrotor: max_k |p_k - p_0|_inf per primitive (rounded up)
   double reach_box[4] = {0, 0, 0, 0};  // planar: union of the canonical swept boxes (lo x, hi x, lo y, hi y)
   int16_t* d_node_obs = nullptr;    // [N_cap][DP_NODE_OBS] obstacles within a node's reach box
   int32_t* d_node_nobs = nullptr;   // [N_cap] their count (-1: more than DP_NODE_OBS)

@@ -67,25 +67,29 @@ __device__ __forceinline__ bool app_reject(const dp_dev_sys& S, const app_bounds
   }
 }
 
-// One wave per node, 64 primitives per step: the applicable ones are
-// recorded as a bit mask per 64 primitives (mask[i * words + j / 64]) and
-// counted; the fill pass turns the masks into the (node, primitive) pair
-// list without evaluating the metric again.
+// kAppParts waves per node, each over a share of the primitives, 64 per
+// step: the applicable ones are recorded as a bit mask per 64 primitives
+// (mask[i * words + j / 64]) and counted into app[i] (cleared before); the
+// fill pass turns the masks into the (node, primitive) pair list without
+// evaluating the metric again.
+constexpr int kAppParts = 4;
+
 template <int NX>
 __global__ __launch_bounds__(64) void k_app_count(const dp_dev_sys* __restrict__ Sp, app_bounds B,
                                                   const double* __restrict__ nodes, int32_t Ncap, int32_t N,
                                                   const double* __restrict__ start, int32_t P, double r_app,
                                                   int32_t* __restrict__ app, uint64_t* __restrict__ mask) {
   const dp_dev_sys& S = *Sp;
-  const int32_t i = blockIdx.x;
+  const int32_t i = blockIdx.x / kAppParts, part = blockIdx.x % kAppParts;
   if (i >= N) return;
   const int lane = threadIdx.x;
   const int32_t words = (P + 63) / 64;
+  const int32_t w0 = (int32_t)((int64_t)words * part / kAppParts), w1 = (int32_t)((int64_t)words * (part + 1) / kAppParts);
   double a[NX];
 #pragma unroll
   for (int c = 0; c < NX; ++c) a[c] = dp_is_offset_dim(S, c) ? 0.0 : nodes[(int64_t)c * Ncap + i];
   int32_t cnt = 0;
-  for (int32_t j0 = 0; j0 < P; j0 += 64) {
+  for (int32_t j0 = 64 * w0; j0 < P && j0 < 64 * w1; j0 += 64) {
     const int32_t j = j0 + lane;
     bool hit = false;
     if (j < P && !app_reject<NX>(S, B, a, start, P, j, r_app)) {
@@ -98,7 +102,7 @@ __global__ __launch_bounds__(64) void k_app_count(const dp_dev_sys* __restrict__
     if (lane == 0) mask[(int64_t)i * words + j0 / 64] = m;
     cnt += __popcll(m);
   }
-  if (lane == 0) app[i] = cnt;
+  if (lane == 0 && cnt) atomicAdd(&app[i], cnt);
 }
 
 __global__ __launch_bounds__(64) void k_app_fill(int32_t N, int32_t P, const int32_t* __restrict__ app,
@@ -459,8 +463,9 @@ int dp_launch_count(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   const app_bounds B = make_app_bounds(c, r_app);
   return dispatch_nx(c->desc.nx, [&](auto NXc) -> int {
     constexpr int NX = decltype(NXc)::value;
-    k_app_count<NX><<<N, 64, 0, s>>>(c->d_sys, B, c->d_nodes, c->N_cap, N, c->d_start, c->P, r_app, c->d_app,
-                                     c->d_app_mask);
+    dp_fill_i32(c->d_app, N, 0, s);
+    k_app_count<NX><<<N * kAppParts, 64, 0, s>>>(c->d_sys, B, c->d_nodes, c->N_cap, N, c->d_start, c->P, r_app,
+                                                 c->d_app, c->d_app_mask);
     dp_scan_small(c->d_app, N, c->d_total, c->pair_cap, c->d_counters, DPC_PAIRS, s);
     DP_HIP_RET(hipGetLastError());
     return DP_OK;
@@ -490,8 +495,9 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   const app_bounds B = make_app_bounds(c, r_app);
   int rc = dispatch_nx(nx, [&](auto NXc) -> int {
     constexpr int NX = decltype(NXc)::value;
-    k_app_count<NX><<<N, 64, 0, s>>>(c->d_sys, B, c->d_nodes, c->N_cap, N, c->d_start, c->P, r_app, c->d_app,
-                                     c->d_app_mask);
+    dp_fill_i32(c->d_app, N, 0, s);
+    k_app_count<NX><<<N * kAppParts, 64, 0, s>>>(c->d_sys, B, c->d_nodes, c->N_cap, N, c->d_start, c->P, r_app,
+                                                 c->d_app, c->d_app_mask);
     dp_scan_small(c->d_app, N, c->d_total, c->pair_cap, c->d_counters, DPC_PAIRS, s);
     k_app_fill<<<N, 64, 0, s>>>(N, c->P, c->d_app, c->d_app_mask, c->d_pnode, c->d_pprim, c->pair_cap);
     return DP_OK;

@@ -37,6 +37,7 @@ struct dp_quad_args {
   const int32_t* off;
   const double* st;       // [13][S]
   int32_t S;
+  const double* start;    // [13][P] start state of each primitive (the pairs read it, not st)
   const double* act;      // [A][4]
   const float* act32;     // [A][4]
   int32_t A;
@@ -47,12 +48,14 @@ struct dp_quad_args {
   int32_t nb, ns;
   double prim_reach;      // max over primitives of max_k |p_k - p_0|_inf
   double tmax;            // longest primitive duration
+  const float* preach;    // [P] per primitive: max_k |p_k - p_0|_inf + (L - 1) dt, the reach at unit speed offset
   const double* vis;
   int64_t vstride, nv;
   const int32_t* vstart;
   dp_grid_geom grid;
   const int32_t* app;     // pair offsets per node [N+1]
   const int32_t* chunk_off;  // [N+1] wave chunks per node, scanned
+  const int32_t* chunk_node;  // [chunks] node of each wave chunk
   const int32_t* qperm;   // [pairs] each node's pairs, those the primitive's rate bounds reject first
   const int32_t* nshort;  // [N] how many of them
   const int32_t* pnode;
@@ -76,8 +79,8 @@ struct dp_quad_args {
   // per primitive state s: attitude / rate trajectory from the stored start
   // (k_quad_rot), shared by every pair of the primitive
   const double* rot64;    // [7][S] qx qy qz qw wx wy wz
-  const double* dv64;     // [3][S] velocity increment of the step into state s
-  const float* dv32;      // [3][S] the same in FP32
+  const double* dv64;     // [S][4] velocity increment of the step into state s (x y z, pad)
+  const float* dv32;      // [S][4] the same in FP32
   const int32_t* rbad64;  // [P] first state whose attitude / rate leaves the bounds (INT_MAX: none)
   const int32_t* rbad32;  // [P] the same in FP32
   const int32_t* rnear32; // [P] first FP32 state within the re-check margin of an attitude / rate bound
@@ -213,15 +216,15 @@ __global__ void k_quad_rot(const dp_dev_sys* __restrict__ Sp, int32_t P, const i
         const float u32[4] = {uf.x, uf.y, uf.z, uf.w};
         quad_rot_step32(K, y, u32, dvf);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          dv64[(int64_t)c * Sn + s0 + k] = dv[c];
-          dv32[(int64_t)c * Sn + s0 + k] = dvf[c];
+        for (int c = 0; c < 4; ++c) {
+          dv64[(int64_t)(s0 + k) * 4 + c] = c < 3 ? dv[c] : 0.0;
+          dv32[(int64_t)(s0 + k) * 4 + c] = c < 3 ? dvf[c] : 0.0f;
         }
       } else {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          dv64[(int64_t)c * Sn + s0] = 0.0;
-          dv32[(int64_t)c * Sn + s0] = 0.0f;
+        for (int c = 0; c < 4; ++c) {
+          dv64[(int64_t)s0 * 4 + c] = 0.0;
+          dv32[(int64_t)s0 * 4 + c] = 0.0f;
         }
       }
 #pragma unroll
@@ -250,10 +253,10 @@ __global__ void k_quad_rot(const dp_dev_sys* __restrict__ Sp, int32_t P, const i
 
 // FP32 start state of pair (node, primitive j), relative to the node:
 // stored p0 and v0 plus the node velocity (the position / velocity offset).
-__device__ __forceinline__ void quad32_start(const dp_quad_args& a, int32_t s0, const double* nvel, float* s) {
+__device__ __forceinline__ void quad32_start(const dp_quad_args& a, int32_t j, const double* nvel, float* s) {
 #pragma unroll
   for (int c = 0; c < 13; ++c) {
-    const double v = a.st[(int64_t)c * a.S + s0];
+    const double v = a.start[(int64_t)c * a.P + j];
     s[c] = (c >= 7 && c <= 9) ? (float)(v + nvel[c - 7]) : (float)v;
   }
 }
@@ -276,14 +279,7 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
   long long c_b = 0, c_c = 0, c_k = 0;
 
   for (int32_t w = blockIdx.x * kQWaves + wave; w < total_chunks; w += gridDim.x * kQWaves) {
-    // node of chunk w: last i with chunk_off[i] <= w
-    int32_t lo = 0, hi = a.N;
-    while (hi - lo > 1) {
-      const int32_t mid = (lo + hi) >> 1;
-      if (a.chunk_off[mid] <= w) lo = mid;
-      else hi = mid;
-    }
-    const int32_t node = lo;
+    const int32_t node = a.chunk_node[w];
     const int32_t tot = *a.total;
     const int32_t pbeg = a.app[node] < tot ? a.app[node] : tot;
     const int32_t pend = a.app[node + 1] < tot ? a.app[node + 1] : tot;
@@ -303,10 +299,29 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
       np[c] = a.nodes[(int64_t)c * a.Ncap + node];
       nvel[c] = a.nodes[(int64_t)(7 + c) * a.Ncap + node];
     }
-    // reach box of the node's rollouts (positions relative to the node)
+    // Reach box of this chunk's rollouts (positions relative to the node):
+    // the largest displacement of its primitives plus their duration times
+    // the node's speed, with a 0.05 m margin.  Only obstacles meeting it are
+    // staged; a state leaving it defers the pair to the FP64 re-check, which
+    // tests every obstacle that meets the pair's own path.
     const double vmax = fmax(fabs(nvel[0]), fmax(fabs(nvel[1]), fabs(nvel[2])));
-    const double reach = a.prim_reach + a.tmax * vmax + 0.5;
+    double reach = 0.0;
+    if (!short_chunk) {
+      float rl = 0.0f;
+      if (vp) {
+        const int32_t jj = a.pprim[q];
+        const int32_t len = a.off[jj + 1] - a.off[jj];
+        rl = a.preach[jj] + (float)((len - 1) * a.sys->dt * vmax);
+      }
+#pragma unroll
+      for (int d = 32; d > 0; d >>= 1) rl = fmaxf(rl, __shfl_xor(rl, d, 64));
+      reach = (double)rl + 0.05;
+    }
     const float freach = (float)reach;
+    // obstacles are staged out to the robot radius plus 2 near_eps beyond the
+    // box: an FP32 state inside it is then more than r + near_eps from every
+    // obstacle left out, on some axis, and so is its FP64 counterpart
+    const double stage_pad = S.radius + 2.0 * S.near_eps;
     // stage candidate obstacles relative to the node, ballot-compacted
     int nbc = 0, nsc = 0;
     for (int o0 = 0; o0 < (short_chunk ? 0 : a.nb); o0 += 64) {
@@ -319,7 +334,7 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const double d = b[c] - np[c];
-          keep = keep && fabs(d) <= reach + b[3 + c] + S.radius;
+          keep = keep && fabs(d) <= reach + b[3 + c] + stage_pad;
           rel[c] = (float)d;
           rel[3 + c] = (float)b[3 + c];
         }
@@ -343,7 +358,7 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const double d = b[c] - np[c];
-          keep = keep && fabs(d) <= reach + b[3] + S.radius;
+          keep = keep && fabs(d) <= reach + b[3] + stage_pad;
           rel[c] = (float)d;
         }
         rel[3] = (float)b[3];
@@ -364,14 +379,13 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
       const int32_t j = a.pprim[q];
       const int32_t s0 = a.off[j];
       const int32_t L1 = a.off[j + 1] - s0;
-      const int32_t a0 = s0 - j;
       // position (relative to the node) and velocity; the attitude / rate
       // part of every step is the primitive's (k_quad_rot)
       float s[6];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        s[c] = (float)a.st[(int64_t)c * a.S + s0];
-        s[3 + c] = (float)(a.st[(int64_t)(7 + c) * a.S + s0] + nvel[c]);
+        s[c] = (float)a.start[(int64_t)c * a.P + j];
+        s[3 + c] = (float)(a.start[(int64_t)(7 + c) * a.P + j] + nvel[c]);
       }
       const int32_t rbad = a.rbad32[j], rnear = a.rnear32[j];
       float plo[3], phi[3], vlo[3], vhi[3];
@@ -384,14 +398,18 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
       }
       int first_bad = INT_MAX, first_col = INT_MAX;
       bool near_bound_any = false, near_col_before = false, near = false;
+      float4 dnext = make_float4(0.f, 0.f, 0.f, 0.f);
       for (int k = 0; k < L1; ++k) {
         if (k > 0) {
+          const float dv[3] = {dnext.x, dnext.y, dnext.z};
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
             s[c] = s[c] + K.dt * s[3 + c];
-            s[3 + c] = s[3 + c] + a.dv32[(int64_t)c * a.S + s0 + k];
+            s[3 + c] = s[3 + c] + dv[c];
           }
         }
+        // the next step's increment is loaded while this state is checked
+        if (k + 1 < L1) dnext = reinterpret_cast<const float4*>(a.dv32)[s0 + k + 1];
         // bounds with margin (attitude and rate: the primitive's first
         // offending / near state)
         bool bad = k >= rbad, nb_ = k >= rnear;
@@ -399,7 +417,7 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
         for (int c = 0; c < 3; ++c) {
           bad = bad || !(s[c] >= plo[c] && s[c] <= phi[c]);
           nb_ = nb_ || fabsf(s[c] - plo[c]) <= eps || fabsf(s[c] - phi[c]) <= eps;
-          nb_ = nb_ || fabsf(s[c]) > freach;  // left the staged reach box
+          nb_ = nb_ || (!short_chunk && fabsf(s[c]) > freach);  // left the staged reach box
           bad = bad || !(s[3 + c] >= vlo[c] && s[3 + c] <= vhi[c]);
           nb_ = nb_ || fabsf(s[3 + c] - vlo[c]) <= eps_v || fabsf(s[3 + c] - vhi[c]) <= eps_v;
         }
@@ -491,11 +509,11 @@ __device__ __forceinline__ int64_t quad64_pair(const dp_quad_args& a, int64_t t)
 
 // Start state of pair (node i, primitive j) in FP64: stored state plus the
 // node's position / velocity (the offset dimensions).
-__device__ __forceinline__ void quad64_start(const dp_dev_sys& S, const dp_quad_args& a, int32_t i, int32_t s0,
+__device__ __forceinline__ void quad64_start(const dp_dev_sys& S, const dp_quad_args& a, int32_t i, int32_t j,
                                              double* s) {
 #pragma unroll
   for (int c = 0; c < 13; ++c) {
-    const double v = a.st[(int64_t)c * a.S + s0];
+    const double v = a.start[(int64_t)c * a.P + j];
     s[c] = dp_is_offset_dim(S, c) ? v + a.nodes[(int64_t)c * a.Ncap + i] : v;
   }
 }
@@ -503,9 +521,9 @@ __device__ __forceinline__ void quad64_start(const dp_dev_sys& S, const dp_quad_
 // State k of the pair from state k-1: translation with the primitive's
 // velocity increment (dp_quad_trans_step64); s[3..6], s[10..12] untouched.
 __device__ __forceinline__ void quad64_advance(const dp_quad_args& a, int32_t s0, int k, double* s) {
-  double dv[3], o[13];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) dv[c] = a.dv64[(int64_t)c * a.S + s0 + k];
+  const double4 d = reinterpret_cast<const double4*>(a.dv64)[s0 + k];
+  const double dv[3] = {d.x, d.y, d.z};
+  double o[13];
   dp_quad_trans_step64(a.sys->dt, s, dv, o);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -557,7 +575,6 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
     const int32_t i = a.pnode[q], j = a.pprim[q];
     const int32_t s0 = a.off[j];
     const int32_t L1 = a.off[j + 1] - s0;
-    const int32_t a0 = s0 - j;
     // s[0..2] position, s[7..9] velocity; attitude and rate come from the
     // primitive's trajectory (k_quad_rot) when a full state is needed
     double s[13];
@@ -565,7 +582,7 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
     const int32_t rbad = a.rbad64[j];
     int ncand = 0;  // obstacles listed for the second rollout (-1: more than kQ64Cand, test all)
     if (full) {
-      quad64_start(S, a, i, s0, s);
+      quad64_start(S, a, i, j, s);
       double bl[3] = {s[0], s[1], s[2]}, bh[3] = {s[0], s[1], s[2]};
       for (int k = 0; k < L1; ++k) {
         if (k > 0) quad64_advance(a, s0, k, s);
@@ -606,7 +623,7 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
       double node[13];
 #pragma unroll
       for (int c = 0; c < 13; ++c) node[c] = a.nodes[(int64_t)c * a.Ncap + i];
-      quad64_start(S, a, i, s0, s);
+      quad64_start(S, a, i, j, s);
       lb0 = dp_lower_bound_time(S, node, s);
     }
     int chosen = -1, nn = 0;
@@ -755,6 +772,12 @@ __global__ __launch_bounds__(64) void k_quad_split(int32_t N, const int32_t* __r
   }
 }
 
+// Node of every wave chunk (after the scan of the chunk counts).
+__global__ void k_quad_chunk_nodes(int32_t N, const int32_t* __restrict__ chunk_off, int32_t* __restrict__ chunk_node) {
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+    for (int32_t w = chunk_off[i]; w < chunk_off[i + 1]; ++w) chunk_node[w] = i;
+}
+
 // FP64 window of each node: the pairs the FP32 pass found FREE are ranked in
 // canonical order (near-contact pairs, re-checked in FP64 whatever their
 // place, are not counted), so a pair's rank never exceeds its rank among
@@ -833,7 +856,7 @@ __global__ void k_quad64_trace(dp_quad_args a, const int32_t* __restrict__ pairs
     const int32_t s0 = a.off[j];
     const int32_t L1 = a.off[j + 1] - s0;
     double s[13];
-    quad64_start(S, a, node, s0, s);
+    quad64_start(S, a, node, j, s);
     for (int k = 0; k < L1 && k < maxlen; ++k) {
       if (k > 0) quad64_advance(a, s0, k, s);
       quad64_rot(a, s0 + k, s);
@@ -861,7 +884,7 @@ __global__ void k_quad32_trace(dp_quad_args a, const int32_t* __restrict__ pairs
       nvel[c] = a.nodes[(int64_t)(7 + c) * a.Ncap + node];
     }
     float s[13];
-    quad32_start(a, s0, nvel, s);
+    quad32_start(a, j, nvel, s);
     for (int k = 0; k < L1 && k < maxlen; ++k) {
       if (k > 0) {
         const float4 uv = reinterpret_cast<const float4*>(a.act32)[a0 + k - 1];
@@ -889,6 +912,7 @@ dp_quad_args make_quad_args(dp_ctx* c, const dp_expand_params* p, double goal_bo
   a.off = c->d_off;
   a.st = c->d_st;
   a.S = c->S;
+  a.start = c->d_start;
   a.act = c->d_act;
   a.act32 = c->d_act32;
   a.A = c->S - c->P;
@@ -900,6 +924,7 @@ dp_quad_args make_quad_args(dp_ctx* c, const dp_expand_params* p, double goal_bo
   a.nb = c->nb;
   a.ns = c->ns;
   a.prim_reach = c->prim_reach;
+  a.preach = c->d_preach;
   a.tmax = (c->maxlen - 1) * c->desc.dt;
   a.vis = c->d_vis_sorted;
   a.vstride = c->nv_cap;
@@ -909,6 +934,7 @@ dp_quad_args make_quad_args(dp_ctx* c, const dp_expand_params* p, double goal_bo
   a.app = c->d_app;
   a.chunk_off = c->d_chunk;
   a.qperm = c->d_qperm;
+  a.chunk_node = c->d_chunk_node;
   a.nshort = c->d_nshort;
   a.pnode = c->d_pnode;
   a.pprim = c->d_pprim;
@@ -1011,6 +1037,7 @@ int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, doub
     k_quad_split<<<c->N, 64, 0, s>>>(c->N, c->d_app, c->d_total, c->d_pprim, c->d_off, c->d_rbad32, c->d_qperm,
                                      c->d_nshort, c->d_chunk);
     dp_scan_small(c->d_chunk, c->N, nullptr, INT64_MAX, c->d_counters, -1, s);
+    k_quad_chunk_nodes<<<dp_grid_blocks(c->N, 256, 1024), 256, 0, s>>>(c->N, c->d_chunk, c->d_chunk_node);
     const int64_t max_chunks = (int64_t)c->N * ((c->P + 63) / 64);
     int rc = dp_record_event(c, 5, s);
     if (rc) return rc;

@@ -50,6 +50,6 @@ def test_quadrotor_shapes_and_autotune(need_gpu):
         ex.upload_nodes(wl.nodes, wl.g_parent)
         best, trials = ex.autotune(wl.params)
         print("autotune", best)
-        assert len(trials) == 27 and best["check_ms"] == min(t["check_ms"] for t in trials)
+        assert len(trials) == 36 and best["check_ms"] == min(t["check_ms"] for t in trials)
         ex.expand(wl.nodes, wl.g_parent, wl.params)
         same(ex.fetch(), ref)
