@@ -24,18 +24,12 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
-FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32, public spec
-FP64_PEAK_TFLOPS = 78.6   # MI355X vector FP64, public spec
-# HBM bytes per launch of the check kernel, measured with rocprofv3 --pmc
-# FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy (profiles/ holds the run).
+# HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy (profiles/ holds the runs):
+# workload -> (kernel group, bytes, source); used when that group dominates.
 MEASURED_TRAFFIC = {
-    "unicycle1_scale": (1.386e9, "profiles/k_check_planar_unicycle1_scale_hbm_traffic.txt"),
-}
-# HBM bytes per launch of the resolution's two large kernels (k_conf_waves,
-# k_resolve_ticketed), from the same counter runs.  The resolution is bound
-# by its dependency chain, not by HBM; the figure shows how far from it.
-MEASURED_RESOLVE_TRAFFIC = {
-    "unicycle1_scale": (0.706e9, "profiles/k_check_planar_unicycle1_scale_hbm_traffic.txt"),
+    "quadrotor": ("check_main", 97.0e6, "profiles/k_quad32_quadrotor_hbm_traffic.txt"),
+    "unicycle1_scale": ("check_main", 1.386e9, "profiles/k_check_planar_unicycle1_scale_hbm_traffic.txt"),
 }
 
 
@@ -443,8 +437,8 @@ def main():
                      "algorithmic_bytes_per_launch": abytes, "launch_ms": dom_ms},
     }
     key = args.workload
-    if key in MEASURED_TRAFFIC and not use_sur:
-        line["roofline"]["traffic"], line["roofline"]["traffic_source"] = MEASURED_TRAFFIC[key]
+    if key in MEASURED_TRAFFIC and not use_sur and MEASURED_TRAFFIC[key][0] == dominant:
+        _, line["roofline"]["traffic"], line["roofline"]["traffic_source"] = MEASURED_TRAFFIC[key]
     if is_quad:
         line["fp32_vs_fp64"] = rollout_error(ex, out)
         progress(f"FP32 vs FP64 rollout error: {line['fp32_vs_fp64']}")
