@@ -29,7 +29,7 @@ SYS_UNICYCLE2 = 1
 SYS_QUADROTOR = 2
 SYS_CAR_TRAILER = 3
 
-V_FREE, V_BOUNDS, V_COLLIDE, V_SURROGATE_REJECT = 0, 1, 2, 3
+V_FREE, V_BOUNDS, V_COLLIDE, V_SURROGATE_REJECT, V_CONSTRAINT = 0, 1, 2, 3, 4
 BATCH_GRAPH, BATCH_STREAMS = 0, 1
 R_NONE, R_NOVEL, R_DUP_VISITED, R_DUP_BATCH, R_CAPPED = 0, 1, 2, 3, 4
 
@@ -83,10 +83,21 @@ class Counts(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
         "nodes", "pairs", "bounds", "collide", "free_", "surrogate_reject", "novel",
         "dup_visited", "dup_batch", "capped", "goal_hits", "checked", "recheck_fp64",
-        "resolve_rounds", "conf_pool_used", "fp64_deferred")]
+        "resolve_rounds", "conf_pool_used", "fp64_deferred", "constraint")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+class SelectParams(C.Structure):
+    _fields_ = [("radius", C.c_double), ("cost_jump", C.c_double), ("choose_first", C.c_int32),
+                ("reserved", C.c_int32), ("goal", C.c_double * DP_MAX_NX)]
+
+
+class SelectOut(C.Structure):
+    _fields_ = [("prim", C.POINTER(C.c_int32)), ("pair", C.POINTER(C.c_int64)), ("distance", C.POINTER(C.c_double)),
+                ("state", C.POINTER(C.c_double)), ("start", C.POINTER(C.c_double)), ("g", C.POINTER(C.c_double)),
+                ("h", C.POINTER(C.c_double))]
 
 
 class HealthInfo(C.Structure):
@@ -211,6 +222,12 @@ SIGNATURES = {
     "dp_debug_rollout_states": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
                                           C.POINTER(C.c_double)]),
     "dp_surrogate_get": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "dp_set_order": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32]),
+    "dp_set_direction": (C.c_int, [C.c_void_p, C.c_int32]),
+    "dp_set_constraints": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32]),
+    "dp_expand_select": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(SelectParams), C.c_void_p]),
+    "dp_fetch_select": (C.c_int, [C.c_void_p, C.POINTER(SelectOut)]),
+    "dp_reverse_primitives": (C.c_int, [C.POINTER(Prims), C.c_int32, C.c_double, C.c_uint64, C.POINTER(PrimSet)]),
     "dp_search_options_default": (None, [C.POINTER(SearchOptions)]),
     "dp_search": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(SearchOptions),
                             C.POINTER(SearchInfo)]),

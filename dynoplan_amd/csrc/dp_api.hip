@@ -228,6 +228,13 @@ void dp_destroy(dp_ctx* c) {
   dev_free(&c->d_off);
   dev_free(&c->d_st);
   dev_free(&c->d_start);
+  dev_free(&c->d_last);
+  dev_free(&c->d_cost);
+  dev_free(&c->d_order);
+  dev_free(&c->d_con);
+  dev_free(&c->d_targets);
+  dev_free(&c->d_sel_q);
+  dev_free(&c->d_sel);
   dev_free(&c->d_geo);
   dev_free(&c->d_sweep);
   dev_free(&c->d_srange);
@@ -314,11 +321,16 @@ int dp_upload_primitives(dp_ctx* c, const dp_prims* pr, int32_t max_motions) {
   }
   const int32_t S = off[P];
   const double* st = pr->states;
-  std::vector<double> soa((size_t)nx * S), start((size_t)nx * P);
+  std::vector<double> soa((size_t)nx * S), start((size_t)nx * P), last((size_t)nx * P), cost((size_t)P);
   for (int32_t s = 0; s < S; ++s)
     for (int k = 0; k < nx; ++k) soa[(size_t)k * S + s] = st[(int64_t)s * nx + k];
   for (int32_t j = 0; j < P; ++j)
-    for (int k = 0; k < nx; ++k) start[(size_t)k * P + j] = st[(int64_t)off[j] * nx + k];
+    for (int k = 0; k < nx; ++k) {
+      start[(size_t)k * P + j] = st[(int64_t)off[j] * nx + k];
+      last[(size_t)k * P + j] = st[(int64_t)(off[j + 1] - 1) * nx + k];
+    }
+  // Motion::cost (dbrrt.cpp:645-648): as given, else ref_dt per action
+  for (int32_t j = 0; j < P; ++j) cost[j] = pr->cost ? pr->cost[j] : (off[j + 1] - off[j] - 1) * c->desc.dt;
 
   // Footprint geometry per state (compute_col_shape, robots.cpp:3190-3215):
   // translation leaves orientation unchanged, so cos/sin and the trailer's
@@ -371,6 +383,26 @@ int dp_upload_primitives(dp_ctx* c, const dp_prims* pr, int32_t max_motions) {
       c->reach_box[2] = std::min(c->reach_box[2], sweep[(size_t)2 * P + j]);
       c->reach_box[3] = std::max(c->reach_box[3], sweep[(size_t)3 * P + j]);
     }
+    // The reverse search places primitive j at node - last_j: its swept box
+    // relative to the node is sweep_j - last_j.  The pair test rounds
+    // sweep_j + (node - last_j) instead, a few ulps away: the union box is
+    // widened by 1e-6 (relative to the extents) to stay conservative.
+    double rb[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+    double ext = 1.0;
+    for (int32_t j = 0; j < P; ++j) {
+      const double lx = last[j], ly = last[(size_t)P + j];
+      rb[0] = std::min(rb[0], sweep[j] - lx);
+      rb[1] = std::max(rb[1], sweep[(size_t)P + j] - lx);
+      rb[2] = std::min(rb[2], sweep[(size_t)2 * P + j] - ly);
+      rb[3] = std::max(rb[3], sweep[(size_t)3 * P + j] - ly);
+      ext = std::max({ext, std::fabs(sweep[j]), std::fabs(sweep[(size_t)P + j]), std::fabs(sweep[(size_t)2 * P + j]),
+                      std::fabs(sweep[(size_t)3 * P + j]), std::fabs(lx), std::fabs(ly)});
+    }
+    const double wpad = 1e-6 * ext;
+    c->reach_box_bwd[0] = rb[0] - wpad;
+    c->reach_box_bwd[1] = rb[1] + wpad;
+    c->reach_box_bwd[2] = rb[2] - wpad;
+    c->reach_box_bwd[3] = rb[3] + wpad;
   }
   // Per-primitive range of every state component: the check kernel proves
   // "no state out of bounds" from it without reading the states.  A NaN
@@ -396,12 +428,18 @@ int dp_upload_primitives(dp_ctx* c, const dp_prims* pr, int32_t max_motions) {
   if ((rc = dev_alloc(&c->d_off, P + 1))) return rc;
   if ((rc = dev_alloc(&c->d_st, (int64_t)nx * S))) return rc;
   if ((rc = dev_alloc(&c->d_start, (int64_t)nx * P))) return rc;
+  if ((rc = dev_alloc(&c->d_last, (int64_t)nx * P))) return rc;
+  if ((rc = dev_alloc(&c->d_cost, P))) return rc;
   if ((rc = dev_alloc(&c->d_sweep, (int64_t)4 * P))) return rc;
   if ((rc = dev_alloc(&c->d_srange, (int64_t)2 * nx * P))) return rc;
   if ((rc = dev_alloc(&c->d_goalidx, (int64_t)8 * P))) return rc;
   HIP_CHECK(hipMemcpy(c->d_off, off.data(), sizeof(int32_t) * (P + 1), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(c->d_st, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(c->d_start, start.data(), sizeof(double) * start.size(), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(c->d_last, last.data(), sizeof(double) * last.size(), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(c->d_cost, cost.data(), sizeof(double) * P, hipMemcpyHostToDevice));
+  dev_free(&c->d_order);  // a pair order refers to the previous primitive count
+  c->order_rows = 0;
   HIP_CHECK(hipMemcpy(c->d_sweep, sweep.data(), sizeof(double) * sweep.size(), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(c->d_srange, srange.data(), sizeof(double) * srange.size(), hipMemcpyHostToDevice));
   if (!quad) {
@@ -628,6 +666,7 @@ int dp_upload_nodes(dp_ctx* c, const double* nodes, const double* g_parent, int3
   HIP_CHECK(hipMemcpy(c->d_gpar, g.data(), sizeof(double) * N, hipMemcpyHostToDevice));
   if (c->N != N) dp_bump_epoch();
   c->N = N;
+  c->sel_valid = false;
   return DP_OK;
 }
 
@@ -684,6 +723,7 @@ int dp_expand(dp_ctx* c, const dp_expand_params* p, void* stream) {
   ARG_CHECK(p->alpha > 0.0f && p->alpha < 1.0f, "Alpha needs to be between 0 and 1!");  // dbastar.cpp:588-590
   ARG_CHECK(p->delta >= 0.0f, "delta < 0 is not implemented");                          // dbastar.cpp:592-594
   ARG_CHECK(p->num_check_goal >= 0 && p->num_check_goal <= 8, "num_check_goal must be in [0, 8]");
+  ARG_CHECK(!(p->use_surrogate && !c->forward), "the surrogate scores forward placements only");
   if (c->P <= 0 || !c->have_world || c->N <= 0) {
     dp_set_error("upload primitives, world and nodes before dp_expand");
     return DP_ERR_STATE;
@@ -696,8 +736,151 @@ int dp_expand(dp_ctx* c, const dp_expand_params* p, void* stream) {
   if (rc) return rc;
   rc = dp_launch_expand(c, p, s);
   c->last_pairs = -1;
+  c->sel_valid = false;
   if (!rc) ++c->n_batches;
   return rc;
+}
+
+int dp_set_order(dp_ctx* c, const int32_t* order, int32_t rows) {
+  ARG_CHECK(c && rows >= 0 && (rows == 0 || order), "invalid argument");
+  if (c->P <= 0) {
+    dp_set_error("upload primitives before dp_set_order");
+    return DP_ERR_STATE;
+  }
+  HIP_CHECK(hipSetDevice(c->device));
+  if (rows == 0) {
+    HIP_CHECK(hipDeviceSynchronize());
+    dev_free(&c->d_order);
+    c->order_rows = 0;
+    dp_bump_epoch();
+    return DP_OK;
+  }
+  const int32_t P = c->P;
+  std::vector<uint8_t> seen((size_t)P);
+  for (int32_t r = 0; r < rows; ++r) {
+    std::fill(seen.begin(), seen.end(), 0);
+    for (int32_t k = 0; k < P; ++k) {
+      const int32_t j = order[(int64_t)r * P + k];
+      ARG_CHECK(j >= 0 && j < P && !seen[j], "a row of the pair order is not a permutation of 0..P-1");
+      seen[j] = 1;
+    }
+  }
+  HIP_CHECK(hipDeviceSynchronize());  // a batch in flight may read the previous order
+  int rc = dev_alloc(&c->d_order, (int64_t)rows * P);
+  if (rc) return rc;
+  HIP_CHECK(hipMemcpy(c->d_order, order, sizeof(int32_t) * (size_t)rows * P, hipMemcpyHostToDevice));
+  c->order_rows = rows;
+  return DP_OK;
+}
+
+int dp_set_direction(dp_ctx* c, int32_t forward) {
+  ARG_CHECK(c, "null context");
+  ARG_CHECK(forward || c->desc.system != DP_SYS_QUADROTOR, "the reverse search is defined for planar systems");
+  c->forward = forward ? 1 : 0;
+  dp_bump_epoch();
+  return DP_OK;
+}
+
+int dp_set_constraints(dp_ctx* c, const double* times, const double* states, int32_t n) {
+  ARG_CHECK(c && n >= 0 && (n == 0 || (times && states)), "invalid argument");
+  ARG_CHECK(n == 0 || c->desc.system != DP_SYS_QUADROTOR, "constraints are defined for planar systems");
+  HIP_CHECK(hipSetDevice(c->device));
+  HIP_CHECK(hipDeviceSynchronize());  // a batch in flight may read the previous list
+  const int nx = c->desc.nx;
+  if (n > c->con_cap) {
+    int rc = dev_alloc(&c->d_con, (int64_t)n * (nx + 1));
+    if (rc) return rc;
+    c->con_cap = n;
+  }
+  if (n > 0) {
+    std::vector<double> h((size_t)n * (nx + 1));
+    for (int32_t k = 0; k < n; ++k) h[k] = times[k];
+    for (int64_t k = 0; k < (int64_t)n * nx; ++k) h[(size_t)n + k] = states[k];
+    HIP_CHECK(hipMemcpy(c->d_con, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+  }
+  c->ncon = n;
+  dp_bump_epoch();
+  return DP_OK;
+}
+
+int dp_expand_select(dp_ctx* c, const double* targets, const dp_select_params* sp, void* stream) {
+  ARG_CHECK(c && targets && sp, "null argument");
+  ARG_CHECK(sp->radius >= 0.0, "negative radius");
+  ARG_CHECK(c->desc.system != DP_SYS_QUADROTOR, "dp_expand_select is defined for planar systems");
+  if (!c->forward || c->ncon > 0) {
+    dp_set_error("dp_expand_select runs forward without constraints (dbrrt.cpp:580-660)");
+    return DP_ERR_STATE;
+  }
+  if (c->P <= 0 || !c->have_world || c->N <= 0) {
+    dp_set_error("upload primitives, world and nodes before dp_expand_select");
+    return DP_ERR_STATE;
+  }
+  HIP_CHECK(hipSetDevice(c->device));
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  const int nx = c->desc.nx;
+  int rc;
+  if (c->targets_cap < c->N_cap) {
+    if ((rc = dev_alloc(&c->d_targets, (int64_t)nx * c->N_cap))) return rc;
+    c->targets_cap = c->N_cap;
+  }
+  if (c->sel_cap < c->N_cap) {
+    if ((rc = dev_alloc(&c->d_sel_q, 2 * (int64_t)c->N_cap))) return rc;  // pair, then primitive
+    if ((rc = dev_alloc(&c->d_sel, (int64_t)(2 * nx + 3) * c->N_cap))) return rc;
+    c->sel_cap = c->N_cap;
+  }
+  std::vector<double> soa((size_t)nx * c->N_cap, 0.0);
+  for (int32_t i = 0; i < c->N; ++i)
+    for (int k = 0; k < nx; ++k) soa[(size_t)k * c->N_cap + i] = targets[(int64_t)i * nx + k];
+  HIP_CHECK(hipStreamSynchronize(s));
+  HIP_CHECK(hipMemcpy(c->d_targets, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
+  dp_expand_params p{};
+  p.delta = 1.0f;  // the duplicate radius is not used: no duplicate test runs
+  p.alpha = 0.5f;
+  p.limit_branching_factor = -1;
+  for (int k = 0; k < DP_MAX_NX; ++k) p.goal[k] = sp->goal[k];
+  c->select_mode = true;
+  c->select_radius = sp->radius;
+  c->select_cost_jump = sp->cost_jump;
+  c->select_first = sp->choose_first ? 1 : 0;
+  rc = dp_size_pairs(c, &p, s);
+  if (!rc) rc = dp_launch_expand(c, &p, s);
+  c->select_mode = false;
+  c->last_pairs = -1;
+  c->sel_valid = rc == DP_OK;
+  if (!rc) ++c->n_batches;
+  return rc;
+}
+
+int dp_fetch_select(dp_ctx* c, dp_select_out* o) {
+  ARG_CHECK(c && o, "null argument");
+  if (!c->sel_valid) {
+    dp_set_error("no dp_expand_select since the last expansion");
+    return DP_ERR_STATE;
+  }
+  dp_counts cnt;
+  int rc = dp_get_counts(c, &cnt);
+  if (rc) return rc;
+  const int nx = c->desc.nx;
+  const int32_t N = c->N;
+  std::vector<int32_t> q((size_t)N);
+  std::vector<double> v((size_t)N * (2 * nx + 3));
+  HIP_CHECK(hipMemcpy(q.data(), c->d_sel_q, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(v.data(), c->d_sel, sizeof(double) * v.size(), hipMemcpyDeviceToHost));
+  std::vector<int32_t> prim((size_t)N);
+  HIP_CHECK(hipMemcpy(prim.data(), c->d_sel_q + c->sel_cap, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+  for (int32_t i = 0; i < N; ++i) {
+    const double* r = v.data() + (size_t)i * (2 * nx + 3);
+    if (o->prim) o->prim[i] = prim[i];
+    if (o->pair) o->pair[i] = q[i];
+    if (o->distance) o->distance[i] = r[2 * nx];
+    for (int k = 0; k < nx; ++k) {
+      if (o->state) o->state[(int64_t)i * nx + k] = r[k];
+      if (o->start) o->start[(int64_t)i * nx + k] = r[nx + k];
+    }
+    if (o->g) o->g[i] = r[2 * nx + 1];
+    if (o->h) o->h[i] = r[2 * nx + 2];
+  }
+  return DP_OK;
 }
 
 int dp_expand_batch(dp_ctx* c, const double* nodes, const double* g_parent, int32_t N,
@@ -765,6 +948,7 @@ int dp_get_counts(dp_ctx* c, dp_counts* o) {
   o->resolve_rounds = h[DPC_ROUNDS];
   o->conf_pool_used = h[DPC_CONF];
   o->fp64_deferred = deferred;
+  o->constraint = h[DPC_CONSTRAINT];
   if (h[DPC_OVERFLOW]) {
     dp_set_error("pair capacity exceeded");
     return DP_ERR_CAPACITY;

@@ -57,7 +57,8 @@ void dp_set_error(const std::string& msg);
 enum {
   DPC_PAIRS = 0, DPC_BOUNDS, DPC_COLLIDE, DPC_FREE, DPC_SURR, DPC_NOVEL, DPC_DUPV, DPC_DUPB,
   DPC_CAPPED, DPC_GOAL, DPC_CHECKED, DPC_RECHECK, DPC_ROUNDS, DPC_ERRFLAG, DPC_OVERFLOW,
-  DPC_NODES_LEFT, DPC_CONF, DPC_TICKET, DPC_CELLS, DPC_WINDOW_OVF, DPC_DEFERRED, DPC_Q64, DPC_COUNT
+  DPC_NODES_LEFT, DPC_CONF, DPC_TICKET, DPC_CELLS, DPC_WINDOW_OVF, DPC_DEFERRED, DPC_Q64, DPC_CONSTRAINT,
+  DPC_COUNT
 };
 // Sticky counters (never reset by a batch): resolutions finished by
 // host-launched rounds after a wave gave up, FP64 window overflows.
@@ -168,6 +169,8 @@ struct dp_ctx {
   int32_t* d_off = nullptr;      // P+1
   double* d_st = nullptr;        // [nx][S]
   double* d_start = nullptr;     // [nx][P]
+  double* d_last = nullptr;      // [nx][P] last states (applicability key of the reverse search)
+  double* d_cost = nullptr;      // [P] motion cost (given, or ref_dt per action)
   double* d_geo = nullptr;       // [4*nbox][S] : relx rely c s (per box)
   double* d_sweep = nullptr;     // [4][P] canonical swept footprint box (planar) / centre box (quad)
   double* d_srange = nullptr;    // [2][nx][P] per-primitive min / max of each state component (canonical)
@@ -251,6 +254,22 @@ struct dp_ctx {
   double prim_reach = 0.0;          // max over primitives of max_k |p_k - p_0|_inf
   float* d_preach = nullptr;        // [P] quadrotor: max_k |p_k - p_0|_inf per primitive (rounded up)
   double reach_box[4] = {0, 0, 0, 0};  // planar: union of the canonical swept boxes (lo x, hi x, lo y, hi y)
+  double reach_box_bwd[4] = {0, 0, 0, 0};  // the same relative to each primitive's last position, widened
+  // settings of the other callers (tdbastar, dbrrt)
+  int32_t* d_order = nullptr;       // [order_rows][P] pair order per node (NULL: ascending)
+  int32_t order_rows = 0;
+  int32_t forward = 1;              // 0: tdbastar's reverse search
+  int32_t ncon = 0, con_cap = 0;    // tdbastar constraints
+  double* d_con = nullptr;          // [con_cap] times, then [con_cap][nx] states
+  double* d_targets = nullptr;      // [nx][N_cap] dbrrt targets of dp_expand_select
+  int32_t targets_cap = 0;
+  int32_t* d_sel_q = nullptr;       // [N_cap] chosen pair per node (-1: none)
+  double* d_sel = nullptr;          // [N_cap][2 nx + 3] last state, first state, distance, g, h
+  int32_t sel_cap = 0;
+  bool sel_valid = false;           // the last expansion was a dp_expand_select
+  bool select_mode = false;         // set while dp_expand_select enqueues its kernels
+  double select_radius = 0.0, select_cost_jump = 0.0;
+  int32_t select_first = 0;
   int16_t* d_node_obs = nullptr;    // [N_cap][DP_NODE_OBS] obstacles within a node's reach box
   int32_t* d_node_nobs = nullptr;   // [N_cap] their count (-1: more than DP_NODE_OBS)
   dp_mlp mlp;

@@ -673,6 +673,77 @@ int dp_load_primitives(const char* path, int32_t format, int32_t system, int32_t
   return DP_OK;
 }
 
+// reverse_motions (dbrrt.cpp:252-339) for the translation-invariant
+// unicycles: translate by minus the last position (robot.offset of the last
+// state, transform_primitive(-offset)), reverse states and actions, perturb
+// the new first and last states.  The reversed primitive's cost is ref_dt per
+// action (traj_to_motion copies the trajectory's own cost, :3187, which the
+// translated copy does not carry over).
+int dp_reverse_primitives(const dp_prims* in, int32_t system, double noise, uint64_t seed, dp_prim_set* out) {
+  if (!in || !out || !in->offsets || !in->states || in->P <= 0) {
+    dp_set_error("invalid argument");
+    return DP_ERR_ARG;
+  }
+  std::memset(out, 0, sizeof(*out));
+  if (system != DP_SYS_UNICYCLE1 && system != DP_SYS_UNICYCLE2) {
+    dp_set_error("reverse of motion primitives only working for translation invariant systems (unicycles)");
+    return DP_ERR_ARG;
+  }
+  dp_system_desc sys;
+  if (dp_system_default(system, &sys) != DP_OK) return DP_ERR_ARG;
+  const int nx = sys.nx, nu = sys.nu;
+  if (in->nx != nx || (in->actions && in->nu != nu)) {
+    dp_set_error("primitive sizes do not match the system");
+    return DP_ERR_ARG;
+  }
+  const int32_t P = in->P;
+  const int64_t S = in->offsets[P];
+  out->P = P;
+  out->nx = nx;
+  out->nu = nu;
+  out->offsets = (int32_t*)std::malloc(sizeof(int32_t) * (P + 1));
+  out->states = (double*)std::malloc(sizeof(double) * S * nx);
+  out->actions = (double*)std::malloc(sizeof(double) * (S - P > 0 ? S - P : 1) * nu);
+  out->cost = (double*)std::malloc(sizeof(double) * P);
+  if (!out->offsets || !out->states || !out->actions || !out->cost) {
+    dp_free_primitives(out);
+    dp_set_error("out of host memory");
+    return DP_ERR_NOMEM;
+  }
+  uint64_t st = seed;
+  for (int32_t j = 0; j <= P; ++j) out->offsets[j] = in->offsets[j];
+  for (int32_t j = 0; j < P; ++j) {
+    const int32_t s0 = in->offsets[j], L1 = in->offsets[j + 1] - s0;
+    if (L1 < 1) {
+      dp_free_primitives(out);
+      dp_set_error("primitive with no state");
+      return DP_ERR_ARG;
+    }
+    const double* last = in->states + (int64_t)(s0 + L1 - 1) * nx;
+    const double ox = last[0], oy = last[1];
+    for (int32_t k = 0; k < L1; ++k) {
+      const double* src = in->states + (int64_t)(s0 + L1 - 1 - k) * nx;
+      double* dst = out->states + (int64_t)(s0 + k) * nx;
+      for (int c = 0; c < nx; ++c) dst[c] = src[c];
+      dst[0] = src[0] + -ox;  // transform_primitive(-offset)
+      dst[1] = src[1] + -oy;
+    }
+    const int32_t L = L1 - 1;
+    for (int32_t k = 0; k < L; ++k)
+      for (int u = 0; u < nu; ++u)
+        out->actions[(int64_t)(s0 - j + k) * nu + u] =
+            in->actions ? in->actions[(int64_t)(s0 - j + L - 1 - k) * nu + u] : 0.0;
+    if (noise != 0.0) {
+      double* f = out->states + (int64_t)s0 * nx;
+      double* b = out->states + (int64_t)(s0 + L1 - 1) * nx;
+      for (int c = 0; c < nx; ++c) f[c] += noise * ((double)(splitmix64(st) >> 11) * 0x1.0p-53 * 2.0 - 1.0);
+      for (int c = 0; c < nx; ++c) b[c] += noise * ((double)(splitmix64(st) >> 11) * 0x1.0p-53 * 2.0 - 1.0);
+    }
+    out->cost[j] = sys.dt * (double)L;
+  }
+  return DP_OK;
+}
+
 void dp_free_primitives(dp_prim_set* set) {
   if (!set) return;
   std::free(set->offsets);

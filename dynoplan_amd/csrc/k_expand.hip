@@ -17,6 +17,7 @@
 //   k_grid.hip        contender grid, earlier-conflict lists and the round-
 //                     based duplicate / branching-cap resolution (:856-918)
 #include <hip/hip_runtime.h>
+#include <float.h>
 #include <limits.h>
 
 #include <cmath>
@@ -74,10 +75,13 @@ __device__ __forceinline__ bool app_reject(const dp_dev_sys& S, const app_bounds
 // evaluating the metric again.
 constexpr int kAppParts = 4;
 
+// With a pair order (order != nullptr, [rows][P]) bit r of node i's mask is
+// primitive order[(i % rows) * P + r]: the pairs come out in that order.
 template <int NX>
 __global__ __launch_bounds__(64) void k_app_count(const dp_dev_sys* __restrict__ Sp, app_bounds B,
                                                   const double* __restrict__ nodes, int32_t Ncap, int32_t N,
                                                   const double* __restrict__ start, int32_t P, double r_app,
+                                                  const int32_t* __restrict__ order, int32_t rows,
                                                   int32_t* __restrict__ app, uint64_t* __restrict__ mask) {
   const dp_dev_sys& S = *Sp;
   const int32_t i = blockIdx.x / kAppParts, part = blockIdx.x % kAppParts;
@@ -85,14 +89,16 @@ __global__ __launch_bounds__(64) void k_app_count(const dp_dev_sys* __restrict__
   const int lane = threadIdx.x;
   const int32_t words = (P + 63) / 64;
   const int32_t w0 = (int32_t)((int64_t)words * part / kAppParts), w1 = (int32_t)((int64_t)words * (part + 1) / kAppParts);
+  const int32_t* row = order ? order + (int64_t)(i % rows) * P : nullptr;
   double a[NX];
 #pragma unroll
   for (int c = 0; c < NX; ++c) a[c] = dp_is_offset_dim(S, c) ? 0.0 : nodes[(int64_t)c * Ncap + i];
   int32_t cnt = 0;
   for (int32_t j0 = 64 * w0; j0 < P && j0 < 64 * w1; j0 += 64) {
-    const int32_t j = j0 + lane;
+    const int32_t r = j0 + lane;
+    const int32_t j = row && r < P ? row[r] : r;
     bool hit = false;
-    if (j < P && !app_reject<NX>(S, B, a, start, P, j, r_app)) {
+    if (r < P && !app_reject<NX>(S, B, a, start, P, j, r_app)) {
       double b[NX];
 #pragma unroll
       for (int c = 0; c < NX; ++c) b[c] = start[(int64_t)c * P + j];
@@ -106,12 +112,14 @@ __global__ __launch_bounds__(64) void k_app_count(const dp_dev_sys* __restrict__
 }
 
 __global__ __launch_bounds__(64) void k_app_fill(int32_t N, int32_t P, const int32_t* __restrict__ app,
-                                                 const uint64_t* __restrict__ mask, int32_t* __restrict__ pnode,
+                                                 const uint64_t* __restrict__ mask, const int32_t* __restrict__ order,
+                                                 int32_t rows, int32_t* __restrict__ pnode,
                                                  int32_t* __restrict__ pprim, int64_t pair_cap) {
   const int32_t i = blockIdx.x;
   if (i >= N) return;
   const int lane = threadIdx.x;
   const int32_t words = (P + 63) / 64;
+  const int32_t* row = order ? order + (int64_t)(i % rows) * P : nullptr;
   int64_t q0 = app[i];
   for (int32_t w = 0; w < words; ++w) {
     const uint64_t m = mask[(int64_t)i * words + w];
@@ -119,7 +127,7 @@ __global__ __launch_bounds__(64) void k_app_fill(int32_t N, int32_t P, const int
       const int64_t q = q0 + __popcll(m & ((1ull << lane) - 1ull));
       if (q < pair_cap) {
         pnode[q] = i;
-        pprim[q] = w * 64 + lane;
+        pprim[q] = row ? row[w * 64 + lane] : w * 64 + lane;
       }
     }
     q0 += __popcll(m);
@@ -169,6 +177,16 @@ struct dp_check_args {
   double goal_bound, r_dup, cdf;
   int32_t surrogate;  // pairs already marked DP_V_SURROGATE_REJECT are skipped
   long long* counters;
+  // tdbastar's reverse search: placement offset = node - last position of
+  // the primitive, new state = first placed state
+  int32_t backward;
+  const double* last;  // [nx][P]
+  // tdbastar constraints: times [ncon], then states [ncon][nx]
+  int32_t ncon;
+  const double* con;
+  double con_delta;
+  // dbrrt selection: no goal check or duplicate test, resolution DP_R_NONE
+  int32_t select;
 };
 
 namespace {
@@ -234,7 +252,7 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
   int16_t* cand = s_cand + (int64_t)(wave * SEG + seg) * nobs;
   const int32_t total = *a.total;
   const double Tdup = dp_pos_bound(S, a.r_dup);
-  long long c_b = 0, c_c = 0, c_f = 0, c_k = 0;
+  long long c_b = 0, c_c = 0, c_f = 0, c_k = 0, c_x = 0;
 
   for (int64_t q0 = ((int64_t)blockIdx.x * kWaves + wave) * SEG; q0 < total;
        q0 += (int64_t)gridDim.x * kWaves * SEG) {
@@ -249,6 +267,10 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
       L1 = a.off[j + 1] - st0;
       ox = a.nodes[i];
       oy = a.nodes[(int64_t)a.Ncap + i];
+      if (a.backward) {  // offset - motion->traj.states.back().head(2), tdbastar.hpp:193-196
+        ox = ox - a.last[j];
+        oy = oy - a.last[(int64_t)a.P + j];
+      }
     }
     // ---- is_state_valid on every translated state -----------------------
     // First from the primitive's component ranges: with the translation
@@ -349,10 +371,51 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
       if (need_col && first_col != INT_MAX) vd = DP_V_COLLIDE;
     }
 
+    // ---- tdbastar constraints on a valid pair (tdbastar.cpp:408-443) -----
+    // LPP constraints at a time; the first violated one in list order wins.
+    if (a.ncon > 0 && __ballot(vp && vd == DP_V_FREE)) {
+      const bool nc = vp && vd == DP_V_FREE;
+      bool reaches = false;
+      float gf = 0.0f;
+      if (nc) {
+        double s[NX];
+        load_translated<NX>(a, (int64_t)st0 + (a.backward ? L1 - 1 : 0), ox, oy, s);
+        reaches = dp_distance(S, s, a.goal) <= a.con_delta;
+        gf = (float)a.gpar[i];  // best_node_gScore is passed as float
+      }
+      int viol_k = -1;
+      for (int cb = 0; __ballot(nc && viol_k < 0 && cb < a.ncon) != 0; cb += LPP) {
+        const int ci = cb + k0;
+        bool viol = false;
+        int k = -1;
+        if (nc && viol_k < 0 && ci < a.ncon) {
+          const float toff = (float)(a.con[ci] - (double)gf);
+          const int ti = (int)lround((double)toff / S.dt);
+          if (reaches && ti >= L1 - 1) k = L1 - 1;
+          if (ti >= 0 && ti < L1 - 1) k = ti;
+          if (k >= 0) {
+            double s[NX], cs[NX];
+            load_translated<NX>(a, (int64_t)st0 + k, ox, oy, s);
+#pragma unroll
+            for (int c = 0; c < NX; ++c) cs[c] = a.con[a.ncon + (int64_t)ci * NX + c];
+            viol = dp_distance(S, s, cs) <= a.con_delta;
+          }
+        }
+        const uint64_t m = (__ballot(viol) >> shift) & segbits;
+        const int src = m ? shift + __ffsll((long long)m) - 1 : lane;
+        const int kk = __shfl(k, src);
+        if (nc && viol_k < 0 && m) viol_k = kk;
+      }
+      if (nc && viol_k >= 0) {
+        vd = DP_V_CONSTRAINT;
+        first_col = viol_k;
+      }
+    }
+
     // ---- FREE: check_goal, g, h, duplicate vs visited --------------------
     const bool fr = vp && vd == DP_V_FREE;
     int chosen = -1;
-    if (a.check_goal && __ballot(fr)) {
+    if (a.check_goal && !a.select && __ballot(fr)) {
       // goal indices LPP at a time; the first in index order that hits wins
       for (int gb = 0; gb < a.num_goal; gb += LPP) {
         const int gk = gb + k0;
@@ -370,7 +433,9 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
     double xf[NX];
     double gq = 0.0, hq = 0.0;
     if (fr) {
-      const int fi = chosen != -1 ? chosen : L1 - 1;
+      // the new node's state: the last placed state, or the first for the
+      // reverse search (check_goal, tdbastar.cpp:453-458)
+      const int fi = chosen != -1 ? chosen : (a.backward ? 0 : L1 - 1);
       load_translated<NX>(a, (int64_t)st0 + fi, ox, oy, xf);
       hq = dp_lower_bound_time(S, xf, a.goal);
       const double cm = chosen != -1 ? chosen * S.dt : (L1 - 1) * S.dt;
@@ -384,7 +449,7 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
       for (int c = 0; c < NX; ++c) xf[c] = 0.0;
     }
     bool dupv = false;
-    if (a.nv > 0 && __ballot(fr)) {
+    if (a.nv > 0 && !a.select && __ballot(fr)) {
       bool hit = false;
       // the 9 runs over the pair's lanes (one round from 16 lanes per pair)
       for (int k = k0; fr && k < 9 && !hit; k += LPP)
@@ -393,9 +458,9 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
     }
     if (vp && k0 == 0) {
       a.verdict[q] = vd;
-      a.firstk[q] = vd == DP_V_BOUNDS ? first_bad : (vd == DP_V_COLLIDE ? first_col : -1);
+      a.firstk[q] = vd == DP_V_BOUNDS ? first_bad : (vd == DP_V_COLLIDE || vd == DP_V_CONSTRAINT ? first_col : -1);
       a.goal_out[q] = chosen;
-      a.res[q] = fr ? DP_R_UNDECIDED : DP_R_NONE;
+      a.res[q] = fr && !a.select ? DP_R_UNDECIDED : DP_R_NONE;
       a.flags[q] = (int8_t)((dupv ? 1 : 0) | (chosen != -1 ? 2 : 0));
       if (fr) {
 #pragma unroll
@@ -405,15 +470,106 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
       }
       c_b += vd == DP_V_BOUNDS;
       c_c += vd == DP_V_COLLIDE;
+      c_x += vd == DP_V_CONSTRAINT;
       c_f += fr;
       c_k += vd != DP_V_BOUNDS;
     }
   }
   if (k0 == 0) {
+    if (c_x) atomicAdd((unsigned long long*)&a.counters[DPC_CONSTRAINT], (unsigned long long)c_x);
     if (c_b) atomicAdd((unsigned long long*)&a.counters[DPC_BOUNDS], (unsigned long long)c_b);
     if (c_c) atomicAdd((unsigned long long*)&a.counters[DPC_COLLIDE], (unsigned long long)c_c);
     if (c_f) atomicAdd((unsigned long long*)&a.counters[DPC_FREE], (unsigned long long)c_f);
     if (c_k) atomicAdd((unsigned long long*)&a.counters[DPC_CHECKED], (unsigned long long)c_k);
+  }
+}
+
+// dbrrt's choice per node (dbrrt.cpp:627-660): among the node's FREE pairs,
+// in pair order, the first with the smallest distance(last placed state,
+// target) below DBL_MAX (the reference starts from
+// numeric_limits<double>::max() and keeps a candidate only when d < min), or
+// the first FREE one with choose_first.  One wave per node.  Writes the pair
+// index (-1: none) and [last state, first state, distance, g, h].
+template <int NX>
+__global__ __launch_bounds__(64) void k_select(dp_check_args a, const int32_t* __restrict__ app, int32_t N,
+                                               const double* __restrict__ targets, const double* __restrict__ cost,
+                                               double cost_jump, int32_t choose_first, int32_t* __restrict__ sel_q,
+                                               int32_t* __restrict__ sel_prim, double* __restrict__ sel) {
+  const dp_dev_sys& S = *a.sys;
+  const int32_t i = blockIdx.x;
+  if (i >= N) return;
+  const int lane = threadIdx.x;
+  const int32_t tot = *a.total;
+  const int32_t b = app[i] < tot ? app[i] : tot;
+  const int32_t e = app[i + 1] < tot ? app[i + 1] : tot;
+  double tg[NX];
+#pragma unroll
+  for (int c = 0; c < NX; ++c) tg[c] = targets[(int64_t)c * a.Ncap + i];
+  const double ox = a.nodes[i], oy = a.nodes[(int64_t)a.Ncap + i];
+  double best_d = DBL_MAX;
+  int32_t best_q = -1;
+  for (int32_t cb = b; cb < e; cb += 64) {
+    const int32_t q = cb + lane;
+    double d = DBL_MAX;
+    bool ok = false;
+    if (q < e && a.verdict[q] == DP_V_FREE) {
+      const int32_t j = a.pprim[q];
+      double x[NX];
+      load_translated<NX>(a, (int64_t)a.off[j + 1] - 1, ox, oy, x);
+      d = dp_distance(S, x, tg);
+      ok = d < DBL_MAX;  // NaN and +inf never enter
+    }
+    if (choose_first) {
+      const uint64_t m = __ballot(ok);
+      if (m) {
+        const int src = __ffsll((long long)m) - 1;
+        best_d = __shfl(d, src);
+        best_q = cb + src;
+        break;
+      }
+      continue;
+    }
+    // first minimum of the chunk: (d, q) lexicographic over the lanes that qualify
+    double md = ok ? d : DBL_MAX;
+    int32_t mq = ok ? q : INT_MAX;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+      const double od = __shfl_xor(md, w);
+      const int32_t oq = __shfl_xor(mq, w);
+      if (od < md || (od == md && oq < mq)) {
+        md = od;
+        mq = oq;
+      }
+    }
+    if (mq != INT_MAX && md < best_d) {  // strictly below every earlier chunk's minimum
+      best_d = md;
+      best_q = mq;
+    }
+  }
+  if (lane == 0) {
+    sel_q[i] = best_q;
+    sel_prim[i] = best_q >= 0 ? a.pprim[best_q] : -1;
+  }
+  double* o = sel + (int64_t)i * (2 * NX + 3);
+  if (best_q < 0) {
+    if (lane < 2 * NX + 3) o[lane] = NAN;
+    return;
+  }
+  if (lane == 0) {
+    const int32_t j = a.pprim[best_q];
+    double x[NX], x0[NX], nd[NX];
+    load_translated<NX>(a, (int64_t)a.off[j + 1] - 1, ox, oy, x);
+    load_translated<NX>(a, (int64_t)a.off[j], ox, oy, x0);
+#pragma unroll
+    for (int c = 0; c < NX; ++c) nd[c] = a.nodes[(int64_t)c * a.Ncap + i];
+#pragma unroll
+    for (int c = 0; c < NX; ++c) {
+      o[c] = x[c];
+      o[NX + c] = x0[c];
+    }
+    o[2 * NX] = best_d;
+    o[2 * NX + 1] = a.gpar[i] + cost[j] + cost_jump * dp_lower_bound_time(S, nd, x0);  // dbrrt.cpp:645-648
+    o[2 * NX + 2] = dp_lower_bound_time(S, x, a.goal);                                  // dbrrt.cpp:641-642
   }
 }
 
@@ -457,15 +613,23 @@ static app_bounds make_app_bounds(const dp_ctx* c, double r) {
   return B;
 }
 
+// Applicability radius: alpha * delta as the float product of dbastar.cpp:585-586,
+// or dbrrt's Expander radius (options_dbrrt.delta) for dp_expand_select.
+static double app_radius(const dp_ctx* c, const dp_expand_params* p) {
+  return c->select_mode ? c->select_radius : (double)(p->alpha * p->delta);
+}
+// Applicability keys: first states, or last states for the reverse search.
+static const double* app_keys(const dp_ctx* c) { return c->forward ? c->d_start : c->d_last; }
+
 int dp_launch_count(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
-  const double r_app = (double)(p->alpha * p->delta);  // dbastar.cpp:585-586 (float product)
+  const double r_app = app_radius(c, p);
   const int32_t N = c->N;
   const app_bounds B = make_app_bounds(c, r_app);
   return dispatch_nx(c->desc.nx, [&](auto NXc) -> int {
     constexpr int NX = decltype(NXc)::value;
     dp_fill_i32(c->d_app, N, 0, s);
-    k_app_count<NX><<<N * kAppParts, 64, 0, s>>>(c->d_sys, B, c->d_nodes, c->N_cap, N, c->d_start, c->P, r_app,
-                                                 c->d_app, c->d_app_mask);
+    k_app_count<NX><<<N * kAppParts, 64, 0, s>>>(c->d_sys, B, c->d_nodes, c->N_cap, N, app_keys(c), c->P, r_app,
+                                                 c->d_order, c->order_rows, c->d_app, c->d_app_mask);
     dp_scan_small(c->d_app, N, c->d_total, c->pair_cap, c->d_counters, DPC_PAIRS, s);
     DP_HIP_RET(hipGetLastError());
     return DP_OK;
@@ -481,11 +645,11 @@ int dp_launch_count(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
 int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   const int nx = c->desc.nx;
   const int32_t N = c->N;
-  const double r_app = (double)(p->alpha * p->delta);                   // dbastar.cpp:585-586 (float product)
+  const double r_app = app_radius(c, p);                                 // dbastar.cpp:585-586 (float product)
   const double r_dup = (1. - (double)p->alpha) * (double)p->delta;      // dbastar.cpp:852
   const double goal_bound = p->delta_factor_goal * (double)p->delta;    // dbastar.cpp:827
 
-  if (c->grid_dirty || c->grid_radius != r_dup) {
+  if (!c->select_mode && (c->grid_dirty || c->grid_radius != r_dup)) {
     int rc = dp_launch_visited_rebuild(c, r_dup, s);
     if (rc) return rc;
   }
@@ -496,10 +660,11 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   int rc = dispatch_nx(nx, [&](auto NXc) -> int {
     constexpr int NX = decltype(NXc)::value;
     dp_fill_i32(c->d_app, N, 0, s);
-    k_app_count<NX><<<N * kAppParts, 64, 0, s>>>(c->d_sys, B, c->d_nodes, c->N_cap, N, c->d_start, c->P, r_app,
-                                                 c->d_app, c->d_app_mask);
+    k_app_count<NX><<<N * kAppParts, 64, 0, s>>>(c->d_sys, B, c->d_nodes, c->N_cap, N, app_keys(c), c->P, r_app,
+                                                 c->d_order, c->order_rows, c->d_app, c->d_app_mask);
     dp_scan_small(c->d_app, N, c->d_total, c->pair_cap, c->d_counters, DPC_PAIRS, s);
-    k_app_fill<<<N, 64, 0, s>>>(N, c->P, c->d_app, c->d_app_mask, c->d_pnode, c->d_pprim, c->pair_cap);
+    k_app_fill<<<N, 64, 0, s>>>(N, c->P, c->d_app, c->d_app_mask, c->d_order, c->order_rows, c->d_pnode, c->d_pprim,
+                                c->pair_cap);
     return DP_OK;
   });
   if (rc) return rc;
@@ -561,6 +726,13 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   a.cdf = p->cost_delta_factor;
   a.surrogate = p->use_surrogate ? 1 : 0;
   a.counters = c->d_counters;
+  a.backward = c->forward ? 0 : 1;
+  a.last = c->d_last;
+  a.ncon = c->select_mode ? 0 : c->ncon;
+  a.con = c->d_con;
+  a.con_delta = (double)p->delta;
+  a.select = c->select_mode ? 1 : 0;
+  if (c->select_mode) a.nv = 0;
 
   if (c->desc.system == DP_SYS_QUADROTOR) {
     rc = dp_launch_quad(c, p, goal_bound, r_dup, s);
@@ -586,9 +758,9 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   const int64_t groups = ((int64_t)N * c->P + seg - 1) / seg;
   const int blocks = dp_grid_blocks(groups, kWaves, c->tiles.check_blocks);
   if (N > 0) {
-    k_node_obs<<<dp_grid_blocks(N, kWaves, 1024), 256, 0, s>>>(c->d_nodes, c->N_cap, N, c->d_obs, c->nb, c->reach_box[0],
-                                                              c->reach_box[1], c->reach_box[2], c->reach_box[3],
-                                                              c->d_node_obs, c->d_node_nobs);
+    const double* rb = c->forward ? c->reach_box : c->reach_box_bwd;
+    k_node_obs<<<dp_grid_blocks(N, kWaves, 1024), 256, 0, s>>>(c->d_nodes, c->N_cap, N, c->d_obs, c->nb, rb[0], rb[1],
+                                                              rb[2], rb[3], c->d_node_obs, c->d_node_nobs);
     DP_HIP_RET(hipGetLastError());
   }
   if ((rc = dp_record_event(c, 5, s))) return rc;
@@ -611,6 +783,24 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   DP_EV(c->ev[6], s);
   DP_EV(c->ev[2], s);
 
+  if (c->select_mode) {
+    rc = dispatch_nx(nx, [&](auto NXc) -> int {
+      constexpr int NX = decltype(NXc)::value;
+      if constexpr (NX == 13) {
+        return DP_ERR_ARG;
+      } else {
+        k_select<NX><<<N, 64, 0, s>>>(a, c->d_app, N, c->d_targets, c->d_cost, c->select_cost_jump,
+                                      c->select_first, c->d_sel_q, c->d_sel_q + c->sel_cap, c->d_sel);
+        return DP_OK;
+      }
+    });
+    if (rc) return rc;
+    DP_HIP_RET(hipGetLastError());
+    DP_EV(c->ev[7], s);
+    DP_EV(c->ev[3], s);
+    c->timing_valid = !c->no_events;
+    return DP_OK;
+  }
   rc = dp_launch_resolve(c, p, r_dup, s);
   if (rc) return rc;
   DP_EV(c->ev[3], s);

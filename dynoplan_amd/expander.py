@@ -50,7 +50,7 @@ class GpuExpander:
         self.close()
 
     # ---- inputs -------------------------------------------------------------
-    def upload_primitives(self, offsets, states, actions=None, max_motions=0):
+    def upload_primitives(self, offsets, states, actions=None, max_motions=0, cost=None):
         offsets = np.ascontiguousarray(offsets, np.int32)
         states = np.ascontiguousarray(states, np.float64)
         p = N.Prims()
@@ -64,7 +64,51 @@ class GpuExpander:
         if actions is not None:
             actions = np.ascontiguousarray(actions, np.float64)
             p.actions = _dptr(actions)
+        if cost is not None:
+            cost = np.ascontiguousarray(cost, np.float64)
+            p.cost = _dptr(cost)
         N.check(self.lib.dp_upload_primitives(self.ctx, C.byref(p), int(max_motions)))
+
+    # ---- settings of the other callers (tdbastar, dbrrt) ----------------------
+    def set_order(self, order):
+        """Pair order per node: [rows][P] permutations (node i uses row i % rows);
+        None restores ascending order."""
+        if order is None:
+            N.check(self.lib.dp_set_order(self.ctx, None, 0))
+            return
+        order = np.ascontiguousarray(np.atleast_2d(order), np.int32)
+        N.check(self.lib.dp_set_order(self.ctx, _iptr(order), order.shape[0]))
+
+    def set_direction(self, forward):
+        N.check(self.lib.dp_set_direction(self.ctx, 1 if forward else 0))
+
+    def set_constraints(self, times, states):
+        times = np.ascontiguousarray(times, np.float64).reshape(-1)
+        states = np.ascontiguousarray(states, np.float64).reshape(-1, self.nx)
+        n = times.size
+        N.check(self.lib.dp_set_constraints(self.ctx, _dptr(times) if n else None, _dptr(states) if n else None, n))
+
+    def expand_select(self, nodes, g_parent, targets, radius, goal, cost_jump=1.0, choose_first=False):
+        """dbrrt's expansion step for each node: the chosen primitive per node."""
+        self.upload_nodes(nodes, g_parent)
+        targets = np.ascontiguousarray(targets, np.float64).reshape(-1, self.nx)
+        sp = N.SelectParams()
+        sp.radius = float(radius)
+        sp.cost_jump = float(cost_jump)
+        sp.choose_first = 1 if choose_first else 0
+        for k, v in enumerate(goal):
+            sp.goal[k] = float(v)
+        N.check(self.lib.dp_expand_select(self.ctx, _dptr(targets), C.byref(sp), None))
+        n = targets.shape[0]
+        out = {"prim": np.zeros(n, np.int32), "pair": np.zeros(n, np.int64), "distance": np.zeros(n),
+               "state": np.zeros((n, self.nx)), "start": np.zeros((n, self.nx)), "g": np.zeros(n), "h": np.zeros(n)}
+        so = N.SelectOut()
+        so.prim = _iptr(out["prim"])
+        so.pair = out["pair"].ctypes.data_as(C.POINTER(C.c_int64))
+        for k in ("distance", "state", "start", "g", "h"):
+            setattr(so, k, _dptr(out[k]))
+        N.check(self.lib.dp_fetch_select(self.ctx, C.byref(so)))
+        return out
 
     def upload_world(self, lo, hi, boxes, spheres=None):
         boxes = np.ascontiguousarray(boxes, np.float64).reshape(-1, 6)

@@ -24,6 +24,34 @@ def load_primitives(path, system, max_motions=0, noise=0.0, seed=0, fmt=N.FMT_AU
     return offsets, states, actions, cost
 
 
+def reverse_primitives(offsets, states, actions, system, noise=0.0, seed=0):
+    """reverse_motions (dbrrt.cpp:252-339) through dp_reverse_primitives:
+    (offsets, states, actions, cost) of the reversed set."""
+    lib = N.load()
+    offsets = np.ascontiguousarray(offsets, np.int32)
+    states = np.ascontiguousarray(states, np.float64)
+    actions = np.ascontiguousarray(actions, np.float64)
+    p = N.Prims()
+    p.P = offsets.size - 1
+    p.nx = states.shape[1]
+    p.nu = actions.shape[1]
+    p.offsets = offsets.ctypes.data_as(C.POINTER(C.c_int32))
+    p.states = states.ctypes.data_as(C.POINTER(C.c_double))
+    p.actions = actions.ctypes.data_as(C.POINTER(C.c_double))
+    ps = N.PrimSet()
+    N.check(lib.dp_reverse_primitives(C.byref(p), system, float(noise), int(seed), C.byref(ps)))
+    try:
+        P, nx, nu = ps.P, ps.nx, ps.nu
+        o = np.ctypeslib.as_array(ps.offsets, (P + 1,)).copy()
+        S = int(o[-1])
+        st = np.ctypeslib.as_array(ps.states, (S, nx)).copy()
+        ac = np.ctypeslib.as_array(ps.actions, (max(S - P, 1), nu)).copy()[: S - P]
+        cost = np.ctypeslib.as_array(ps.cost, (P,)).copy()
+    finally:
+        lib.dp_free_primitives(C.byref(ps))
+    return o, st, ac, cost
+
+
 def save_primitives(path, offsets, states, actions, cost=None, fmt=N.FMT_AUTO):
     lib = N.load()
     offsets = np.ascontiguousarray(offsets, np.int32)

@@ -66,8 +66,9 @@ enum dp_system_id {
 
 #define DP_MAX_NX 16
 
-/* Per-pair verdict (pre-cap). */
-enum { DP_V_FREE = 0, DP_V_BOUNDS = 1, DP_V_COLLIDE = 2, DP_V_SURROGATE_REJECT = 3 };
+/* Per-pair verdict (pre-cap).  DP_V_CONSTRAINT: valid but for a tdbastar
+ * constraint (dp_set_constraints). */
+enum { DP_V_FREE = 0, DP_V_BOUNDS = 1, DP_V_COLLIDE = 2, DP_V_SURROGATE_REJECT = 3, DP_V_CONSTRAINT = 4 };
 /* Duplicate/insert resolution of a FREE pair. */
 enum { DP_R_NONE = 0, DP_R_NOVEL = 1, DP_R_DUP_VISITED = 2, DP_R_DUP_BATCH = 3, DP_R_CAPPED = 4 };
 
@@ -146,6 +147,7 @@ typedef struct dp_counts {
   int64_t conf_pool_used;   /* diagnostic: earlier-conflict entries past the inline slots */
   int64_t fp64_deferred;    /* quadrotor: FREE pairs past their node's FP64 window (rolled out in FP64
                                only if the node's cap did not fall before them) */
+  int64_t constraint;       /* pairs valid but for a tdbastar constraint (DP_V_CONSTRAINT) */
 } dp_counts;
 
 /* Caller-owned host buffers for dp_fetch_pairs (any pointer may be NULL). */
@@ -494,6 +496,94 @@ void dp_idbastar_options_default(dp_idbastar_options* opt);
  * trajectory is available through dp_search_trajectory afterwards. */
 int dp_idbastar(dp_ctx* ctx, const dp_prims* library, const double* start, const double* goal,
                 const dp_search_options* search, const dp_idbastar_options* opt, dp_idbastar_info* info);
+
+/* ---- the other callers of the expansion kernels -----------------------------
+ * tdbastar (src/tdbastar/tdbastar.cpp:313-447, 778-905) and dbrrt
+ * (src/dbrrt/dbrrt.cpp:580-660) run the same per-pair work as dbastar: the
+ * applicable primitives of a node, placed at it, checked for bounds and
+ * collisions.  They differ in the order of the applicable list, the search
+ * direction, tdbastar's constraints and dbrrt's choice of one primitive per
+ * node.  The first three are settings of a context that every later
+ * dp_expand follows; dbrrt's choice is dp_expand_select. */
+
+/* Pair order inside each node.  Expander::expand_lazy shuffles the
+ * applicable list (dbastar.hpp:302-303, tdbastar.hpp:276-277) and dbrrt
+ * reshuffles a node's stored list (dbrrt.cpp:611).  order is [rows][P] (P
+ * the uploaded primitive count), each row a permutation of 0..P-1: node i's
+ * applicable primitives come in the order they appear in row (i mod rows),
+ * and that order is the batch's pair order everywhere (duplicates, branching
+ * cap, dp_fetch_pairs).  rows = 0 (order NULL) restores ascending order;
+ * dp_upload_primitives also restores it.  DP_ERR_ARG if a row is not a
+ * permutation. */
+int dp_set_order(dp_ctx* ctx, const int32_t* order, int32_t rows);
+
+/* Search direction, tdbastar's reverse_search (tdbastar.cpp:575-576, 792).
+ * forward = 0: applicability is keyed on each primitive's last state
+ * (nigh_factory_t, nigh_custom_spaces.hpp:576-586), the primitive is placed
+ * with offset - last position so that it ends at the node
+ * (tdbastar.hpp:193-196), and the new node's state is the placed
+ * primitive's first state (check_goal, tdbastar.cpp:453-458).  Planar
+ * systems only (the reference assumes a 2-D offset); DP_ERR_ARG for the
+ * quadrotor.  forward = 1 restores dbastar's direction. */
+int dp_set_direction(dp_ctx* ctx, int32_t forward);
+
+/* tdbastar's constraints (Constraint, tdbastar.hpp:155-158): n states
+ * [n][nx] at times [n].  Every pair that passes the bounds and collision
+ * checks is tested against them (tdbastar.cpp:402-443): reaches_goal =
+ * distance(first placed state, or the last one backward, goal) <= delta;
+ * per constraint, in list order, the time index
+ * lround((float)(time - (float)g_parent) / ref_dt) selects state k when
+ * 0 <= k < L-1, or the last state when k >= L-1 and reaches_goal; the pair is
+ * violating when distance(state, constrained state) <= delta (delta and goal
+ * from dp_expand_params).  A violating pair gets DP_V_CONSTRAINT, first_k =
+ * the state index of the first violated constraint, and no duplicate test or
+ * cap count.  n = 0 clears.  Planar systems only (DP_ERR_ARG for the
+ * quadrotor). */
+int dp_set_constraints(dp_ctx* ctx, const double* times, const double* states, int32_t n);
+
+/* dbrrt's expansion step (dbrrt.cpp:580-660) for each uploaded node: its
+ * applicable primitives (nearestR(canonical node, radius) over first states;
+ * the Expander is built with options_dbrrt.delta, dbrrt.cpp:413-414), in
+ * the context's pair order, each checked by check_lazy_trajectory (bounds of
+ * every state, then collision; dbastar.cpp:391-490); of the valid ones the
+ * first with the smallest distance(last state, target) (strict <,
+ * dbrrt.cpp:627-642), or the first valid one with choose_first.  Backward
+ * expansion in dbrrt is this step over the reversed primitive set
+ * (dp_reverse_primitives) in a second context. */
+typedef struct dp_select_params {
+  double radius;            /* applicability radius, options_dbrrt.delta */
+  double cost_jump;         /* options_dbrrt.cost_jump */
+  int32_t choose_first;     /* options_dbrrt.choose_first_motion_valid */
+  int32_t reserved;
+  double goal[DP_MAX_NX];   /* for h = lower_bound_time(new state, goal) */
+} dp_select_params;
+
+/* Per node, caller-owned host buffers (any pointer may be NULL). */
+typedef struct dp_select_out {
+  int32_t* prim;            /* chosen primitive, -1 when no applicable primitive is valid */
+  int64_t* pair;            /* its index in the batch's pair order (dp_fetch_pairs), -1 when none */
+  double* distance;         /* distance(last placed state, target), NaN when none */
+  double* state;            /* [N][nx] last placed state (__expand_end), NaN when none */
+  double* start;            /* [N][nx] first placed state (__expand_start) */
+  double* g;                /* g_parent + motion cost + cost_jump * lower_bound_time(node, start) */
+  double* h;                /* lower_bound_time(state, goal) */
+} dp_select_out;
+
+/* targets: host [N][nx], one per uploaded node.  Enqueued on `stream` (NULL
+ * = the context stream); dp_fetch_select waits for it.  Verdicts of every
+ * pair are available through dp_fetch_pairs (resolution DP_R_NONE: no
+ * duplicate test runs) and dp_get_counts. */
+int dp_expand_select(dp_ctx* ctx, const double* targets, const dp_select_params* params, void* stream);
+int dp_fetch_select(dp_ctx* ctx, dp_select_out* out);
+
+/* reverse_motions (dbrrt.cpp:252-339) for the unicycles: each primitive
+ * translated by minus its last position, states and actions reversed, the
+ * first and last states perturbed by noise * U[-1, 1] drawn from a seeded
+ * stream (the reference draws 1e-7 * Eigen::Random; noise = 0 disables it),
+ * cost = ref_dt * #actions.  DP_ERR_ARG for other systems (the reference
+ * supports only translation-invariant ones).  Release with
+ * dp_free_primitives. */
+int dp_reverse_primitives(const dp_prims* in, int32_t system, double noise, uint64_t seed, dp_prim_set* out);
 
 /* ---- many planning instances in one launch ---------------------------------
  * No counterpart in the reference, which plans one instance per process.  A

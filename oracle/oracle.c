@@ -15,6 +15,7 @@
  */
 #include "oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -563,14 +564,78 @@ int64_t oracle_radius(const dp_system_desc* sys, const double* visited, int64_t 
   return w;
 }
 
+/* tdbastar's constraint test of one pair (tdbastar.cpp:417-443): the state
+ * index a constraint at `time` selects in a placed primitive of L1 states
+ * expanded from a node with gScore g (passed as float, tdbastar.cpp:317),
+ * or -1 when it selects none.
+ *   float time_offset = constraint.time - best_node_gScore;
+ *   int time_index = std::lround(time_offset / robot.ref_dt);
+ * The last state counts only when the pair reaches the goal. */
+int32_t oracle_constraint_index(double time, double g, double dt, int32_t L1, int32_t reaches_goal) {
+  const float time_offset = (float)(time - (double)(float)g);
+  const int32_t ti = (int32_t)lround((double)time_offset / dt);
+  int32_t k = -1;
+  if (reaches_goal && ti >= L1 - 1) k = L1 - 1;
+  if (ti >= 0 && ti < L1 - 1) k = ti;
+  return k;
+}
+
+/* The primitive placed at the node.  Forward: transform_primitive with the
+ * node's offset (dbastar.hpp:174-177).  Backward (tdbastar's reverse search,
+ * tdbastar.hpp:193-196): with offset - last position, so that the placed
+ * primitive ends at the node; planar systems (the reference assumes a 2-D
+ * offset there). */
+static int32_t place_primitive(const dp_system_desc* sys, const dp_prims* prims, int32_t j, const double* node,
+                               int backward, double* out) {
+  if (!backward) return oracle_transform_primitive(sys, prims, j, node, out);
+  const double* last = prims->states + (int64_t)(prims->offsets[j + 1] - 1) * sys->nx;
+  double o[DP_MAX_NX];
+  for (int c = 0; c < sys->nx; ++c) o[c] = is_offset_dim(sys, c) ? node[c] - last[c] : node[c];
+  return oracle_transform_primitive(sys, prims, j, o, out);
+}
+
+/* Applicability key of primitive j: its first state, or for the reverse
+ * search its last state (nigh_factory_t, nigh_custom_spaces.hpp:576-586). */
+static const double* app_key(const dp_system_desc* sys, const dp_prims* prims, int32_t j, int backward) {
+  return prims->states + (int64_t)(backward ? prims->offsets[j + 1] - 1 : prims->offsets[j]) * sys->nx;
+}
+
+/* First violated constraint of a valid pair (tdbastar.cpp:408-443), in list
+ * order: returns the state index it checked, -1 if none is violated. */
+static int32_t constraint_violation(const dp_system_desc* sys, const oracle_ext* ext, const double* traj, int32_t L1,
+                                    double g, const dp_expand_params* par) {
+  const int nx = sys->nx;
+  const double delta = (double)par->delta;
+  const double* rg_state = traj + (int64_t)(ext->backward ? L1 - 1 : 0) * nx;
+  const int32_t reaches_goal = oracle_distance(sys, rg_state, par->goal) <= delta;
+  for (int32_t c = 0; c < ext->n_constraints; ++c) {
+    const int32_t k = oracle_constraint_index(ext->con_times[c], g, sys->dt, L1, reaches_goal);
+    if (k >= 0 && oracle_distance(sys, traj + (int64_t)k * nx, ext->con_states + (int64_t)c * nx) <= delta)
+      return k;
+  }
+  return -1;
+}
+
 int64_t oracle_expand(const dp_system_desc* sys, const dp_prims* prims, int32_t max_motions,
                       const dp_world* world, const double* visited, int64_t nv,
                       const double* nodes, const double* g_parent, int32_t N,
                       const dp_expand_params* par, int32_t nn_mode, int32_t col_mode,
                       dp_pairs_out* out, dp_counts* counts) {
+  return oracle_expand_ext(sys, prims, max_motions, world, visited, nv, nodes, g_parent, N, par, nn_mode, col_mode,
+                           NULL, out, counts);
+}
+
+int64_t oracle_expand_ext(const dp_system_desc* sys, const dp_prims* prims, int32_t max_motions,
+                          const dp_world* world, const double* visited, int64_t nv,
+                          const double* nodes, const double* g_parent, int32_t N,
+                          const dp_expand_params* par, int32_t nn_mode, int32_t col_mode,
+                          const oracle_ext* ext, dp_pairs_out* out, dp_counts* counts) {
   if (!sys || !prims || !world || !nodes || !par || !counts) return DP_ERR_ARG;
   if (!(par->alpha > 0.0f && par->alpha < 1.0f)) return DP_ERR_ARG; /* dbastar.cpp:588-590 */
   if (sys->dist_w[0] <= 0.0) return DP_ERR_ARG;
+  const int backward = ext && ext->backward;
+  if (backward && sys->system == DP_SYS_QUADROTOR) return DP_ERR_ARG;
+  if (ext && ext->order && ext->order_rows <= 0) return DP_ERR_ARG;
   series_init();
   const int nx = sys->nx;
   int32_t P = prims->P;
@@ -606,13 +671,14 @@ int64_t oracle_expand(const dp_system_desc* sys, const dp_prims* prims, int32_t 
     canonical_state(sys, node, canon);
     int32_t novel_in_node = 0;
     int broke = 0; /* the reference's loop over this node's pairs has ended */
-    for (int32_t j = 0; j < P; ++j) {
-      const double* start = prims->states + (int64_t)prims->offsets[j] * nx;
-      if (!(oracle_distance(sys, canon, start) <= r_app)) continue; /* nearestR, <= */
+    const int32_t* row = ext && ext->order ? ext->order + (int64_t)(i % ext->order_rows) * P : NULL;
+    for (int32_t r = 0; r < P; ++r) {
+      const int32_t j = row ? row[r] : r; /* the shuffled applicable list, dbastar.hpp:302-303 */
+      if (!(oracle_distance(sys, canon, app_key(sys, prims, j, backward)) <= r_app)) continue; /* nearestR, <= */
       int64_t q = np++;
       int wr = out && q < out->capacity;
       if (out && q >= out->capacity) overflow = 1;
-      int32_t L1 = oracle_transform_primitive(sys, prims, j, node, traj);
+      int32_t L1 = place_primitive(sys, prims, j, node, backward, traj);
 
       int8_t verdict = DP_V_FREE;
       int32_t first_k = -1;
@@ -639,12 +705,22 @@ int64_t oracle_expand(const dp_system_desc* sys, const dp_prims* prims, int32_t 
             break;
           }
       }
+      if (verdict == DP_V_FREE && ext && ext->n_constraints > 0) {
+        const int32_t k = constraint_violation(sys, ext, traj, L1, g_parent[i], par);
+        if (k >= 0) {
+          first_k = k;
+          verdict = DP_V_CONSTRAINT;
+        }
+      }
       int32_t chosen = -1;
       int8_t res = DP_R_NONE;
       double gq = NAN, hq = NAN;
-      const double* x = traj + (int64_t)(L1 - 1) * nx;
+      /* the new node's state: the last placed state, or the first for the
+       * reverse search (check_goal, tdbastar.cpp:453-458) */
+      const double* x = traj + (int64_t)(backward ? 0 : L1 - 1) * nx;
       if (verdict == DP_V_BOUNDS) counts->bounds++;
       if (verdict == DP_V_COLLIDE) counts->collide++;
+      if (verdict == DP_V_CONSTRAINT) counts->constraint++;
       if (verdict == DP_V_FREE) {
         counts->free_++;
         if (par->check_intermediate_goal) { /* check_goal, dbastar.cpp:492-512 */
@@ -708,6 +784,91 @@ int64_t oracle_expand(const dp_system_desc* sys, const dp_prims* prims, int32_t 
   nn_free(&vis);
   nn_free(&acc);
   return overflow ? DP_ERR_CAPACITY : np;
+}
+
+/* ------------------------------------------------------------------ */
+/* dbrrt's expansion step                                                */
+/* ------------------------------------------------------------------ */
+
+/* For each node, dbrrt.cpp:580-660: the applicable primitives
+ * (nearestR(canonical node, radius) over first states; Expander built with
+ * options_dbrrt.delta, dbrrt.cpp:413-414) in the order of row i % order_rows
+ * (ascending when order is NULL), each checked by check_lazy_trajectory
+ * (bounds of every state, then collision; dbastar.cpp:391-490).  Among the
+ * valid ones the first with the smallest distance(last state, target_i)
+ * (strict <, dbrrt.cpp:627-642), or the first valid one with choose_first.
+ * Per node: the primitive (-1 when none is valid), its pair index in the
+ * batch's pair order, the distance, the last state (__expand_end), the first
+ * state (__expand_start), g = g_parent + cost + cost_jump *
+ * lower_bound_time(node, first) (:645-648; cost = prims->cost[j], or ref_dt
+ * per action without one) and h = lower_bound_time(last, goal) (:641-642).
+ * Returns the number of pairs. */
+int64_t oracle_select(const dp_system_desc* sys, const dp_prims* prims, int32_t max_motions, const dp_world* world,
+                      const double* nodes, const double* g_parent, int32_t N, double radius, const double* targets,
+                      int32_t choose_first, double cost_jump, const double* goal, const int32_t* order,
+                      int32_t order_rows, int32_t* out_prim, int64_t* out_pair, double* out_dist, double* out_state,
+                      double* out_start, double* out_g, double* out_h) {
+  if (!sys || !prims || !world || !nodes || !g_parent || !targets || !goal || N < 0) return DP_ERR_ARG;
+  if (order && order_rows <= 0) return DP_ERR_ARG;
+  series_init();
+  const int nx = sys->nx;
+  int32_t P = prims->P;
+  if (max_motions > 0 && max_motions < P) P = max_motions;
+  int32_t maxlen = 1;
+  for (int32_t j = 0; j < P; ++j) {
+    int32_t l = prims->offsets[j + 1] - prims->offsets[j];
+    if (l > maxlen) maxlen = l;
+  }
+  double* traj = (double*)malloc(sizeof(double) * (size_t)maxlen * nx);
+  int64_t np = 0;
+  for (int32_t i = 0; i < N; ++i) {
+    const double* node = nodes + (int64_t)i * nx;
+    const double* target = targets + (int64_t)i * nx;
+    double canon[DP_MAX_NX];
+    canonical_state(sys, node, canon);
+    const int32_t* row = order ? order + (int64_t)(i % order_rows) * P : NULL;
+    double min_d = DBL_MAX;
+    int32_t best = -1;
+    int64_t best_q = -1;
+    int done = 0;
+    for (int32_t r = 0; r < P; ++r) {
+      const int32_t j = row ? row[r] : r;
+      if (!(oracle_distance(sys, canon, app_key(sys, prims, j, 0)) <= radius)) continue;
+      const int64_t q = np++;
+      if (done) continue; /* choose_first: the loop has ended (dbrrt.cpp:640-641) */
+      const int32_t L1 = oracle_transform_primitive(sys, prims, j, node, traj);
+      int valid = 1;
+      for (int32_t k = 0; k < L1 && valid; ++k) valid = oracle_state_valid(sys, world, traj + (int64_t)k * nx);
+      for (int32_t k = 0; k < L1 && valid; ++k) valid = !oracle_state_collides(sys, world, traj + (int64_t)k * nx);
+      if (!valid) continue;
+      const double* last = traj + (int64_t)(L1 - 1) * nx;
+      const double d = oracle_distance(sys, last, target);
+      if (d < min_d) {
+        min_d = d;
+        best = j;
+        best_q = q;
+        const double cost = prims->cost ? prims->cost[j] : (L1 - 1) * sys->dt;
+        if (out_state) memcpy(out_state + (int64_t)i * nx, last, sizeof(double) * nx);
+        if (out_start) memcpy(out_start + (int64_t)i * nx, traj, sizeof(double) * nx);
+        if (out_g) out_g[i] = g_parent[i] + cost + cost_jump * oracle_lower_bound_time(sys, node, traj);
+        if (out_h) out_h[i] = oracle_lower_bound_time(sys, last, goal);
+        if (choose_first) done = 1;
+      }
+    }
+    if (out_prim) out_prim[i] = best;
+    if (out_pair) out_pair[i] = best_q;
+    if (out_dist) out_dist[i] = best >= 0 ? min_d : NAN;
+    if (best < 0) {
+      for (int c = 0; c < nx; ++c) {
+        if (out_state) out_state[(int64_t)i * nx + c] = NAN;
+        if (out_start) out_start[(int64_t)i * nx + c] = NAN;
+      }
+      if (out_g) out_g[i] = NAN;
+      if (out_h) out_h[i] = NAN;
+    }
+  }
+  free(traj);
+  return np;
 }
 
 /* ------------------------------------------------------------------ */

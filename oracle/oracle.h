@@ -52,6 +52,38 @@ int64_t oracle_expand_mt(const dp_system_desc* sys, const dp_prims* prims, int32
                          const dp_expand_params* par, int32_t nn_mode, int32_t col_mode,
                          dp_pairs_out* out, dp_counts* counts, int32_t nthreads);
 
+/* Settings of the other callers of the expansion (NULL = dbastar's):
+ * order: [order_rows][P], node i's pairs in the order of row i % order_rows
+ * (the shuffled applicable list, dbastar.hpp:302-303); backward: tdbastar's
+ * reverse search (applicability on last states, primitive placed to end at
+ * the node, new state = first placed state); constraints: tdbastar's
+ * (time, state) list checked on every valid pair (DP_V_CONSTRAINT). */
+typedef struct oracle_ext {
+  const int32_t* order;
+  int32_t order_rows;
+  int32_t backward;
+  int32_t n_constraints;
+  int32_t reserved;
+  const double* con_times;   /* [n_constraints] */
+  const double* con_states;  /* [n_constraints][nx] */
+} oracle_ext;
+
+int64_t oracle_expand_ext(const dp_system_desc* sys, const dp_prims* prims, int32_t max_motions,
+                          const dp_world* world, const double* visited, int64_t nv,
+                          const double* nodes, const double* g_parent, int32_t N,
+                          const dp_expand_params* par, int32_t nn_mode, int32_t col_mode,
+                          const oracle_ext* ext, dp_pairs_out* out, dp_counts* counts);
+
+/* tdbastar's constraint time index (tdbastar.cpp:417-427). */
+int32_t oracle_constraint_index(double time, double g, double dt, int32_t L1, int32_t reaches_goal);
+
+/* dbrrt's expansion step per node (dbrrt.cpp:580-660); see oracle.c. */
+int64_t oracle_select(const dp_system_desc* sys, const dp_prims* prims, int32_t max_motions, const dp_world* world,
+                      const double* nodes, const double* g_parent, int32_t N, double radius, const double* targets,
+                      int32_t choose_first, double cost_jump, const double* goal, const int32_t* order,
+                      int32_t order_rows, int32_t* out_prim, int64_t* out_pair, double* out_dist, double* out_state,
+                      double* out_start, double* out_g, double* out_h);
+
 /* Radius query over the visited set (brute force, ascending CSR). */
 int64_t oracle_radius(const dp_system_desc* sys, const double* visited, int64_t nv, const double* queries,
                       int64_t n, double radius, int64_t* out_offsets, int64_t* out_idx, int64_t idx_cap);

@@ -21,6 +21,12 @@ COL_BRUTE, COL_SWEPT = 0, 1
 _lib = None
 
 
+class OracleExt(C.Structure):
+    _fields_ = [("order", C.POINTER(C.c_int32)), ("order_rows", C.c_int32), ("backward", C.c_int32),
+                ("n_constraints", C.c_int32), ("reserved", C.c_int32), ("con_times", C.POINTER(C.c_double)),
+                ("con_states", C.POINTER(C.c_double))]
+
+
 def load():
     global _lib
     if _lib is not None:
@@ -33,6 +39,15 @@ def load():
     lib.oracle_expand.argtypes = [P(N.SystemDesc), P(N.Prims), C.c_int32, P(N.World), P(C.c_double), C.c_int64,
                                   P(C.c_double), P(C.c_double), C.c_int32, P(N.ExpandParams), C.c_int32,
                                   C.c_int32, P(N.PairsOut), P(N.Counts)]
+    lib.oracle_expand_ext.restype = C.c_int64
+    lib.oracle_expand_ext.argtypes = lib.oracle_expand.argtypes[:-2] + [P(OracleExt), P(N.PairsOut), P(N.Counts)]
+    lib.oracle_constraint_index.restype = C.c_int32
+    lib.oracle_constraint_index.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32]
+    lib.oracle_select.restype = C.c_int64
+    lib.oracle_select.argtypes = [P(N.SystemDesc), P(N.Prims), C.c_int32, P(N.World), P(C.c_double), P(C.c_double),
+                                  C.c_int32, C.c_double, P(C.c_double), C.c_int32, C.c_double, P(C.c_double),
+                                  P(C.c_int32), C.c_int32, P(C.c_int32), P(C.c_int64), P(C.c_double),
+                                  P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double)]
     lib.oracle_expand_mt.restype = C.c_int64
     lib.oracle_expand_mt.argtypes = lib.oracle_expand.argtypes + [C.c_int32]
     lib.oracle_radius.restype = C.c_int64
@@ -93,10 +108,12 @@ def world_struct(lo, hi, boxes, spheres):
 
 
 def expand(wl, nodes=None, g_parent=None, visited=None, params=None, nn_mode=NN_GRID, col_mode=COL_SWEPT,
-           max_motions=0, threads=0, want_out=True, cap=None):
+           max_motions=0, threads=0, want_out=True, cap=None, order=None, backward=False, constraints=None):
     """Run the oracle on a synth.Workload (optionally overriding parts).
     threads > 0 runs oracle_expand_mt on that many OpenMP threads; with
-    want_out False only the counts are produced (timing runs)."""
+    want_out False only the counts are produced (timing runs).  order
+    ([rows][P]), backward and constraints ((times, states)) select the
+    settings of the other callers (oracle_expand_ext, single-threaded)."""
     lib = load()
     nodes = np.ascontiguousarray(wl.nodes if nodes is None else nodes, np.float64)
     g_parent = np.ascontiguousarray(wl.g_parent if g_parent is None else g_parent, np.float64)
@@ -126,7 +143,25 @@ def expand(wl, nodes=None, g_parent=None, visited=None, params=None, nn_mode=NN_
     args = (C.byref(wl.desc), C.byref(pr), max_motions, C.byref(w), dptr(visited), visited.shape[0],
             dptr(nodes), dptr(g_parent), nodes.shape[0], C.byref(params), nn_mode, col_mode,
             C.byref(po) if want_out else None, C.byref(cnt))
-    n = lib.oracle_expand_mt(*args, threads) if threads > 0 else lib.oracle_expand(*args)
+    if order is not None or backward or constraints is not None:
+        ext = OracleExt()
+        keep = []
+        if order is not None:
+            order = np.ascontiguousarray(np.atleast_2d(order), np.int32)
+            keep.append(order)
+            ext.order = order.ctypes.data_as(C.POINTER(C.c_int32))
+            ext.order_rows = order.shape[0]
+        ext.backward = 1 if backward else 0
+        if constraints is not None:
+            ct = np.ascontiguousarray(constraints[0], np.float64).reshape(-1)
+            cs = np.ascontiguousarray(constraints[1], np.float64).reshape(-1, nx)
+            keep += [ct, cs]
+            ext.n_constraints = ct.size
+            ext.con_times = dptr(ct)
+            ext.con_states = dptr(cs)
+        n = lib.oracle_expand_ext(*args[:-2], C.byref(ext), *args[-2:])
+    else:
+        n = lib.oracle_expand_mt(*args, threads) if threads > 0 else lib.oracle_expand(*args)
     if n < 0:
         raise RuntimeError(f"oracle_expand failed: {n}")
     return {k: v[:n] for k, v in out.items()}, cnt.as_dict()
@@ -144,3 +179,43 @@ def radius(desc, visited, queries, r):
     lib.oracle_radius(C.byref(desc), dptr(visited), visited.shape[0], dptr(queries), queries.shape[0], float(r),
                       off.ctypes.data_as(C.POINTER(C.c_int64)), idx.ctypes.data_as(C.POINTER(C.c_int64)), total)
     return off, idx[:total]
+
+
+def select(wl, nodes, g_parent, targets, radius, goal, cost_jump=1.0, choose_first=False, order=None, cost=None,
+           offsets=None, states=None, actions=None):
+    """dbrrt's expansion step per node (oracle_select); the primitive arrays
+    default to the workload's."""
+    lib = load()
+    nx = wl.desc.nx
+    nodes = np.ascontiguousarray(nodes, np.float64).reshape(-1, nx)
+    g_parent = np.ascontiguousarray(g_parent, np.float64)
+    targets = np.ascontiguousarray(targets, np.float64).reshape(-1, nx)
+    offsets = np.ascontiguousarray(wl.offsets if offsets is None else offsets, np.int32)
+    states = np.ascontiguousarray(wl.states if states is None else states, np.float64)
+    actions = wl.actions if actions is None else actions
+    actions = np.ascontiguousarray(actions, np.float64) if actions is not None else None
+    boxes = np.ascontiguousarray(wl.boxes, np.float64)
+    spheres = np.ascontiguousarray(wl.spheres, np.float64)
+    pr = prims_struct(offsets, states, actions)
+    if cost is not None:
+        cost = np.ascontiguousarray(cost, np.float64)
+        pr.cost = dptr(cost)
+    w = world_struct(wl.lo, wl.hi, boxes, spheres)
+    goal = np.ascontiguousarray(np.resize(np.asarray(goal, np.float64), nx))
+    n = nodes.shape[0]
+    out = {"prim": np.zeros(n, np.int32), "pair": np.zeros(n, np.int64), "distance": np.zeros(n),
+           "state": np.zeros((n, nx)), "start": np.zeros((n, nx)), "g": np.zeros(n), "h": np.zeros(n)}
+    rows = 0
+    optr = None
+    if order is not None:
+        order = np.ascontiguousarray(np.atleast_2d(order), np.int32)
+        rows = order.shape[0]
+        optr = order.ctypes.data_as(C.POINTER(C.c_int32))
+    npairs = lib.oracle_select(C.byref(wl.desc), C.byref(pr), 0, C.byref(w), dptr(nodes), dptr(g_parent), n,
+                               float(radius), dptr(targets), 1 if choose_first else 0, float(cost_jump), dptr(goal),
+                               optr, rows, out["prim"].ctypes.data_as(C.POINTER(C.c_int32)),
+                               out["pair"].ctypes.data_as(C.POINTER(C.c_int64)), dptr(out["distance"]),
+                               dptr(out["state"]), dptr(out["start"]), dptr(out["g"]), dptr(out["h"]))
+    if npairs < 0:
+        raise RuntimeError(f"oracle_select failed: {npairs}")
+    return out, int(npairs)

@@ -38,6 +38,6 @@ def prefix(out, n_pairs):
 
 
 def counts_consistent(c):
-    assert c["pairs"] == c["bounds"] + c["collide"] + c["free_"] + c["surrogate_reject"]
+    assert c["pairs"] == c["bounds"] + c["collide"] + c["free_"] + c["surrogate_reject"] + c.get("constraint", 0)
     assert c["free_"] == c["novel"] + c["dup_visited"] + c["dup_batch"] + c["capped"]
     assert c["checked"] == c["pairs"] - c["bounds"] - c["surrogate_reject"]

@@ -43,8 +43,10 @@ STRUCTS = {
     "dp_world": (N.World, ["lo", "hi", "boxes", "spheres"]),
     "dp_expand_params": (N.ExpandParams, ["alpha", "delta_factor_goal", "limit_branching_factor", "fp32_rollout",
                                           "goal"]),
-    "dp_counts": (N.Counts, ["pairs", "checked", "resolve_rounds"]),
+    "dp_counts": (N.Counts, ["pairs", "checked", "resolve_rounds", "constraint"]),
     "dp_pairs_out": (N.PairsOut, ["node", "verdict", "final_state", "h"]),
+    "dp_select_params": (N.SelectParams, ["radius", "cost_jump", "choose_first", "goal"]),
+    "dp_select_out": (N.SelectOut, ["prim", "pair", "distance", "state", "start", "g", "h"]),
 }
 
 
