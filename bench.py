@@ -222,16 +222,27 @@ def bench_multi(args):
         params.append(p)
     progress(f"uploaded {len(exs)} contexts")
 
+    def health_sum():
+        hs = [e.health() for e in exs]
+        return {k: sum(h[k] for h in hs) for k in ("batches", "wave_giveups", "window_overflows")}
+
     def run(mode, steps, warmup):
         b = GpuBatch(exs, mode)
         for _ in range(warmup):
             b.expand_async(params)
             torch.cuda.synchronize()
+            for e in exs:
+                e.counts()  # finishes a batch whose FP64 window overflowed (later batches use a wider one)
+        h0 = health_sum()
         t = time.perf_counter()
         for _ in range(steps):
             b.expand_async(params)
         torch.cuda.synchronize()
         el = time.perf_counter() - t
+        h1 = health_sum()
+        integ = {k: h1[k] - h0[k] for k in h0}
+        integ["device_complete"] = integ["wave_giveups"] == 0 and integ["window_overflows"] == 0
+        run.integrity = integ
         lat = []
         for _ in range(steps):
             b.expand_async(params)
@@ -240,7 +251,10 @@ def bench_multi(args):
         b.close()
         return el, np.array(lat), st
 
-    el_g, lat_g, st_g = run(N.BATCH_GRAPH, args.steps, args.warmup)
+    warm = max(args.warmup, 3)
+    el_g, lat_g, st_g = run(N.BATCH_GRAPH, args.steps, warm)
+    integrity = run.integrity
+    progress(f"graph integrity {integrity}")
     counts = [e.counts() for e in exs]
     pairs = sum(c["pairs"] for c in counts)
     value = pairs * args.steps / el_g
@@ -270,7 +284,7 @@ def bench_multi(args):
         "unit": "pairs/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": warm,
         "ms_per_step": 1e3 * el_g / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
@@ -288,6 +302,7 @@ def bench_multi(args):
                                "instance_p99": float(np.percentile(lat_s, 99))},
         "sequential_value": pairs * args.steps / el_q,
         "graph_stats": st_g,
+        "integrity": integrity,
         "roofline": {"bound": "hbm", "kernel": "whole batch (all kernels)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": abytes},

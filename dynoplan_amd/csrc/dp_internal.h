@@ -279,6 +279,7 @@ struct dp_ctx {
   // counters
   long long* d_counters = nullptr;  // [DPC_COUNT]
   long long* d_sticky = nullptr;    // [DPS_COUNT] never reset
+  int32_t* d_wshift = nullptr;      // [2] quadrotor FP64 window growth (k_quad_window)
   int64_t n_batches = 0;            // dp_expand calls
   int64_t host_completions = 0;     // batches finished by host-launched work in dp_get_counts
   // test-only limits (dp_debug_set_limits): inline conflict slots used

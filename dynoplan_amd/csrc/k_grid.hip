@@ -539,6 +539,7 @@ struct round_args {
   unsigned long long* trace;  // optional [DP_TRACE_FIELDS * N], fields listed at DP_TRACE_FIELDS
   long long* counters;
   long long* sticky;  // [DPS_COUNT] never reset
+  int32_t* wshift;    // quadrotor FP64 window: [0] growth for later batches, [1] the growth this batch used
 };
 
 // Pause between two polls of a contender's verdict: grows with the number of
@@ -966,6 +967,8 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
       if (lane == 0) {
         atomicAdd((unsigned long long*)&a.counters[DPC_WINDOW_OVF], 1ull);
         if (a.sticky) atomicAdd((unsigned long long*)&a.sticky[DPS_WINDOW], 1ull);
+        // later batches of this context use a window twice as wide
+        if (a.wshift) atomicMax(&a.wshift[0], a.wshift[1] + 1);
       }
       overflow = false;
     }
@@ -1115,6 +1118,7 @@ static round_args make_round_args(dp_ctx* c) {
   a.trace = c->d_trace;
   a.counters = c->d_counters;
   a.sticky = c->d_sticky;
+  a.wshift = c->d_wshift;
   return a;
 }
 

@@ -787,15 +787,25 @@ __global__ void k_quad_chunk_nodes(int32_t N, const int32_t* __restrict__ chunk_
 // resolved again with every deferred pair rolled out).  The reference
 // never computes a pair after its node's cap (dbastar.cpp:916-918).  A
 // deferred pair's verdict (FREE) is exact from the FP32 pass.
+// The window is `window` FREE pairs, doubled wshift[0] times (at most 2^10)
+// unless `fixed`: the resolution raises wshift[0] when a node overflows, so a
+// context's later batches stay on the device; wshift[1] records the growth
+// this batch used.
 __global__ __launch_bounds__(64) void k_quad_window(int32_t N, const int32_t* __restrict__ app,
                                                     const int32_t* __restrict__ total, int8_t* __restrict__ verdict,
                                                     int32_t* __restrict__ firstk, int32_t* __restrict__ goal_out,
                                                     int8_t* __restrict__ res, int8_t* __restrict__ flags,
-                                                    int32_t window, int32_t* __restrict__ q64,
-                                                    long long* __restrict__ counters) {
+                                                    int32_t window, int32_t fixed, int32_t* __restrict__ wshift,
+                                                    int32_t* __restrict__ q64, long long* __restrict__ counters) {
   const int32_t node = blockIdx.x;
   if (node >= N) return;
   const int lane = threadIdx.x;
+  if (!fixed) {
+    const int32_t sh = wshift[0] < 10 ? wshift[0] : 10;
+    const int64_t w = (int64_t)window << sh;
+    window = w < INT32_MAX ? (int32_t)w : INT32_MAX;
+    if (node == 0 && lane == 0) wshift[1] = sh;
+  }
   const uint64_t lt = (1ull << lane) - 1ull;
   const int32_t tot = *total;
   const int32_t b = app[node] < tot ? app[node] : tot;
@@ -1044,9 +1054,10 @@ int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, doub
     k_quad32<<<dp_grid_blocks(max_chunks, kQWaves, c->tiles.quad32_blocks), 256, lds, s>>>(a);
     if ((rc = dp_record_event(c, 6, s))) return rc;
     const int32_t window = dp_fp64_window(c, p->limit_branching_factor);
+    const int32_t fixed = c->fp64_window > 0 || p->limit_branching_factor < 0 ? 1 : 0;
     if (c->N > 0)
       k_quad_window<<<c->N, 64, 0, s>>>(c->N, c->d_app, c->d_total, c->d_verdict, c->d_firstk, c->d_goal, c->d_res,
-                                        c->d_flags, window, c->d_q64, c->d_counters);
+                                        c->d_flags, window, fixed, c->d_wshift, c->d_q64, c->d_counters);
   } else {
     int rc = dp_record_event(c, 5, s);
     if (!rc) rc = dp_record_event(c, 6, s);
