@@ -487,14 +487,16 @@ int dp_idbastar(dp_ctx* c, const dp_prims* lib, const double* start, const doubl
     return (double)(z >> 11) * 0x1.0p-53 * 2.0 - 1.0;
   };
   const dp_dev_sys& S = c->hsys;
-  double delta = o->delta_0;
-  double max_motions = (double)o->num_primitives_0;
+  // Options_dbastar::delta is a float and max_motions a size_t: each
+  // update of idbastar.cpp:128-146 rounds (truncates) to them
+  float delta = (float)o->delta_0;
+  int64_t max_motions = o->num_primitives_0;
   bool solved_db = false;
   int rc;
   for (int32_t it = 0;; ++it) {
     if (it > 0) {  // idbastar.cpp:128-146 (new_schedule)
-      delta *= solved_db ? o->delta_rate : 0.9999;
-      max_motions *= o->num_primitives_rate;
+      delta = (float)((double)delta * (solved_db ? o->delta_rate : 0.9999));
+      max_motions = (int64_t)((double)max_motions * o->num_primitives_rate);
     }
     // primitive set of this iteration: inserted motions, then the library
     std::vector<int32_t> off{0};
@@ -518,11 +520,11 @@ int dp_idbastar(dp_ctx* c, const dp_prims* lib, const double* start, const doubl
     pr.offsets = off.data();
     pr.states = st.data();
     pr.actions = nu ? ac.data() : nullptr;
-    const int32_t use = (int32_t)std::min<double>(max_motions, (double)pr.P);
+    const int32_t use = (int32_t)std::min<int64_t>(max_motions, (int64_t)pr.P);
     if ((rc = dp_upload_primitives(c, &pr, use))) return rc;
 
     dp_search_options so = *sopt;
-    so.delta = (float)delta;
+    so.delta = delta;
     dp_search_info si;
     if ((rc = dp_search(c, start, goal, &so, &si))) return rc;
     solved_db = si.solved != 0;

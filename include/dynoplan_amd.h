@@ -460,10 +460,15 @@ int dp_search_trajectory_size(const dp_ctx* ctx, int64_t* n_states, int64_t* n_a
  * as the reference's new_schedule: iteration 0 uses delta_0 and
  * num_primitives_0; later iterations multiply delta by delta_rate after a
  * solved run (by 0.9999 otherwise) and the primitive count by
- * num_primitives_rate.  After a solved run the solution trajectory is cut
- * into `cuts` pieces, made canonical (offset dims of the first state
- * subtracted), its end states perturbed by 1e-7 U[-1,1] (seeded), and the
- * pieces are inserted before the library (idbastar.cpp:234-291), then the
+ * num_primitives_rate, rounding to float delta and truncating to an integer
+ * count each time (Options_dbastar's float delta and size_t max_motions).
+ * After a solved run the solution trajectory is cut into `cuts` pieces of
+ * floor(k * A / cuts) .. floor((k+1) * A / cuts) of its A actions, each made
+ * canonical (the offset dims of its first state subtracted from every
+ * state), its first and last states perturbed by 1e-7 U[-1,1] (one seeded
+ * splitmix64 stream, first state then last, pieces in order; quadrotor
+ * quaternions renormalised), and the pieces are inserted before every
+ * earlier primitive in cut order (idbastar.cpp:234-291); then the
  * primitives are uploaded again.  Trajectory optimisation between the
  * iterations (crocoddyl in the reference) is not part of this library: the
  * raw search solution stands in for the optimised one. */

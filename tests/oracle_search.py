@@ -203,3 +203,112 @@ def search(wl, start, goal, delta=0.3, alpha=0.5, max_expands=100000, limit_bran
         a0 = int(wl.offsets[j]) - j
         actions.extend(wl.actions[a0:a0 + na])
     return info, np.array(states), np.array(actions).reshape(-1, wl.actions.shape[1])
+
+
+MASK64 = (1 << 64) - 1
+
+
+def _splitmix_uniform(seed):
+    """U[-1, 1] draws from a splitmix64 stream (the library's seeded stand-in
+    for Eigen::VectorXd::Random)."""
+    state = [seed & MASK64]
+
+    def unif():
+        state[0] = (state[0] + 0x9E3779B97F4A7C15) & MASK64
+        z = state[0]
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        z ^= z >> 31
+        return float(z >> 11) * 2.0 ** -53 * 2.0 - 1.0
+
+    return unif
+
+
+def idbastar(wl, start, goal, delta_0=0.3, num_primitives_0=1000, delta_rate=0.9, num_primitives_rate=1.5, max_it=10,
+             max_num_sol=5, cuts=5, seed=1, add_primitives=True, max_expands=100000):
+    """iDb-A* (idbastar.cpp:128-312) over the oracle search: the
+    new_schedule (delta rounded to float, the primitive count truncated to an
+    integer at every update), one search per iteration over the first
+    max_motions primitives, and after a solved run the solution cut into
+    `cuts` pieces (actions floor(k A / cuts) .. floor((k+1) A / cuts)), made
+    canonical (offset dims of the first state subtracted), end states
+    perturbed by 1e-7 U[-1, 1] and inserted before every earlier primitive
+    (:234-291).  The raw search solution stands in for the optimised one (no
+    trajectory optimisation).  Returns (info, best states, best actions)."""
+    d = wl.desc
+    nx, nu = d.nx, wl.actions.shape[1]
+    offset_dims = [0, 1, 2, 7, 8, 9] if d.system == N.SYS_QUADROTOR else [0, 1]
+    unif = _splitmix_uniform(seed if seed else 1)
+    added = []  # (states, actions), newest first
+    lib_pieces = [(wl.states[wl.offsets[j]:wl.offsets[j + 1]],
+                   wl.actions[wl.offsets[j] - j:wl.offsets[j + 1] - j - 1]) for j in range(wl.offsets.size - 1)]
+    info = dict(solved=0, iterations=0, solutions=0, cost=1e8, last_delta=0.0, last_primitives=0,
+                primitives_added=0, expands=0, exit_criteria=None)
+    best = (np.zeros((0, nx)), np.zeros((0, nu)))
+    delta = f32(delta_0)
+    max_motions = int(num_primitives_0)
+    solved_db = False
+    it = 0
+    while True:
+        if it > 0:
+            delta = f32(float(delta) * (delta_rate if solved_db else 0.9999))
+            max_motions = int(float(max_motions) * num_primitives_rate)
+        pieces = added + lib_pieces
+        use = min(max_motions, len(pieces))
+        pieces = pieces[:use]
+        off = np.zeros(use + 1, np.int32)
+        off[1:] = np.cumsum([p[0].shape[0] for p in pieces])
+        w = synth_workload_like(wl, off, np.concatenate([p[0] for p in pieces]),
+                                np.concatenate([p[1] for p in pieces]).reshape(-1, nu))
+        si, st, ac = search(w, start, goal, delta=float(delta), max_expands=max_expands)
+        solved_db = si["solved"] == 1
+        info["iterations"] = it + 1
+        info["last_delta"] = float(delta)
+        info["last_primitives"] = use
+        info["expands"] += si["expands"]
+        if solved_db:
+            info["solutions"] += 1
+            info["solved"] = 1
+            if si["cost"] < info["cost"]:
+                info["cost"] = si["cost"]
+                best = (st, ac)
+            ns = st.shape[0]
+            if add_primitives and nu and ns >= 2:
+                na = ns - 1
+                k_cuts = min(cuts, na)
+                new = []
+                for k in range(k_cuts):
+                    a0, a1 = na * k // k_cuts, na * (k + 1) // k_cuts
+                    ps = st[a0:a1 + 1].copy()
+                    pa = ac[a0:a1].copy()
+                    off0 = ps[0].copy()
+                    for dd in offset_dims:
+                        ps[:, dd] = ps[:, dd] - off0[dd]
+                    for r in (0, a1 - a0):
+                        for dd in range(nx):
+                            ps[r, dd] = ps[r, dd] + 1e-7 * unif()
+                        if d.system == N.SYS_QUADROTOR:
+                            q = ps[r, 3:7]
+                            n = math.sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3])
+                            ps[r, 3:7] = q / n
+                    new.append((ps, pa))
+                added = new + added
+                info["primitives_added"] += k_cuts
+        finished = False
+        if it + 1 >= max_it:
+            finished = True
+            info["exit_criteria"] = N.IDB_MAX_IT
+        if info["solutions"] >= max_num_sol:
+            finished = True
+            info["exit_criteria"] = N.IDB_MAX_SOLUTION
+        if finished:
+            break
+        it += 1
+    return info, best[0], best[1]
+
+
+def synth_workload_like(wl, offsets, states, actions):
+    from dynoplan_amd import synth
+    w = synth.Workload(dict(wl))
+    w["offsets"], w["states"], w["actions"] = offsets, states, actions
+    return w

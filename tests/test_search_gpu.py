@@ -92,6 +92,26 @@ def test_search_batched_frontier(need_gpu):
         assert st2.shape[0] >= 1
 
 
+def test_idbastar_equals_oracle_over_iterations(need_gpu):
+    """Three iterations of iDb-A* (idbastar.cpp:128-312), each solved, with
+    the solution pieces inserted before the library: every iteration's
+    search runs over the primitives the oracle restatement builds, so the
+    summed expansions, the schedule's last delta and primitive count, the
+    best cost and the best trajectory all match bit for bit."""
+    wl = bugtrap()
+    kw = dict(delta_0=0.8, num_primitives_0=60, num_primitives_rate=1.5, delta_rate=0.9, max_it=4, max_num_sol=3)
+    ref, rs, ra = OS.idbastar(wl, START, GOAL, max_expands=10000, **kw)
+    with expander_for(wl) as ex:
+        info, st, ac = ex.idbastar(wl.offsets, wl.states, wl.actions, START, GOAL, search={"max_expands": 10000},
+                                   timelimit_s=600.0, **kw)
+    print(f"iDb-A*: gpu {info}\n oracle {ref}")
+    assert ref["iterations"] >= 2 and ref["primitives_added"] > 0
+    for k in ("solved", "iterations", "solutions", "cost", "last_delta", "last_primitives", "primitives_added",
+              "expands", "exit_criteria"):
+        assert info[k] == ref[k], k
+    assert np.array_equal(st, rs) and np.array_equal(ac, ra)
+
+
 def test_idbastar_schedule(need_gpu):
     """iDb-A* (idbastar.cpp:128-312): the first iteration is the plain search
     at delta_0 with the first num_primitives_0 primitives; later iterations
