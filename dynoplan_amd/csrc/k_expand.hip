@@ -111,6 +111,102 @@ __global__ __launch_bounds__(64) void k_app_count(const dp_dev_sys* __restrict__
   if (lane == 0 && cnt) atomicAdd(&app[i], cnt);
 }
 
+// The same query for kAppNodes nodes per wave (a shared pair order or
+// none): each lane loads its primitive's key once per 64-primitive chunk and
+// tests it against every node of the group, so the keys stream from L2 once
+// per group instead of once per node.  The exact early rejections run for
+// all (node, primitive) combinations; the combinations they leave go to a
+// wave-local queue and the full metric runs on them 64 at a time, packed,
+// instead of on every chunk that has at least one such lane.  Bit r of node
+// i's mask word w is set exactly when k_app_count would set it.
+constexpr int kAppNodes = 16;
+
+template <int NX>
+__device__ __forceinline__ bool app_reject_key(const dp_dev_sys& S, const app_bounds& B, const double* a,
+                                               const double* k, double r) {
+  if constexpr (NX == 13) {
+    double dot = a[3] * k[3];
+    dot = dot + a[4] * k[4];
+    dot = dot + a[5] * k[5];
+    dot = dot + a[6] * k[6];
+    if (fabs(dot) < B.c_q) return true;
+    double s = 0.0;
+#pragma unroll
+    for (int c = 10; c < 13; ++c) {
+      const double d = a[c] - k[c];
+      s = c == 10 ? d * d : s + d * d;
+    }
+    return s > B.T_w;
+  } else {
+    return S.w[1] * dp_so2(a[2], k[2]) > r;
+  }
+}
+
+template <int NX>
+__global__ __launch_bounds__(64) void k_app_blk(const dp_dev_sys* __restrict__ Sp, app_bounds B,
+                                                const double* __restrict__ nodes, int32_t Ncap, int32_t N,
+                                                const double* __restrict__ keys, int32_t P, double r_app,
+                                                const int32_t* __restrict__ row, int32_t parts,
+                                                int32_t* __restrict__ app, uint64_t* __restrict__ mask) {
+  __shared__ double s_a[kAppNodes * NX];
+  __shared__ uint16_t s_q[kAppNodes * 64];
+  __shared__ unsigned long long s_m[kAppNodes];
+  const dp_dev_sys& S = *Sp;
+  const int32_t grp = blockIdx.x / parts, part = blockIdx.x % parts;
+  const int32_t i0 = grp * kAppNodes;
+  if (i0 >= N) return;
+  const int32_t nb = N - i0 < kAppNodes ? N - i0 : kAppNodes;
+  const int lane = threadIdx.x;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const int32_t words = (P + 63) / 64;
+  const int32_t w0 = (int32_t)((int64_t)words * part / parts), w1 = (int32_t)((int64_t)words * (part + 1) / parts);
+  for (int t = lane; t < kAppNodes * NX; t += 64) {
+    const int b = t / NX, c = t % NX;
+    s_a[t] = b < nb && !dp_is_offset_dim(S, c) ? nodes[(int64_t)c * Ncap + i0 + b] : 0.0;
+  }
+  if (lane < kAppNodes) s_m[lane] = 0ull;
+  __syncthreads();
+  int32_t cnt = 0;  // lane b < nb: node i0 + b
+  for (int32_t w = w0; w < w1; ++w) {
+    const int32_t r = w * 64 + lane;
+    const int32_t j = r < P ? (row ? row[r] : r) : 0;
+    double k[NX];
+#pragma unroll
+    for (int c = 0; c < NX; ++c) k[c] = r < P ? keys[(int64_t)c * P + j] : 0.0;
+    // exact rejections for every node of the group; survivors queued
+    int qn = 0;
+    for (int b = 0; b < nb; ++b) {
+      const bool pass = r < P && !app_reject_key<NX>(S, B, s_a + b * NX, k, r_app);
+      const uint64_t m = __ballot(pass);
+      if (pass) s_q[qn + __popcll(m & lt)] = (uint16_t)((b << 6) | lane);
+      qn += __popcll(m);
+    }
+    __syncthreads();
+    // the full metric on the survivors, 64 at a time
+    for (int e0 = 0; e0 < qn; e0 += 64) {
+      const int e = e0 + lane;
+      if (e < qn) {
+        const int ent = s_q[e];
+        const int b = ent >> 6, lj = ent & 63;
+        const int32_t jj = row ? row[w * 64 + lj] : w * 64 + lj;
+        double kk[NX];
+#pragma unroll
+        for (int c = 0; c < NX; ++c) kk[c] = keys[(int64_t)c * P + jj];
+        if (dp_within(S, s_a + b * NX, kk, r_app)) atomicOr(&s_m[b], 1ull << lj);
+      }
+    }
+    __syncthreads();
+    if (lane < nb) {
+      const unsigned long long m = s_m[lane];
+      mask[(int64_t)(i0 + lane) * words + w] = m;
+      cnt += __popcll(m);
+      s_m[lane] = 0ull;
+    }
+    __syncthreads();
+  }
+  if (lane < nb && cnt) atomicAdd(&app[i0 + lane], cnt);
+}
+
 __global__ __launch_bounds__(64) void k_app_fill(int32_t N, int32_t P, const int32_t* __restrict__ app,
                                                  const uint64_t* __restrict__ mask, const int32_t* __restrict__ order,
                                                  int32_t rows, int32_t* __restrict__ pnode,
@@ -613,13 +709,34 @@ static app_bounds make_app_bounds(const dp_ctx* c, double r) {
   return B;
 }
 
+// Applicability masks and counts of the uploaded nodes (app cleared here).
+template <int NX>
+static void launch_app(dp_ctx* c, const app_bounds& B, double r_app, hipStream_t s);
+
 // Applicability radius: alpha * delta as the float product of dbastar.cpp:585-586,
 // or dbrrt's Expander radius (options_dbrrt.delta) for dp_expand_select.
 static double app_radius(const dp_ctx* c, const dp_expand_params* p) {
   return c->select_mode ? c->select_radius : (double)(p->alpha * p->delta);
 }
-// Applicability keys: first states, or last states for the reverse search.
-static const double* app_keys(const dp_ctx* c) { return c->forward ? c->d_start : c->d_last; }
+
+template <int NX>
+static void launch_app(dp_ctx* c, const app_bounds& B, double r_app, hipStream_t s) {
+  const int32_t N = c->N;
+  dp_fill_i32(c->d_app, N, 0, s);
+  const double* keys = c->forward ? c->d_start : c->d_last;
+  if (c->d_order && c->order_rows > 1) {  // a pair order per node: one wave per node part
+    k_app_count<NX><<<N * kAppParts, 64, 0, s>>>(c->d_sys, B, c->d_nodes, c->N_cap, N, keys, c->P, r_app, c->d_order,
+                                                 c->order_rows, c->d_app, c->d_app_mask);
+    return;
+  }
+  const int32_t groups = (N + kAppNodes - 1) / kAppNodes;
+  const int32_t words = (c->P + 63) / 64;
+  // about 4096 waves over the (node group, primitive part) grid
+  int32_t parts = (4096 + groups - 1) / groups;
+  parts = parts < 1 ? 1 : (parts > words ? words : parts);
+  k_app_blk<NX><<<groups * parts, 64, 0, s>>>(c->d_sys, B, c->d_nodes, c->N_cap, N, keys, c->P, r_app, c->d_order,
+                                              parts, c->d_app, c->d_app_mask);
+}
 
 int dp_launch_count(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   const double r_app = app_radius(c, p);
@@ -627,9 +744,7 @@ int dp_launch_count(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   const app_bounds B = make_app_bounds(c, r_app);
   return dispatch_nx(c->desc.nx, [&](auto NXc) -> int {
     constexpr int NX = decltype(NXc)::value;
-    dp_fill_i32(c->d_app, N, 0, s);
-    k_app_count<NX><<<N * kAppParts, 64, 0, s>>>(c->d_sys, B, c->d_nodes, c->N_cap, N, app_keys(c), c->P, r_app,
-                                                 c->d_order, c->order_rows, c->d_app, c->d_app_mask);
+    launch_app<NX>(c, B, r_app, s);
     dp_scan_small(c->d_app, N, c->d_total, c->pair_cap, c->d_counters, DPC_PAIRS, s);
     DP_HIP_RET(hipGetLastError());
     return DP_OK;
@@ -659,9 +774,7 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   const app_bounds B = make_app_bounds(c, r_app);
   int rc = dispatch_nx(nx, [&](auto NXc) -> int {
     constexpr int NX = decltype(NXc)::value;
-    dp_fill_i32(c->d_app, N, 0, s);
-    k_app_count<NX><<<N * kAppParts, 64, 0, s>>>(c->d_sys, B, c->d_nodes, c->N_cap, N, app_keys(c), c->P, r_app,
-                                                 c->d_order, c->order_rows, c->d_app, c->d_app_mask);
+    launch_app<NX>(c, B, r_app, s);
     dp_scan_small(c->d_app, N, c->d_total, c->pair_cap, c->d_counters, DPC_PAIRS, s);
     k_app_fill<<<N, 64, 0, s>>>(N, c->P, c->d_app, c->d_app_mask, c->d_order, c->order_rows, c->d_pnode, c->d_pprim,
                                 c->pair_cap);
