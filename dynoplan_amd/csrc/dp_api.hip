@@ -232,6 +232,8 @@ void dp_destroy(dp_ctx* c) {
   dev_free(&c->d_st);
   dev_free(&c->d_start);
   dev_free(&c->d_last);
+  dev_free(&c->d_start32);
+  dev_free(&c->d_last32);
   dev_free(&c->d_cost);
   dev_free(&c->d_order);
   dev_free(&c->d_con);
@@ -440,6 +442,13 @@ int dp_upload_primitives(dp_ctx* c, const dp_prims* pr, int32_t max_motions) {
   HIP_CHECK(hipMemcpy(c->d_st, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(c->d_start, start.data(), sizeof(double) * start.size(), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(c->d_last, last.data(), sizeof(double) * last.size(), hipMemcpyHostToDevice));
+  {
+    std::vector<float> s32(start.begin(), start.end()), l32(last.begin(), last.end());
+    if ((rc = dev_alloc(&c->d_start32, (int64_t)nx * P))) return rc;
+    if ((rc = dev_alloc(&c->d_last32, (int64_t)nx * P))) return rc;
+    HIP_CHECK(hipMemcpy(c->d_start32, s32.data(), sizeof(float) * s32.size(), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(c->d_last32, l32.data(), sizeof(float) * l32.size(), hipMemcpyHostToDevice));
+  }
   HIP_CHECK(hipMemcpy(c->d_cost, cost.data(), sizeof(double) * P, hipMemcpyHostToDevice));
   dev_free(&c->d_order);  // a pair order refers to the previous primitive count
   c->order_rows = 0;

@@ -170,6 +170,8 @@ struct dp_ctx {
   double* d_st = nullptr;        // [nx][S]
   double* d_start = nullptr;     // [nx][P]
   double* d_last = nullptr;      // [nx][P] last states (applicability key of the reverse search)
+  float* d_start32 = nullptr;    // [nx][P] FP32 copies of the keys (exact rejections with margins)
+  float* d_last32 = nullptr;
   double* d_cost = nullptr;      // [P] motion cost (given, or ref_dt per action)
   double* d_geo = nullptr;       // [4*nbox][S] : relx rely c s (per box)
   double* d_sweep = nullptr;     // [4][P] canonical swept footprint box (planar) / centre box (quad)

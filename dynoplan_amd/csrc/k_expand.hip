@@ -121,35 +121,81 @@ __global__ __launch_bounds__(64) void k_app_count(const dp_dev_sys* __restrict__
 // i's mask word w is set exactly when k_app_count would set it.
 constexpr int kAppNodes = 16;
 
+// Exact rejections of the blocked query, evaluated in FP32 on FP32 copies of
+// the keys and node states, with margins that cover FP32 rounding (about
+// 1e-6 relative on the quantities compared, against 1e-4 of margin): a pair
+// rejected here is rejected by the FP64 metric too.  Quadrotor: the attitude
+// term is w1 * acos(|q_a . q_b|), and for unit quaternions acos(|q_a . q_b|)
+// >= |q_a - sign(q_a . q_b) q_b| (the chord 2 sin(theta / 2) <= theta); every
+// other term is non-negative, so w1 * chord + w3 * |w_a - w_b| above the
+// radius proves d > r.  That leaves about 0.3% more pairs than are
+// applicable for the FP64 metric, against about 3x with the attitude bound
+// alone.  Planar systems: the heading term alone.
+struct app_bounds32 {
+  float c_q;      // quadrotor: |dot| below it rejects
+  float T_w;      // quadrotor: squared rate difference above it rejects
+  float w1, w3;   // metric weights of the attitude / heading and rate terms
+  float R;        // quadrotor: chord bound above it rejects; planar: heading term above it rejects
+};
+
+// Stage 1, for every (node, primitive) of a chunk: quadrotor |dot| >= c_q;
+// planar systems: the heading term.
 template <int NX>
-__device__ __forceinline__ bool app_reject_key(const dp_dev_sys& S, const app_bounds& B, const double* a,
-                                               const double* k, double r) {
+__device__ __forceinline__ bool app_pass1(const app_bounds32& B, const float* a, const float* k) {
   if constexpr (NX == 13) {
-    double dot = a[3] * k[3];
+    float dot = a[3] * k[3];
     dot = dot + a[4] * k[4];
     dot = dot + a[5] * k[5];
     dot = dot + a[6] * k[6];
-    if (fabs(dot) < B.c_q) return true;
-    double s = 0.0;
-#pragma unroll
-    for (int c = 10; c < 13; ++c) {
-      const double d = a[c] - k[c];
-      s = c == 10 ? d * d : s + d * d;
-    }
-    return s > B.T_w;
+    return !(fabsf(dot) < B.c_q);
   } else {
-    return S.w[1] * dp_so2(a[2], k[2]) > r;
+    float d = fabsf(a[2] - k[2]);
+    d = d > 3.14159265f ? 6.28318531f - d : d;
+    return !(B.w1 * d > B.R);
   }
 }
 
+// Stage 2, quadrotor only, on the stage-1 survivors (packed): the chord and
+// rate bound.
+__device__ __forceinline__ bool app_pass2_quad(const app_bounds32& B, const float* a, const float* k) {
+  float dot = a[3] * k[3];
+  dot = dot + a[4] * k[4];
+  dot = dot + a[5] * k[5];
+  dot = dot + a[6] * k[6];
+  const float sg = dot < 0.0f ? -1.0f : 1.0f;
+  // sums of squared differences with their FP32 error bounds: each term
+  // (a - k)^2 is off by at most 8e-7 (a^2 + k^2) (inputs rounded to FP32,
+  // one subtraction, one product, the sum); 2e-6 (a^2 + k^2) is taken off
+  float c2 = 0.0f, s = 0.0f, ec = 0.0f, es = 0.0f;
+#pragma unroll
+  for (int c = 3; c < 7; ++c) {
+    const float d = a[c] - sg * k[c];
+    c2 = c2 + d * d;
+    ec = ec + (a[c] * a[c] + k[c] * k[c]);
+  }
+#pragma unroll
+  for (int c = 10; c < 13; ++c) {
+    const float d = a[c] - k[c];
+    s = s + d * d;
+    es = es + (a[c] * a[c] + k[c] * k[c]);
+  }
+  const float c2_lo = fmaxf(c2 - 2e-6f * ec, 0.0f), s_lo = fmaxf(s - 2e-6f * es, 0.0f);
+  if (s_lo > B.T_w) return false;
+  return !(B.w1 * sqrtf(c2_lo) + B.w3 * sqrtf(s_lo) > B.R);
+}
+
 template <int NX>
-__global__ __launch_bounds__(64) void k_app_blk(const dp_dev_sys* __restrict__ Sp, app_bounds B,
+__global__ __launch_bounds__(64) void k_app_blk(const dp_dev_sys* __restrict__ Sp, app_bounds32 B,
                                                 const double* __restrict__ nodes, int32_t Ncap, int32_t N,
-                                                const double* __restrict__ keys, int32_t P, double r_app,
-                                                const int32_t* __restrict__ row, int32_t parts,
-                                                int32_t* __restrict__ app, uint64_t* __restrict__ mask) {
+                                                const double* __restrict__ keys, const float* __restrict__ keys32,
+                                                int32_t P, double r_app, const int32_t* __restrict__ row,
+                                                int32_t parts, int32_t* __restrict__ app,
+                                                uint64_t* __restrict__ mask) {
   __shared__ double s_a[kAppNodes * NX];
+  __shared__ float s_a32[kAppNodes * NX];
+  __shared__ float s_k[NX * 64];
   __shared__ uint16_t s_q[kAppNodes * 64];
+  __shared__ uint16_t s_q2[kAppNodes * 64];
   __shared__ unsigned long long s_m[kAppNodes];
   const dp_dev_sys& S = *Sp;
   const int32_t grp = blockIdx.x / parts, part = blockIdx.x % parts;
@@ -163,30 +209,58 @@ __global__ __launch_bounds__(64) void k_app_blk(const dp_dev_sys* __restrict__ S
   for (int t = lane; t < kAppNodes * NX; t += 64) {
     const int b = t / NX, c = t % NX;
     s_a[t] = b < nb && !dp_is_offset_dim(S, c) ? nodes[(int64_t)c * Ncap + i0 + b] : 0.0;
+    s_a32[t] = (float)s_a[t];
   }
   if (lane < kAppNodes) s_m[lane] = 0ull;
-  __syncthreads();
+  wave_lds_sync();
   int32_t cnt = 0;  // lane b < nb: node i0 + b
   for (int32_t w = w0; w < w1; ++w) {
     const int32_t r = w * 64 + lane;
     const int32_t j = r < P ? (row ? row[r] : r) : 0;
-    double k[NX];
+    float k[NX];
 #pragma unroll
-    for (int c = 0; c < NX; ++c) k[c] = r < P ? keys[(int64_t)c * P + j] : 0.0;
-    // exact rejections for every node of the group; survivors queued
+    for (int c = 0; c < NX; ++c) {
+      k[c] = r < P ? keys32[(int64_t)c * P + j] : 0.0f;
+      s_k[c * 64 + lane] = k[c];
+    }
+    // stage 1 for every node of the group; survivors queued
     int qn = 0;
     for (int b = 0; b < nb; ++b) {
-      const bool pass = r < P && !app_reject_key<NX>(S, B, s_a + b * NX, k, r_app);
+      const bool pass = r < P && app_pass1<NX>(B, s_a32 + b * NX, k);
       const uint64_t m = __ballot(pass);
       if (pass) s_q[qn + __popcll(m & lt)] = (uint16_t)((b << 6) | lane);
       qn += __popcll(m);
     }
-    __syncthreads();
+    wave_lds_sync();
+    const uint16_t* q = s_q;
+    if constexpr (NX == 13) {
+      // stage 2 on the packed survivors
+      int qn2 = 0;
+      for (int e0 = 0; e0 < qn; e0 += 64) {
+        const int e = e0 + lane;
+        bool pass = false;
+        int ent = 0;
+        if (e < qn) {
+          ent = s_q[e];
+          const int b = ent >> 6, lj = ent & 63;
+          float kk[NX];
+#pragma unroll
+          for (int c = 0; c < NX; ++c) kk[c] = s_k[c * 64 + lj];
+          pass = app_pass2_quad(B, s_a32 + b * NX, kk);
+        }
+        const uint64_t m = __ballot(pass);
+        if (pass) s_q2[qn2 + __popcll(m & lt)] = (uint16_t)ent;
+        qn2 += __popcll(m);
+      }
+      wave_lds_sync();
+      q = s_q2;
+      qn = qn2;
+    }
     // the full metric on the survivors, 64 at a time
     for (int e0 = 0; e0 < qn; e0 += 64) {
       const int e = e0 + lane;
       if (e < qn) {
-        const int ent = s_q[e];
+        const int ent = q[e];
         const int b = ent >> 6, lj = ent & 63;
         const int32_t jj = row ? row[w * 64 + lj] : w * 64 + lj;
         double kk[NX];
@@ -195,38 +269,62 @@ __global__ __launch_bounds__(64) void k_app_blk(const dp_dev_sys* __restrict__ S
         if (dp_within(S, s_a + b * NX, kk, r_app)) atomicOr(&s_m[b], 1ull << lj);
       }
     }
-    __syncthreads();
+    wave_lds_sync();
     if (lane < nb) {
       const unsigned long long m = s_m[lane];
       mask[(int64_t)(i0 + lane) * words + w] = m;
       cnt += __popcll(m);
       s_m[lane] = 0ull;
     }
-    __syncthreads();
+    wave_lds_sync();
   }
   if (lane < nb && cnt) atomicAdd(&app[i0 + lane], cnt);
 }
 
+
+// The masks as the (node, primitive) pair list, one wave per node, 64 mask
+// words at a time: each lane expands its own word's set bits into LDS at the
+// word's offset (an exclusive scan of the counts across the lanes), then the
+// wave copies the batch out with full-width stores.
+template <bool kRow>
 __global__ __launch_bounds__(64) void k_app_fill(int32_t N, int32_t P, const int32_t* __restrict__ app,
                                                  const uint64_t* __restrict__ mask, const int32_t* __restrict__ order,
                                                  int32_t rows, int32_t* __restrict__ pnode,
                                                  int32_t* __restrict__ pprim, int64_t pair_cap) {
+  __shared__ int32_t s_p[64 * 64];
   const int32_t i = blockIdx.x;
   if (i >= N) return;
   const int lane = threadIdx.x;
   const int32_t words = (P + 63) / 64;
-  const int32_t* row = order ? order + (int64_t)(i % rows) * P : nullptr;
+  const int32_t* row = kRow ? order + (int64_t)(i % rows) * P : nullptr;
   int64_t q0 = app[i];
-  for (int32_t w = 0; w < words; ++w) {
-    const uint64_t m = mask[(int64_t)i * words + w];
-    if ((m >> lane) & 1ull) {
-      const int64_t q = q0 + __popcll(m & ((1ull << lane) - 1ull));
+  for (int32_t wb = 0; wb < words; wb += 64) {
+    const int32_t wl = wb + lane;
+    uint64_t m = wl < words ? mask[(int64_t)i * words + wl] : 0ull;
+    int32_t incl = __popcll(m);
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t v = __shfl_up(incl, o);
+      if (lane >= o) incl += v;
+    }
+    int32_t pos = incl - __popcll(m);
+    while (m) {
+      s_p[pos++] = wl * 64 + (__ffsll((long long)m) - 1);
+      m &= m - 1ull;
+    }
+    wave_lds_sync();
+    const int32_t total = __builtin_amdgcn_readlane(incl, 63);
+    for (int32_t k = lane; k < total; k += 64) {
+      const int64_t q = q0 + k;
       if (q < pair_cap) {
+        const int32_t r = s_p[k];
         pnode[q] = i;
-        pprim[q] = row ? row[w * 64 + lane] : w * 64 + lane;
+        if constexpr (kRow) pprim[q] = row[r];
+        else pprim[q] = r;
       }
     }
-    q0 += __popcll(m);
+    q0 += total;
+    wave_lds_sync();
   }
 }
 
@@ -719,23 +817,46 @@ static double app_radius(const dp_ctx* c, const dp_expand_params* p) {
   return c->select_mode ? c->select_radius : (double)(p->alpha * p->delta);
 }
 
+// Margins of the FP32 rejections (app_bounds32): relative 1e-4 on the
+// compared quantities, absolute 1e-6 on top.
+static app_bounds32 make_app_bounds32(const dp_ctx* c, double r) {
+  const dp_system_desc& d = c->desc;
+  app_bounds32 B{-1.0f, INFINITY, (float)d.dist_w[1], (float)d.dist_w[3], INFINITY};
+  if (d.system == DP_SYS_QUADROTOR) {
+    if (d.dist_w[1] > 0.0 && r / d.dist_w[1] < 1.5) B.c_q = (float)(std::cos(r / d.dist_w[1]) - 1e-5);
+    if (d.dist_w[3] > 0.0) B.T_w = (float)((r / d.dist_w[3]) * (r / d.dist_w[3]) * (1.0 + 1e-4) + 1e-6);
+    B.R = (float)(r * (1.0 + 1e-4) + 1e-6);
+  } else {
+    B.R = (float)(r * (1.0 + 1e-4) + d.dist_w[1] * 1e-6 + 1e-6);
+  }
+  return B;
+}
+
+// Primitive parts per node of the blocked applicability: about 4096 waves
+// over the (node group, part) grid, at most one part per mask word.
+static int32_t app_parts(const dp_ctx* c) {
+  const int32_t groups = (c->N + kAppNodes - 1) / kAppNodes;
+  const int32_t words = (c->P + 63) / 64;
+  int32_t parts = (4096 + groups - 1) / groups;
+  return parts < 1 ? 1 : (parts > words ? words : parts);
+}
+static bool app_per_node_order(const dp_ctx* c) { return c->d_order && c->order_rows > 1; }
+
 template <int NX>
 static void launch_app(dp_ctx* c, const app_bounds& B, double r_app, hipStream_t s) {
   const int32_t N = c->N;
   dp_fill_i32(c->d_app, N, 0, s);
   const double* keys = c->forward ? c->d_start : c->d_last;
-  if (c->d_order && c->order_rows > 1) {  // a pair order per node: one wave per node part
+  if (app_per_node_order(c)) {  // a pair order per node: one wave per node part
     k_app_count<NX><<<N * kAppParts, 64, 0, s>>>(c->d_sys, B, c->d_nodes, c->N_cap, N, keys, c->P, r_app, c->d_order,
                                                  c->order_rows, c->d_app, c->d_app_mask);
     return;
   }
   const int32_t groups = (N + kAppNodes - 1) / kAppNodes;
-  const int32_t words = (c->P + 63) / 64;
-  // about 4096 waves over the (node group, primitive part) grid
-  int32_t parts = (4096 + groups - 1) / groups;
-  parts = parts < 1 ? 1 : (parts > words ? words : parts);
-  k_app_blk<NX><<<groups * parts, 64, 0, s>>>(c->d_sys, B, c->d_nodes, c->N_cap, N, keys, c->P, r_app, c->d_order,
-                                              parts, c->d_app, c->d_app_mask);
+  const int32_t parts = app_parts(c);
+  const float* keys32 = c->forward ? c->d_start32 : c->d_last32;
+  k_app_blk<NX><<<groups * parts, 64, 0, s>>>(c->d_sys, make_app_bounds32(c, r_app), c->d_nodes, c->N_cap, N, keys,
+                                              keys32, c->P, r_app, c->d_order, parts, c->d_app, c->d_app_mask);
 }
 
 int dp_launch_count(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
@@ -776,8 +897,12 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
     constexpr int NX = decltype(NXc)::value;
     launch_app<NX>(c, B, r_app, s);
     dp_scan_small(c->d_app, N, c->d_total, c->pair_cap, c->d_counters, DPC_PAIRS, s);
-    k_app_fill<<<N, 64, 0, s>>>(N, c->P, c->d_app, c->d_app_mask, c->d_order, c->order_rows, c->d_pnode, c->d_pprim,
-                                c->pair_cap);
+    if (c->d_order)
+      k_app_fill<true><<<N, 64, 0, s>>>(N, c->P, c->d_app, c->d_app_mask, c->d_order, c->order_rows, c->d_pnode,
+                                        c->d_pprim, c->pair_cap);
+    else
+      k_app_fill<false><<<N, 64, 0, s>>>(N, c->P, c->d_app, c->d_app_mask, nullptr, 1, c->d_pnode, c->d_pprim,
+                                         c->pair_cap);
     return DP_OK;
   });
   if (rc) return rc;
