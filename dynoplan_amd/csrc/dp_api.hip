@@ -292,6 +292,7 @@ void dp_destroy(dp_ctx* c) {
   dev_free(&c->d_rbad64);
   dev_free(&c->d_rbad32);
   dev_free(&c->d_rnear32);
+  dev_free(&c->d_qrec);
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
   c->h_pinned = nullptr;
   dev_free(&c->d_counters);
@@ -518,6 +519,7 @@ int dp_upload_primitives(dp_ctx* c, const dp_prims* pr, int32_t max_motions) {
     if ((rc = dev_alloc(&c->d_rbad64, P))) return rc;
     if ((rc = dev_alloc(&c->d_rbad32, P))) return rc;
     if ((rc = dev_alloc(&c->d_rnear32, P))) return rc;
+    if ((rc = dev_alloc(&c->d_qrec, (int64_t)16 * P))) return rc;
     if ((rc = dp_launch_quad_rot(c, c->stream))) return rc;
     HIP_CHECK(hipStreamSynchronize(c->stream));
   }
@@ -657,7 +659,7 @@ int dp_reserve(dp_ctx* c, int32_t max_nodes, int64_t max_pairs) {
     if ((rc = dev_alloc(&c->d_conf_pool, c->conf_pool_cap))) return rc;
     if (c->desc.system == DP_SYS_QUADROTOR && (rc = dev_alloc(&c->d_dbg32, 3 * max_pairs))) return rc;
     if (c->desc.system == DP_SYS_QUADROTOR && (rc = dev_alloc(&c->d_q64, max_pairs))) return rc;
-    if (c->desc.system == DP_SYS_QUADROTOR && (rc = dev_alloc(&c->d_qperm, max_pairs))) return rc;
+    if (c->desc.system == DP_SYS_QUADROTOR && (rc = dev_alloc(&c->d_qperm, 2 * max_pairs))) return rc;
     c->pair_cap = max_pairs;
   }
   return DP_OK;

@@ -241,7 +241,7 @@ struct dp_ctx {
   int32_t rounds = 32;              // resolution rounds launched per batch
   int32_t* d_chunk = nullptr;       // [N_cap+1] quadrotor: 64-pair chunks per node, scanned
   int32_t* d_nshort = nullptr;      // [N_cap] quadrotor: pairs whose primitive leaves the rate bounds
-  int32_t* d_qperm = nullptr;       // [pair_cap] quadrotor: pairs per node, those first
+  int32_t* d_qperm = nullptr;       // [2][pair_cap] quadrotor: (pair, primitive) per node, rate-bound ones first
   int32_t* d_chunk_node = nullptr;  // quadrotor: node of each FP32 wave chunk
   int64_t chunk_node_cap = 0;
   float* d_dbg32 = nullptr;         // [3][pair_cap] quadrotor: FP32 final positions (error report)
@@ -253,6 +253,7 @@ struct dp_ctx {
   double* d_dv64 = nullptr;         // [S][4] (x y z, pad)
   float* d_dv32 = nullptr;          // [S][4]
   int32_t *d_rbad64 = nullptr, *d_rbad32 = nullptr, *d_rnear32 = nullptr;  // [P]
+  double* d_qrec = nullptr;         // [P][16] quadrotor: the FP32 pass's 128-byte record per primitive
   double prim_reach = 0.0;          // max over primitives of max_k |p_k - p_0|_inf
   float* d_preach = nullptr;        // [P] quadrotor: max_k |p_k - p_0|_inf per primitive (rounded up)
   double reach_box[4] = {0, 0, 0, 0};  // planar: union of the canonical swept boxes (lo x, hi x, lo y, hi y)

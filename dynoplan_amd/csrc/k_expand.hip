@@ -438,7 +438,7 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
   __syncthreads();
 
   constexpr int SEG = 64 / LPP;
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
   const int seg = lane / LPP, k0 = lane % LPP;
   const int shift = seg * LPP;
   const uint64_t segbits = LPP == 64 ? ~0ull : ((1ull << LPP) - 1ull);

@@ -29,6 +29,21 @@
 #define QF_NEAR 4
 #define QF_NEED64 8
 
+// What the FP32 pass reads of primitive j, in one 128-byte record (one
+// cache line) instead of a dozen scattered arrays: start velocity (FP64: the
+// pair's start velocity is (float)(v0 + node velocity)), start position as
+// FP32, the rate-bound / near states, the state range, the reach, and the
+// offset box of the rollout up to the rate-bound state (k_quad_rot).
+struct alignas(16) quad_rec {
+  double v0[3];
+  float p0[3];
+  int32_t rbad, rnear, s0, L1;
+  float reach;
+  float qbox[12];
+  int32_t pad[6];
+};
+static_assert(sizeof(quad_rec) == 128, "one cache line per primitive");
+
 struct dp_quad_args {
   const dp_dev_sys* sys;
   const double* nodes;
@@ -56,7 +71,7 @@ struct dp_quad_args {
   const int32_t* app;     // pair offsets per node [N+1]
   const int32_t* chunk_off;  // [N+1] wave chunks per node, scanned
   const int32_t* chunk_node;  // [chunks] node of each wave chunk
-  const int32_t* qperm;   // [pairs] each node's pairs, those the primitive's rate bounds reject first
+  const int2* qperm;      // [pairs] (pair, primitive) of each node's pairs, those the primitive's rate bounds reject first
   const int32_t* nshort;  // [N] how many of them
   const int32_t* pnode;
   const int32_t* pprim;
@@ -84,6 +99,7 @@ struct dp_quad_args {
   const int32_t* rbad64;  // [P] first state whose attitude / rate leaves the bounds (INT_MAX: none)
   const int32_t* rbad32;  // [P] the same in FP32
   const int32_t* rnear32; // [P] first FP32 state within the re-check margin of an attitude / rate bound
+  const quad_rec* qrec;   // [P] the FP32 pass's view of each primitive
   long long* counters;
 };
 
@@ -184,7 +200,8 @@ __global__ void k_quad_rot(const dp_dev_sys* __restrict__ Sp, int32_t P, const i
                            const double* __restrict__ st, int32_t Sn, const double* __restrict__ act,
                            const float* __restrict__ act32, double* __restrict__ rot64, double* __restrict__ dv64,
                            float* __restrict__ dv32, int32_t* __restrict__ rbad64, int32_t* __restrict__ rbad32,
-                           int32_t* __restrict__ rnear32) {
+                           int32_t* __restrict__ rnear32, const float* __restrict__ preach,
+                           quad_rec* __restrict__ qrec) {
   const dp_dev_sys& S = *Sp;
   const quad_f32 K = make_quad_f32(S);
   const float eps_q = 100.0f * S.near_eps, eps_w = 100.0f * S.near_eps;
@@ -248,6 +265,48 @@ __global__ void k_quad_rot(const dp_dev_sys* __restrict__ Sp, int32_t P, const i
     rbad64[j] = b64;
     rbad32[j] = b32;
     rnear32[j] = n32;
+    quad_rec rec{};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      rec.v0[c] = st[(int64_t)(7 + c) * Sn + s0];
+      rec.p0[c] = (float)st[(int64_t)c * Sn + s0];
+    }
+    rec.rbad = b32;
+    rec.rnear = n32;
+    rec.s0 = s0;
+    rec.L1 = L1;
+    rec.reach = preach[j];
+    // Offsets of the FP32 rollout over states 0..b32, without the pair's
+    // start velocity v0: v_k = v0 + C_k (C_k the sum of the first k FP32
+    // increments) and p_k = p0 + k dt v0 + D_k (D_k = dt times the sum of
+    // C_0..C_{k-1}).  Summed in FP64 from the FP32 increments, widened by
+    // 1e-3 (m, m/s): the FP32 loop's own rounding stays below 1e-4 over 50
+    // steps at these magnitudes.
+    if (b32 < L1) {
+      double C[3] = {0.0, 0.0, 0.0}, D[3] = {0.0, 0.0, 0.0};
+      double cmin[3] = {0.0, 0.0, 0.0}, cmax[3] = {0.0, 0.0, 0.0}, dmin[3] = {0.0, 0.0, 0.0},
+             dmax[3] = {0.0, 0.0, 0.0};
+      const double dt = S.dt;
+      for (int k = 1; k <= b32; ++k) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          D[c] += dt * C[c];
+          C[c] += (double)dv32[(int64_t)(s0 + k) * 4 + c];
+          cmin[c] = fmin(cmin[c], C[c]);
+          cmax[c] = fmax(cmax[c], C[c]);
+          dmin[c] = fmin(dmin[c], D[c]);
+          dmax[c] = fmax(dmax[c], D[c]);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        rec.qbox[c] = (float)(cmin[c] - 1e-3);
+        rec.qbox[3 + c] = (float)(cmax[c] + 1e-3);
+        rec.qbox[6 + c] = (float)(dmin[c] - 1e-3);
+        rec.qbox[9 + c] = (float)(dmax[c] + 1e-3);
+      }
+    }
+    qrec[j] = rec;
   }
 }
 
@@ -264,7 +323,9 @@ __device__ __forceinline__ void quad32_start(const dp_quad_args& a, int32_t j, c
 __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
   extern __shared__ float qsm[];
   const dp_dev_sys& S = *a.sys;
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  // the wave index is uniform: in a scalar register, every per-chunk value
+  // derived from it (chunk, node, its offsets and state) is a scalar load
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
   const int per_wave = 6 * a.nb + 4 * a.ns;
   float* cb = qsm + wave * per_wave;  // compacted boxes (6 floats) then spheres (4 floats)
   const quad_f32 K = make_quad_f32(S);
@@ -292,7 +353,8 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
     const bool short_chunk = cw < cshort;
     const int32_t idx = short_chunk ? pbeg + cw * 64 + lane : pbeg + ns + (cw - cshort) * 64 + lane;
     const bool vp = short_chunk ? idx < pbeg + ns : idx < pend;
-    const int32_t q = vp ? a.qperm[idx] : 0;
+    const int2 qj = vp ? a.qperm[idx] : make_int2(0, 0);
+    const int32_t q = qj.x;
     double np[3], nvel[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -309,9 +371,8 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
     if (!short_chunk) {
       float rl = 0.0f;
       if (vp) {
-        const int32_t jj = a.pprim[q];
-        const int32_t len = a.off[jj + 1] - a.off[jj];
-        rl = a.preach[jj] + (float)((len - 1) * a.sys->dt * vmax);
+        const quad_rec& rj = a.qrec[qj.y];
+        rl = rj.reach + (float)((rj.L1 - 1) * a.sys->dt * vmax);
       }
 #pragma unroll
       for (int d = 32; d > 0; d >>= 1) rl = fmaxf(rl, __shfl_xor(rl, d, 64));
@@ -376,18 +437,18 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
     if (vp) {
-      const int32_t j = a.pprim[q];
-      const int32_t s0 = a.off[j];
-      const int32_t L1 = a.off[j + 1] - s0;
+      const quad_rec& rj = a.qrec[qj.y];
+      const int32_t s0 = rj.s0;
+      const int32_t L1 = rj.L1;
       // position (relative to the node) and velocity; the attitude / rate
       // part of every step is the primitive's (k_quad_rot)
       float s[6];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        s[c] = (float)a.start[(int64_t)c * a.P + j];
-        s[3 + c] = (float)(a.start[(int64_t)(7 + c) * a.P + j] + nvel[c]);
+        s[c] = rj.p0[c];
+        s[3 + c] = (float)(rj.v0[c] + nvel[c]);
       }
-      const int32_t rbad = a.rbad32[j], rnear = a.rnear32[j];
+      const int32_t rbad = rj.rbad, rnear = rj.rnear;
       float plo[3], phi[3], vlo[3], vhi[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -398,18 +459,51 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
       }
       int first_bad = INT_MAX, first_col = INT_MAX;
       bool near_bound_any = false, near_col_before = false, near = false;
-      float4 dnext = make_float4(0.f, 0.f, 0.f, 0.f);
-      for (int k = 0; k < L1; ++k) {
+      // A rate-bound pair whose position and velocity stay, over states
+      // 0..rbad, more than the margins inside their bounds (from the
+      // primitive's offset box and the pair's start) ends BOUNDS at rbad,
+      // near exactly when an attitude / rate state before it is: the
+      // rollout below would find the same.
+      bool fast = false;
+      if (short_chunk) {
+        const float* qb = rj.qbox;
+        const float R = (float)rbad * K.dt;
+        fast = true;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float v0 = s[3 + c];
+          const float vl = v0 + qb[c], vh = v0 + qb[3 + c];
+          const float rv = R * v0;
+          const float pl = s[c] + qb[6 + c] + fminf(rv, 0.0f), ph = s[c] + qb[9 + c] + fmaxf(rv, 0.0f);
+          fast = fast && vl > vlo[c] + 2.0f * eps_v && vh < vhi[c] - 2.0f * eps_v;
+          fast = fast && pl > plo[c] + 2.0f * eps && ph < phi[c] - 2.0f * eps;
+        }
+        if (fast) {
+          first_bad = rbad;
+          near_bound_any = rnear <= rbad;
+        }
+      }
+      // the velocity increments of the next four steps are in flight while
+      // a state is checked: each step waits for a load issued four steps
+      // earlier instead of one (the loop is bound by load latency)
+      const float4* dvp = reinterpret_cast<const float4*>(a.dv32) + s0;
+      const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+      const int32_t Lr = fast ? 0 : L1;
+      float4 d1 = 1 < Lr ? dvp[1] : z4, d2 = 2 < Lr ? dvp[2] : z4, d3 = 3 < Lr ? dvp[3] : z4,
+             d4 = 4 < Lr ? dvp[4] : z4;
+      for (int k = 0; k < (fast ? 0 : L1); ++k) {
         if (k > 0) {
-          const float dv[3] = {dnext.x, dnext.y, dnext.z};
+          const float dv[3] = {d1.x, d1.y, d1.z};
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
             s[c] = s[c] + K.dt * s[3 + c];
             s[3 + c] = s[3 + c] + dv[c];
           }
+          d1 = d2;
+          d2 = d3;
+          d3 = d4;
+          d4 = k + 4 < L1 ? dvp[k + 4] : z4;
         }
-        // the next step's increment is loaded while this state is checked
-        if (k + 1 < L1) dnext = reinterpret_cast<const float4*>(a.dv32)[s0 + k + 1];
         // bounds with margin (attitude and rate: the primitive's first
         // offending / near state)
         bool bad = k >= rbad, nb_ = k >= rnear;
@@ -726,15 +820,21 @@ __global__ __launch_bounds__(256) void k_quad_dupv(dp_quad_args a) {
   }
 }
 
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Per node: its pairs reordered for the FP32 pass, those whose primitive
 // leaves the attitude / rate bounds within its length first (their verdict
 // is BOUNDS whatever the translation, so their rollouts are short and need
-// no collision test), then the rest, each group in canonical order; and the
+// no collision test) by that step, then the rest in canonical order; and the
 // node's wave chunks (64 pairs, one group per chunk).  One wave per node.
 __global__ __launch_bounds__(64) void k_quad_split(int32_t N, const int32_t* __restrict__ app,
                                                    const int32_t* __restrict__ total, const int32_t* __restrict__ pprim,
                                                    const int32_t* __restrict__ off, const int32_t* __restrict__ rbad32,
-                                                   int32_t* __restrict__ qperm, int32_t* __restrict__ nshort,
+                                                   int2* __restrict__ qperm, int32_t* __restrict__ nshort,
                                                    int32_t* __restrict__ chunks) {
   const int32_t node = blockIdx.x;
   if (node >= N) return;
@@ -753,17 +853,44 @@ __global__ __launch_bounds__(64) void k_quad_split(int32_t N, const int32_t* __r
     }
     ns += __popcll(__ballot(sh));
   }
-  int32_t ws = b, wl = b + ns;
+  // short pairs ordered by the step at which their primitive leaves the
+  // rate bounds (a counting sort over 64 bins), so that the lanes of a
+  // chunk roll out about as many steps each; results do not depend on it
+  __shared__ int32_t s_bin[64];
+  s_bin[lane] = 0;
+  wave_sync_lds();
+  for (int32_t cb = b; cb < e; cb += 64) {
+    const int32_t p = cb + lane;
+    if (p < e) {
+      const int32_t j = pprim[p];
+      const int32_t rb = rbad32[j];
+      if (rb < off[j + 1] - off[j]) atomicAdd(&s_bin[rb < 63 ? rb : 63], 1);
+    }
+  }
+  wave_sync_lds();
+  {
+    int32_t incl = s_bin[lane];
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t v = __shfl_up(incl, o);
+      if (lane >= o) incl += v;
+    }
+    s_bin[lane] = b + incl - s_bin[lane];  // first slot of each bin
+  }
+  wave_sync_lds();
+  int32_t wl = b + ns;
   for (int32_t cb = b; cb < e; cb += 64) {
     const int32_t p = cb + lane;
     bool sh = false;
+    int32_t rb = 0, j = 0;
     if (p < e) {
-      const int32_t j = pprim[p];
-      sh = rbad32[j] < off[j + 1] - off[j];
+      j = pprim[p];
+      rb = rbad32[j];
+      sh = rb < off[j + 1] - off[j];
     }
-    const uint64_t ms = __ballot(sh), ml = __ballot(p < e && !sh);
-    if (p < e) qperm[sh ? ws + __popcll(ms & lt) : wl + __popcll(ml & lt)] = p;
-    ws += __popcll(ms);
+    const uint64_t ml = __ballot(p < e && !sh);
+    if (sh) qperm[atomicAdd(&s_bin[rb < 63 ? rb : 63], 1)] = make_int2(p, j);
+    else if (p < e) qperm[wl + __popcll(ml & lt)] = make_int2(p, j);
     wl += __popcll(ml);
   }
   if (lane == 0) {
@@ -943,7 +1070,7 @@ dp_quad_args make_quad_args(dp_ctx* c, const dp_expand_params* p, double goal_bo
   a.grid = c->grid;
   a.app = c->d_app;
   a.chunk_off = c->d_chunk;
-  a.qperm = c->d_qperm;
+  a.qperm = reinterpret_cast<const int2*>(c->d_qperm);
   a.chunk_node = c->d_chunk_node;
   a.nshort = c->d_nshort;
   a.pnode = c->d_pnode;
@@ -972,6 +1099,7 @@ dp_quad_args make_quad_args(dp_ctx* c, const dp_expand_params* p, double goal_bo
   a.rbad64 = c->d_rbad64;
   a.rbad32 = c->d_rbad32;
   a.rnear32 = c->d_rnear32;
+  a.qrec = reinterpret_cast<const quad_rec*>(c->d_qrec);
   a.counters = c->d_counters;
   return a;
 }
@@ -982,7 +1110,8 @@ int dp_launch_quad_rot(dp_ctx* c, hipStream_t s) {
   if (c->P <= 0) return DP_OK;
   k_quad_rot<<<dp_grid_blocks(c->P, 64, 4096), 64, 0, s>>>(c->d_sys, c->P, c->d_off, c->d_st, c->S, c->d_act,
                                                            c->d_act32, c->d_rot64, c->d_dv64, c->d_dv32, c->d_rbad64,
-                                                           c->d_rbad32, c->d_rnear32);
+                                                           c->d_rbad32, c->d_rnear32, c->d_preach,
+                                                           reinterpret_cast<quad_rec*>(c->d_qrec));
   DP_HIP_RET(hipGetLastError());
   return DP_OK;
 }
@@ -1044,7 +1173,8 @@ int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, doub
       dp_set_error("too many obstacles for the FP32 quadrotor staging (max 64 KiB LDS per workgroup)");
       return DP_ERR_ARG;
     }
-    k_quad_split<<<c->N, 64, 0, s>>>(c->N, c->d_app, c->d_total, c->d_pprim, c->d_off, c->d_rbad32, c->d_qperm,
+    k_quad_split<<<c->N, 64, 0, s>>>(c->N, c->d_app, c->d_total, c->d_pprim, c->d_off, c->d_rbad32,
+                                     reinterpret_cast<int2*>(c->d_qperm),
                                      c->d_nshort, c->d_chunk);
     dp_scan_small(c->d_chunk, c->N, nullptr, INT64_MAX, c->d_counters, -1, s);
     k_quad_chunk_nodes<<<dp_grid_blocks(c->N, 256, 1024), 256, 0, s>>>(c->N, c->d_chunk, c->d_chunk_node);
