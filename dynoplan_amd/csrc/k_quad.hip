@@ -276,18 +276,19 @@ __global__ void k_quad_rot(const dp_dev_sys* __restrict__ Sp, int32_t P, const i
     rec.s0 = s0;
     rec.L1 = L1;
     rec.reach = preach[j];
-    // Offsets of the FP32 rollout over states 0..b32, without the pair's
-    // start velocity v0: v_k = v0 + C_k (C_k the sum of the first k FP32
-    // increments) and p_k = p0 + k dt v0 + D_k (D_k = dt times the sum of
-    // C_0..C_{k-1}).  Summed in FP64 from the FP32 increments, widened by
+    // Offsets of the FP32 rollout over states 0..min(b32, L1 - 1), without
+    // the pair's start velocity v0: v_k = v0 + C_k (C_k the sum of the first
+    // k FP32 increments) and p_k = p0 + k dt v0 + D_k (D_k = dt times the sum
+    // of C_0..C_{k-1}).  Summed in FP64 from the FP32 increments, widened by
     // 1e-3 (m, m/s): the FP32 loop's own rounding stays below 1e-4 over 50
     // steps at these magnitudes.
-    if (b32 < L1) {
+    {
+      const int32_t kend = b32 < L1 ? b32 : L1 - 1;
       double C[3] = {0.0, 0.0, 0.0}, D[3] = {0.0, 0.0, 0.0};
       double cmin[3] = {0.0, 0.0, 0.0}, cmax[3] = {0.0, 0.0, 0.0}, dmin[3] = {0.0, 0.0, 0.0},
              dmax[3] = {0.0, 0.0, 0.0};
       const double dt = S.dt;
-      for (int k = 1; k <= b32; ++k) {
+      for (int k = 1; k <= kend; ++k) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           D[c] += dt * C[c];
@@ -491,6 +492,41 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
       const int32_t Lr = fast ? 0 : L1;
       float4 d1 = 1 < Lr ? dvp[1] : z4, d2 = 2 < Lr ? dvp[2] : z4, d3 = 3 < Lr ? dvp[3] : z4,
              d4 = 4 < Lr ? dvp[4] : z4;
+      // Long pairs: the staged obstacles that can come within r + 2 near_eps
+      // of this pair's own states (its position box: the primitive's offset
+      // box plus k dt v0 for k up to L1 - 1, widened by 1e-3), one bit each;
+      // an obstacle left out is farther than that from every state, so it
+      // neither collides nor comes near.  Without room for the bits, all.
+      uint64_t bmask = ~0ull, smask = ~0ull;
+      const bool masked = !short_chunk && nbc + nsc <= 64;
+      if (masked) {
+        const float* qb = rj.qbox;
+        const float R = (float)(L1 - 1) * K.dt;
+        const float pad = r + 2.0f * eps + 1e-3f;
+        float lo[3], hi[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float rv = R * s[3 + c];
+          lo[c] = s[c] + qb[6 + c] + fminf(rv, 0.0f) - pad;
+          hi[c] = s[c] + qb[9 + c] + fmaxf(rv, 0.0f) + pad;
+        }
+        bmask = 0ull;
+        for (int t = 0; t < nbc; ++t) {
+          const float* b = cb + 6 * t;
+          bool hit = true;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) hit = hit && b[c] - b[3 + c] <= hi[c] && b[c] + b[3 + c] >= lo[c];
+          if (hit) bmask |= 1ull << t;
+        }
+        smask = 0ull;
+        for (int t = 0; t < nsc; ++t) {
+          const float* b = cs + 4 * t;
+          bool hit = true;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) hit = hit && b[c] - b[3] <= hi[c] && b[c] + b[3] >= lo[c];
+          if (hit) smask |= 1ull << t;
+        }
+      }
       for (int k = 0; k < (fast ? 0 : L1); ++k) {
         if (k > 0) {
           const float dv[3] = {d1.x, d1.y, d1.z};
@@ -522,7 +558,7 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
         }
         if (first_col == INT_MAX && !short_chunk) {
           bool col = false, nc = false;
-          for (int t = 0; t < nbc; ++t) {
+          auto box_test = [&](int t) {
             const float* b = cb + 6 * t;
             float dx = fabsf(s[0] - b[0]) - b[3];
             float dy = fabsf(s[1] - b[1]) - b[4];
@@ -533,14 +569,21 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
             const float d2 = (dx * dx + dy * dy) + dz * dz;
             col = col || d2 <= r2;
             nc = nc || (d2 >= r_lo2 && d2 <= r_hi2);
-          }
-          for (int t = 0; t < nsc; ++t) {
+          };
+          auto sphere_test = [&](int t) {
             const float* b = cs + 4 * t;
             const float dx = s[0] - b[0], dy = s[1] - b[1], dz = s[2] - b[2];
             const float d2 = (dx * dx + dy * dy) + dz * dz;
             const float rr = r + b[3];
             col = col || d2 <= rr * rr;
             nc = nc || (d2 >= (rr - eps) * (rr - eps) && d2 <= (rr + eps) * (rr + eps));
+          };
+          if (masked) {
+            for (uint64_t m = bmask; m; m &= m - 1ull) box_test(__ffsll((long long)m) - 1);
+            for (uint64_t m = smask; m; m &= m - 1ull) sphere_test(__ffsll((long long)m) - 1);
+          } else {
+            for (int t = 0; t < nbc; ++t) box_test(t);
+            for (int t = 0; t < nsc; ++t) sphere_test(t);
           }
           if (nc) near_col_before = true;
           if (col) first_col = k;
