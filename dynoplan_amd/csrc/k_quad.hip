@@ -33,14 +33,15 @@
 // cache line) instead of a dozen scattered arrays: start velocity (FP64: the
 // pair's start velocity is (float)(v0 + node velocity)), start position as
 // FP32, the rate-bound / near states, the state range, the reach, and the
-// offset box of the rollout up to the rate-bound state (k_quad_rot).
+// offset box of the rollout up to the rate-bound state and over all states
+// (k_quad_rot).
 struct alignas(16) quad_rec {
   double v0[3];
   float p0[3];
   int32_t rbad, rnear, s0, L1;
   float reach;
   float qbox[12];
-  int32_t pad[6];
+  float dbox[6];  // position offset D min / max over every state 0..L1 - 1
 };
 static_assert(sizeof(quad_rec) == 128, "one cache line per primitive");
 
@@ -286,18 +287,27 @@ __global__ void k_quad_rot(const dp_dev_sys* __restrict__ Sp, int32_t P, const i
       const int32_t kend = b32 < L1 ? b32 : L1 - 1;
       double C[3] = {0.0, 0.0, 0.0}, D[3] = {0.0, 0.0, 0.0};
       double cmin[3] = {0.0, 0.0, 0.0}, cmax[3] = {0.0, 0.0, 0.0}, dmin[3] = {0.0, 0.0, 0.0},
-             dmax[3] = {0.0, 0.0, 0.0};
+             dmax[3] = {0.0, 0.0, 0.0}, fmn[3] = {0.0, 0.0, 0.0}, fmx[3] = {0.0, 0.0, 0.0};
       const double dt = S.dt;
-      for (int k = 1; k <= kend; ++k) {
+      for (int k = 1; k < L1; ++k) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           D[c] += dt * C[c];
           C[c] += (double)dv32[(int64_t)(s0 + k) * 4 + c];
-          cmin[c] = fmin(cmin[c], C[c]);
-          cmax[c] = fmax(cmax[c], C[c]);
-          dmin[c] = fmin(dmin[c], D[c]);
-          dmax[c] = fmax(dmax[c], D[c]);
+          fmn[c] = fmin(fmn[c], D[c]);
+          fmx[c] = fmax(fmx[c], D[c]);
+          if (k <= kend) {
+            cmin[c] = fmin(cmin[c], C[c]);
+            cmax[c] = fmax(cmax[c], C[c]);
+            dmin[c] = fmin(dmin[c], D[c]);
+            dmax[c] = fmax(dmax[c], D[c]);
+          }
         }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        rec.dbox[c] = (float)(fmn[c] - 1e-3);
+        rec.dbox[3 + c] = (float)(fmx[c] + 1e-3);
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -695,7 +705,9 @@ __device__ __forceinline__ bool trans_valid64(const dp_dev_sys& S, const double*
 // (inflated by the robot radius and 1e-9 m) are listed in the lane's LDS
 // slice, and the second rollout tests each state against them only.  Both
 // rollouts form the same states.  Other pairs are rolled out once, for the
-// goal checks, the final state, g and h.
+// goal checks, the final state, g and h.  The velocity increments of the
+// next four steps are in flight during each step: with about one wave per
+// SIMD the pass is bound by load latency.
 __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
   extern __shared__ int16_t q64_cand[];  // [blockDim.x][kQ64Cand]
   const dp_dev_sys& S = *a.sys;
@@ -718,11 +730,31 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
     int first_bad = INT_MAX, first_col = INT_MAX;
     const int32_t rbad = a.rbad64[j];
     int ncand = 0;  // obstacles listed for the second rollout (-1: more than kQ64Cand, test all)
+    const double4* dvp = reinterpret_cast<const double4*>(a.dv64) + s0;
     if (full) {
+      // first rollout: bounds of every state and the box of the positions
       quad64_start(S, a, i, j, s);
       double bl[3] = {s[0], s[1], s[2]}, bh[3] = {s[0], s[1], s[2]};
+      double3 d1 = make_double3(0.0, 0.0, 0.0), d2 = d1, d3 = d1, d4 = d1;
+      if (1 < L1) d1 = make_double3(dvp[1].x, dvp[1].y, dvp[1].z);
+      if (2 < L1) d2 = make_double3(dvp[2].x, dvp[2].y, dvp[2].z);
+      if (3 < L1) d3 = make_double3(dvp[3].x, dvp[3].y, dvp[3].z);
+      if (4 < L1) d4 = make_double3(dvp[4].x, dvp[4].y, dvp[4].z);
       for (int k = 0; k < L1; ++k) {
-        if (k > 0) quad64_advance(a, s0, k, s);
+        if (k > 0) {
+          const double dv[3] = {d1.x, d1.y, d1.z};
+          double o[13];
+          dp_quad_trans_step64(S.dt, s, dv, o);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            s[c] = o[c];
+            s[7 + c] = o[7 + c];
+          }
+          d1 = d2;
+          d2 = d3;
+          d3 = d4;
+          if (k + 4 < L1) d4 = make_double3(dvp[k + 4].x, dvp[k + 4].y, dvp[k + 4].z);
+        }
         if (k >= rbad || !trans_valid64(S, s)) {
           first_bad = k;
           break;
@@ -765,8 +797,32 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
     }
     int chosen = -1, nn = 0;
     const int nng = a.check_goal ? a.num_goal : 0;
+    // the goal indices of the pair's primitive, loaded before the rollout
+    // (at most 8; a load per step in the loop would serialise on its latency)
+    int32_t gi[8];
+#pragma unroll
+    for (int g = 0; g < 8; ++g) gi[g] = g < nng ? a.goalidx[(int64_t)g * a.P + j] : -1;
+    int32_t gnext = nng > 0 ? gi[0] : -1;
+    double3 d1 = make_double3(0.0, 0.0, 0.0), d2 = d1, d3 = d1, d4 = d1;
+    if (1 < L1) d1 = make_double3(dvp[1].x, dvp[1].y, dvp[1].z);
+    if (2 < L1) d2 = make_double3(dvp[2].x, dvp[2].y, dvp[2].z);
+    if (3 < L1) d3 = make_double3(dvp[3].x, dvp[3].y, dvp[3].z);
+    if (4 < L1) d4 = make_double3(dvp[4].x, dvp[4].y, dvp[4].z);
     for (int k = 0; k < L1 && first_bad == INT_MAX; ++k) {
-      if (k > 0) quad64_advance(a, s0, k, s);
+      if (k > 0) {
+        const double dv[3] = {d1.x, d1.y, d1.z};
+        double o[13];
+        dp_quad_trans_step64(S.dt, s, dv, o);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          s[c] = o[c];
+          s[7 + c] = o[7 + c];
+        }
+        d1 = d2;
+        d2 = d3;
+        d3 = d4;
+        if (k + 4 < L1) d4 = make_double3(dvp[k + 4].x, dvp[k + 4].y, dvp[k + 4].z);
+      }
       if (full && first_col == INT_MAX) {
         bool col;
         if (ncand < 0) {
@@ -782,7 +838,7 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
         if (col) first_col = k;
       }
       // check_goal at the float-rule indices, in nn order (dbastar.cpp:492-512)
-      while (chosen == -1 && nn < nng && a.goalidx[(int64_t)nn * a.P + j] == k) {
+      while (chosen == -1 && nn < nng && gnext == k) {
         quad64_rot(a, s0 + k, s);
         if (dp_within(S, s, a.goal, a.goal_bound)) {
           chosen = k;
@@ -791,6 +847,9 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
           for (int c = 0; c < 13; ++c) a.fin[(int64_t)c * a.pcap + q] = s[c];
         }
         ++nn;
+#pragma unroll
+        for (int g = 1; g < 8; ++g)
+          if (nn == g) gnext = gi[g];
       }
     }
     if (first_bad == INT_MAX) quad64_rot(a, s0 + L1 - 1, s);  // the full final state
