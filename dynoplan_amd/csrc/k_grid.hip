@@ -181,37 +181,75 @@ __device__ __forceinline__ void scan_earlier(const dp_dev_sys& S, const dp_grid_
 // pool ran out), for which the resolution scans the grid.  The order of a
 // list's entries carries no meaning.
 
-// The non-empty cells of the contender grid.  Each workgroup takes a
-// contiguous range of cells, counts its non-empty ones, reserves room with
-// one global atomic, then lists them in cell order.
-__global__ __launch_bounds__(256) void k_cell_list(int64_t ncells, const int32_t* __restrict__ cstart,
+// Contenders in the 27-cell block around `cell` (the runs k_conf_waves
+// reads, with the same neighbour rules).
+__device__ __forceinline__ int32_t cell_block_count(const dp_grid_geom& G, const int32_t* __restrict__ cstart,
+                                                    int64_t cell) {
+  const int64_t d01 = (int64_t)G.dims[0] * G.dims[1];
+  const int32_t c0 = (int32_t)(cell % G.dims[0]), c1 = (int32_t)((cell / G.dims[0]) % G.dims[1]);
+  const int32_t c2 = (int32_t)(cell / d01);
+  const int32_t x0 = c0 > 0 ? c0 - 1 : 0, x1 = c0 + 1 < G.dims[0] ? c0 + 1 : G.dims[0] - 1;
+  int32_t K = 0;
+  for (int k = 0; k < 9; ++k) {
+    const int32_t n1 = c1 + k % 3 - 1;
+    const int32_t d2 = k / 3 - 1;
+    int32_t n2 = c2 + d2;
+    bool ok = n1 >= 0 && n1 < G.dims[1];
+    if (G.dims[2] == 1) {
+      ok = ok && d2 == 0;
+      n2 = 0;
+    } else if (G.periodic2) {
+      n2 = n2 < 0 ? n2 + G.dims[2] : (n2 >= G.dims[2] ? n2 - G.dims[2] : n2);
+    } else {
+      ok = ok && n2 >= 0 && n2 < G.dims[2];
+    }
+    if (ok) {
+      const int64_t row = ((int64_t)n2 * G.dims[1] + n1) * G.dims[0];
+      K += cstart[row + x1 + 1] - cstart[row + x0];
+    }
+  }
+  return K;
+}
+
+// The non-empty cells of the contender grid, one thread per cell: each
+// workgroup reserves room for its listed cells with one global atomic
+// (cells are listed in no particular order).  A cell whose 27-cell
+// block holds its one contender only is settled here (no earlier
+// contender can conflict: an empty list, and NUC unless goal-forced) and is
+// not listed, when no window applies.
+__global__ __launch_bounds__(256) void k_cell_list(int64_t ncells, dp_grid_geom G, const int32_t* __restrict__ cstart,
+                                                   const int32_t* __restrict__ cidx, int32_t window,
+                                                   int8_t* __restrict__ flags, int16_t* __restrict__ conf_cnt,
                                                    int32_t* __restrict__ list, long long* __restrict__ counters) {
   __shared__ int32_t s_cnt;
   __shared__ long long s_base;
   const int lane = threadIdx.x % 64;
-  const int64_t per = (ncells + gridDim.x - 1) / gridDim.x;
-  const int64_t lo = (int64_t)blockIdx.x * per;
-  const int64_t hi = lo + per < ncells ? lo + per : ncells;
+  // one cell per thread (the grid covers every cell)
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int kind = 0;  // 0: empty, 1: listed, 2: settled here
+  if (c < ncells) {
+    const int32_t b = cstart[c], e = cstart[c + 1];
+    if (e > b) kind = window > 0 || e - b != 1 || cell_block_count(G, cstart, c) != 1 ? 1 : 2;
+    if (kind == 2) {
+      const int32_t q = cidx[b];
+      const int8_t fl = flags[q];
+      if (!(fl & 2)) {  // goal-forced pairs are NOVEL without a list
+        conf_cnt[q] = 0;
+        flags[q] = (int8_t)(fl | DP_F_NUC);
+      }
+    }
+  }
   if (threadIdx.x == 0) s_cnt = 0;
   __syncthreads();
-  int mine = 0;
-  for (int64_t c = lo + threadIdx.x; c < hi; c += blockDim.x) mine += cstart[c + 1] > cstart[c] ? 1 : 0;
-  if (mine) atomicAdd(&s_cnt, mine);
+  const uint64_t m = __ballot(kind == 1);
+  int woff = 0;
+  if (lane == 0 && m) woff = atomicAdd(&s_cnt, __popcll(m));
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
     s_base = s_cnt ? (long long)atomicAdd((unsigned long long*)&counters[DPC_CELLS], (unsigned long long)s_cnt) : 0;
-    s_cnt = 0;
-  }
   __syncthreads();
-  for (int64_t c0 = lo; c0 < hi; c0 += blockDim.x) {
-    const int64_t c = c0 + threadIdx.x;
-    const bool ne = c < hi && cstart[c + 1] > cstart[c];
-    const uint64_t m = __ballot(ne);
-    int woff = 0;
-    if (lane == 0 && m) woff = atomicAdd(&s_cnt, __popcll(m));
-    woff = __shfl(woff, 0, 64);
-    if (ne) list[s_base + woff + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)c;
-  }
+  woff = __shfl(woff, 0, 64);
+  if (kind == 1) list[s_base + woff + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)c;
 }
 
 // One wave per non-empty grid cell: the 27 cells around it
@@ -1178,8 +1216,13 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
                                           c->d_cont_start, c->d_cont_cursor, c->d_cont_idx, c->d_cont_x);
     // earlier-conflict lists; the scatter cursors are free again and hold
     // the list of non-empty cells
-    k_cell_list<<<dp_grid_blocks(nc, 4096, 1024), 256, 0, s>>>(nc, c->d_cont_start, c->d_cont_cursor, c->d_counters);
-    k_conf_waves<NX><<<(int)std::min<int64_t>((nc + 3) / 4, 8192), 256, 0, s>>>(
+    k_cell_list<<<(int)((nc + 255) / 256), 256, 0, s>>>(nc, c->grid, c->d_cont_start, c->d_cont_idx, window,
+                                                                c->d_flags, c->d_conf_cnt, c->d_cont_cursor,
+                                                                c->d_counters);
+    // sparse contenders (the quadrotor's 3-D grid) leave most of a large
+    // grid's waves idle: a smaller grid there
+    const int64_t cw_blocks = std::min<int64_t>((nc + 3) / 4, NX == 13 ? 2048 : 8192);
+    k_conf_waves<NX><<<(int)cw_blocks, 256, 0, s>>>(
         c->d_sys, c->grid, c->d_flags, c->d_pnode, c->d_app, window, c->pair_cap, c->d_cont_start, c->d_cont_idx, c->d_cont_x, c->d_cont_cursor,
         r_dup, c->d_conf_cnt, c->d_conf, c->d_conf_ov, c->d_conf_pool, pool_cap, c->conf_slots, c->d_counters);
     if (dp_record_event(c, 7, s)) return DP_ERR_HIP;
