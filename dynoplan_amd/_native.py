@@ -249,7 +249,6 @@ SIGNATURES = {
     "dp_batch_expand": (C.c_int, [C.c_void_p, C.POINTER(ExpandParams), C.c_void_p]),
     "dp_batch_latency": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "dp_batch_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
-    "dp_debug_fp32_positions": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int64]),
     "dp_load_primitives": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_uint64,
                                      C.POINTER(PrimSet)]),
     "dp_free_primitives": (None, [C.POINTER(PrimSet)]),

@@ -280,7 +280,6 @@ void dp_destroy(dp_ctx* c) {
   dev_free(&c->d_node_cursor);
   dev_free(&c->d_node_count);
   dev_free(&c->d_chunk);
-  dev_free(&c->d_dbg32);
   dev_free(&c->d_q64);
   dev_free(&c->d_qperm);
   dev_free(&c->d_chunk_node);
@@ -657,7 +656,6 @@ int dp_reserve(dp_ctx* c, int32_t max_nodes, int64_t max_pairs) {
     if ((rc = dev_alloc(&c->d_conf_ov, max_pairs))) return rc;
     c->conf_pool_cap = 2 * max_pairs;  // about 1 in 10 lists overflows, by ~15 entries, on dense batches
     if ((rc = dev_alloc(&c->d_conf_pool, c->conf_pool_cap))) return rc;
-    if (c->desc.system == DP_SYS_QUADROTOR && (rc = dev_alloc(&c->d_dbg32, 3 * max_pairs))) return rc;
     if (c->desc.system == DP_SYS_QUADROTOR && (rc = dev_alloc(&c->d_q64, max_pairs))) return rc;
     if (c->desc.system == DP_SYS_QUADROTOR && (rc = dev_alloc(&c->d_qperm, 2 * max_pairs))) return rc;
     c->pair_cap = max_pairs;
@@ -1063,19 +1061,6 @@ int dp_visited_append_novel(dp_ctx* c) {
   return DP_OK;
 }
 
-int dp_debug_fp32_positions(dp_ctx* c, float* out, int64_t n) {
-  ARG_CHECK(c && out && n >= 0, "invalid argument");
-  if (!c->d_dbg32) {
-    dp_set_error("no FP32 rollout data (quadrotor contexts only)");
-    return DP_ERR_STATE;
-  }
-  ARG_CHECK(n <= c->pair_cap, "n exceeds the pair capacity");
-  HIP_CHECK(hipSetDevice(c->device));
-  HIP_CHECK(hipDeviceSynchronize());
-  for (int k = 0; k < 3; ++k)
-    HIP_CHECK(hipMemcpy(out + k * n, c->d_dbg32 + (size_t)k * c->pair_cap, sizeof(float) * n, hipMemcpyDeviceToHost));
-  return DP_OK;
-}
 
 int dp_last_timing(dp_ctx* c, float* ms5) {
   ARG_CHECK(c && ms5, "null argument");

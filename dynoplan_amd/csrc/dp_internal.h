@@ -244,7 +244,6 @@ struct dp_ctx {
   int32_t* d_qperm = nullptr;       // [2][pair_cap] quadrotor: (pair, primitive) per node, rate-bound ones first
   int32_t* d_chunk_node = nullptr;  // quadrotor: node of each FP32 wave chunk
   int64_t chunk_node_cap = 0;
-  float* d_dbg32 = nullptr;         // [3][pair_cap] quadrotor: FP32 final positions (error report)
   int32_t* d_q64 = nullptr;         // [pair_cap] quadrotor: pairs the FP64 pass rolls out (DPC_Q64 of them)
   // quadrotor, per primitive state: attitude / rate trajectory and velocity
   // increments shared by every pair (k_quad_rot); per primitive: first state

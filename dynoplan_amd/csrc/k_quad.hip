@@ -86,7 +86,6 @@ struct dp_quad_args {
   int64_t pcap;
   double* gout;
   double* hout;
-  float* dbg32;           // [3][pcap] FP32 final position (absolute) of FREE pairs, or null
   double goal[DP_MAX_NX];
   double goal_bound, r_dup, cdf;
   int32_t exact;
@@ -614,11 +613,6 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
         fk = -1;
         near = near_bound_any || near_col_before;
       }
-      if (a.dbg32) {
-        const bool f = vd == DP_V_FREE;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) a.dbg32[c * a.pcap + q] = f ? (float)(np[c] + (double)s[c]) : __builtin_nanf("");
-      }
       if (near) {
         a.flags[q] = QF_NEAR;
       } else if (vd == DP_V_FREE) {
@@ -922,17 +916,12 @@ __global__ __launch_bounds__(256) void k_quad_dupv(dp_quad_args a) {
   }
 }
 
-__device__ __forceinline__ void wave_sync_lds() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Per node: its pairs reordered for the FP32 pass, those whose primitive
 // leaves the attitude / rate bounds within its length first (their verdict
 // is BOUNDS whatever the translation, so their rollouts are short and need
-// no collision test) by that step, then the rest in canonical order; and the
-// node's wave chunks (64 pairs, one group per chunk).  One wave per node.
+// no collision test), then the rest, each group in canonical order (the
+// results are written nearly in order); and the node's wave chunks (64
+// pairs, one group per chunk).  One wave per node.
 __global__ __launch_bounds__(64) void k_quad_split(int32_t N, const int32_t* __restrict__ app,
                                                    const int32_t* __restrict__ total, const int32_t* __restrict__ pprim,
                                                    const int32_t* __restrict__ off, const int32_t* __restrict__ rbad32,
@@ -955,44 +944,18 @@ __global__ __launch_bounds__(64) void k_quad_split(int32_t N, const int32_t* __r
     }
     ns += __popcll(__ballot(sh));
   }
-  // short pairs ordered by the step at which their primitive leaves the
-  // rate bounds (a counting sort over 64 bins), so that the lanes of a
-  // chunk roll out about as many steps each; results do not depend on it
-  __shared__ int32_t s_bin[64];
-  s_bin[lane] = 0;
-  wave_sync_lds();
-  for (int32_t cb = b; cb < e; cb += 64) {
-    const int32_t p = cb + lane;
-    if (p < e) {
-      const int32_t j = pprim[p];
-      const int32_t rb = rbad32[j];
-      if (rb < off[j + 1] - off[j]) atomicAdd(&s_bin[rb < 63 ? rb : 63], 1);
-    }
-  }
-  wave_sync_lds();
-  {
-    int32_t incl = s_bin[lane];
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int32_t v = __shfl_up(incl, o);
-      if (lane >= o) incl += v;
-    }
-    s_bin[lane] = b + incl - s_bin[lane];  // first slot of each bin
-  }
-  wave_sync_lds();
-  int32_t wl = b + ns;
+  int32_t ws = b, wl = b + ns;
   for (int32_t cb = b; cb < e; cb += 64) {
     const int32_t p = cb + lane;
     bool sh = false;
-    int32_t rb = 0, j = 0;
+    int32_t j = 0;
     if (p < e) {
       j = pprim[p];
-      rb = rbad32[j];
-      sh = rb < off[j + 1] - off[j];
+      sh = rbad32[j] < off[j + 1] - off[j];
     }
-    const uint64_t ml = __ballot(p < e && !sh);
-    if (sh) qperm[atomicAdd(&s_bin[rb < 63 ? rb : 63], 1)] = make_int2(p, j);
-    else if (p < e) qperm[wl + __popcll(ml & lt)] = make_int2(p, j);
+    const uint64_t ms = __ballot(sh), ml = __ballot(p < e && !sh);
+    if (p < e) qperm[sh ? ws + __popcll(ms & lt) : wl + __popcll(ml & lt)] = make_int2(p, j);
+    ws += __popcll(ms);
     wl += __popcll(ml);
   }
   if (lane == 0) {
@@ -1187,7 +1150,6 @@ dp_quad_args make_quad_args(dp_ctx* c, const dp_expand_params* p, double goal_bo
   a.pcap = c->pair_cap;
   a.gout = c->d_g;
   a.hout = c->d_h;
-  a.dbg32 = c->d_dbg32;
   for (int k = 0; k < DP_MAX_NX; ++k) a.goal[k] = k < 13 ? p->goal[k] : 0.0;
   a.goal_bound = goal_bound;
   a.r_dup = r_dup;

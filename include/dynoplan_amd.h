@@ -391,12 +391,6 @@ int dp_surrogate_get(dp_ctx* ctx, float* params, float* emb);
 /* Numerics check of the training GEMM: C = A B, inputs rounded to BF16. */
 int dp_debug_gemm_bf16(dp_ctx* ctx, const float* A, const float* B, int32_t M, int32_t N, int32_t K, float* C);
 
-/* Quadrotor FP32 rollout error report: the FP32 final position (absolute,
- * [3][n] row per axis) of every pair the FP32 pass found FREE in the last
- * dp_expand (other entries unspecified).  No counterpart in the reference;
- * used to report the FP32-vs-FP64 rollout error per system. */
-int dp_debug_fp32_positions(dp_ctx* ctx, float* out, int64_t n);
-
 /* Quadrotor rollout error report, every state: the states of the listed
  * pairs (indices in the last dp_expand's canonical pair order) as the FP32
  * pass forms them (bits = 32, positions made absolute in FP64) or as the
