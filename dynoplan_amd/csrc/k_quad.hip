@@ -703,10 +703,17 @@ __device__ __forceinline__ bool trans_valid64(const dp_dev_sys& S, const double*
 // next four steps are in flight during each step: with about one wave per
 // SIMD the pass is bound by load latency.
 __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
-  extern __shared__ int16_t q64_cand[];  // [blockDim.x][kQ64Cand]
+  // LDS: the obstacles (boxes [nb][6] then spheres [ns][4]), then the
+  // per-lane candidate lists [blockDim.x][kQ64Cand]
+  extern __shared__ double q64_lds[];
   const dp_dev_sys& S = *a.sys;
   const int64_t n64 = quad64_count(a);
-  int16_t* cand = q64_cand + (int64_t)threadIdx.x * kQ64Cand;
+  if ((int64_t)blockIdx.x * blockDim.x >= n64) return;  // uniform per block
+  const int nobs_d = 6 * a.nb + 4 * a.ns;
+  double* obs = q64_lds;
+  for (int t = threadIdx.x; t < nobs_d; t += blockDim.x) obs[t] = a.obs[t];
+  __syncthreads();
+  int16_t* cand = reinterpret_cast<int16_t*>(q64_lds + nobs_d) + (int64_t)threadIdx.x * kQ64Cand;
   long long c_b = 0, c_c = 0, c_f = 0, c_k = 0, c_r = 0;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n64; t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t q = quad64_pair(a, t);
@@ -765,7 +772,7 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
         const double rr = S.radius + 1e-9;
         for (int o = 0; o < a.nb + a.ns && ncand >= 0; ++o) {
           const bool box = o < a.nb;
-          const double* b = box ? a.obs + 6 * o : a.obs + 6 * a.nb + 4 * (o - a.nb);
+          const double* b = box ? obs + 6 * o : obs + 6 * a.nb + 4 * (o - a.nb);
           bool meet = true;
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
@@ -820,13 +827,13 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
       if (full && first_col == INT_MAX) {
         bool col;
         if (ncand < 0) {
-          col = collides64(S, a.obs, a.nb, a.ns, s);
+          col = collides64(S, obs, a.nb, a.ns, s);
         } else {
           col = false;
           for (int e = 0; e < ncand && !col; ++e) {
             const int o = cand[e];
-            col = o < a.nb ? dp_sphere_box(s[0], s[1], s[2], S.radius, a.obs + 6 * o)
-                           : dp_sphere_sphere(s[0], s[1], s[2], S.radius, a.obs + 6 * a.nb + 4 * (o - a.nb));
+            col = o < a.nb ? dp_sphere_box(s[0], s[1], s[2], S.radius, obs + 6 * o)
+                           : dp_sphere_sphere(s[0], s[1], s[2], S.radius, obs + 6 * a.nb + 4 * (o - a.nb));
           }
         }
         if (col) first_col = k;
@@ -1180,6 +1187,11 @@ int dp_launch_quad_rot(dp_ctx* c, hipStream_t s) {
   return DP_OK;
 }
 
+// LDS of a k_quad64 workgroup of t64 lanes: the obstacles, the candidate lists.
+static size_t quad64_lds(const dp_ctx* c, int t64) {
+  return sizeof(double) * (size_t)(6 * c->nb + 4 * c->ns) + sizeof(int16_t) * (size_t)t64 * kQ64Cand;
+}
+
 int32_t dp_fp64_window(const dp_ctx* c, int32_t limit) {
   if (c->fp64_window > 0) return c->fp64_window;
   return limit >= 0 ? DP_FP64_WINDOW(limit) : INT32_MAX;
@@ -1190,7 +1202,7 @@ int dp_launch_quad_deferred(dp_ctx* c, hipStream_t s) {
   dp_quad_args a = make_quad_args(c, p, p->delta_factor_goal * (double)p->delta, c->round_r_dup);
   a.redo = 1;  // both kernels over every pair of the batch
   const int t64 = c->tiles.quad64_threads;
-  k_quad64<<<dp_grid_blocks(c->pair_cap, t64, c->tiles.quad64_blocks), t64, sizeof(int16_t) * t64 * kQ64Cand, s>>>(a);
+  k_quad64<<<dp_grid_blocks(c->pair_cap, t64, c->tiles.quad64_blocks), t64, quad64_lds(c, t64), s>>>(a);
   if (c->nv > 0) k_quad_dupv<<<dp_grid_blocks(c->pair_cap, 256, 8192), 256, 0, s>>>(a);
   DP_HIP_RET(hipGetLastError());
   return DP_OK;
@@ -1258,7 +1270,7 @@ int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, doub
     if (rc) return rc;
   }
   const int t64 = c->tiles.quad64_threads;
-  k_quad64<<<dp_grid_blocks(c->pair_cap, t64, c->tiles.quad64_blocks), t64, sizeof(int16_t) * t64 * kQ64Cand, s>>>(a);
+  k_quad64<<<dp_grid_blocks(c->pair_cap, t64, c->tiles.quad64_blocks), t64, quad64_lds(c, t64), s>>>(a);
   if (c->nv > 0) k_quad_dupv<<<dp_grid_blocks(c->pair_cap, 256, 8192), 256, 0, s>>>(a);
   DP_HIP_RET(hipGetLastError());
   return DP_OK;
