@@ -1219,10 +1219,7 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
     k_cell_list<<<(int)((nc + 255) / 256), 256, 0, s>>>(nc, c->grid, c->d_cont_start, c->d_cont_idx, window,
                                                                 c->d_flags, c->d_conf_cnt, c->d_cont_cursor,
                                                                 c->d_counters);
-    // sparse contenders (the quadrotor's 3-D grid) leave most of a large
-    // grid's waves idle: a smaller grid there
-    const int64_t cw_blocks = std::min<int64_t>((nc + 3) / 4, NX == 13 ? 2048 : 8192);
-    k_conf_waves<NX><<<(int)cw_blocks, 256, 0, s>>>(
+    k_conf_waves<NX><<<(int)std::min<int64_t>((nc + 3) / 4, 8192), 256, 0, s>>>(
         c->d_sys, c->grid, c->d_flags, c->d_pnode, c->d_app, window, c->pair_cap, c->d_cont_start, c->d_cont_idx, c->d_cont_x, c->d_cont_cursor,
         r_dup, c->d_conf_cnt, c->d_conf, c->d_conf_ov, c->d_conf_pool, pool_cap, c->conf_slots, c->d_counters);
     if (dp_record_event(c, 7, s)) return DP_ERR_HIP;
