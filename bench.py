@@ -221,6 +221,18 @@ def bench_multi(args):
         p.fp32_rollout = 1 if w.desc.system == N.SYS_QUADROTOR else 0
         params.append(p)
     progress(f"uploaded {len(exs)} contexts")
+    # launch shapes: the untimed autotuner on the largest instance of each
+    # system, applied to every instance of that system
+    tiles_by_system = {}
+    if not args.no_autotune:
+        for sysid in sorted({w.desc.system for w in wls}):
+            ids = [i for i, w in enumerate(wls) if w.desc.system == sysid]
+            k = max(ids, key=lambda i: wls[i].nodes.shape[0] * (wls[i].offsets.size - 1))
+            tiles, _ = exs[k].autotune(params[k])
+            tiles_by_system[int(sysid)] = tiles
+            for i in ids:
+                exs[i].set_tiles(**tiles)
+        progress(f"autotune per system: {tiles_by_system}")
 
     def health_sum():
         hs = [e.health() for e in exs]
@@ -294,7 +306,7 @@ def bench_multi(args):
         "config": {"workload": f"{len(wls)} mixed instances (unicycle1, unicycle2, quadrotor), O U{{10..300}}, "
                                f"P U{{1000..10000}}, N U{{64..4096}}, V 10000, one hipGraph launch",
                    "instances": len(wls), "nodes": int(sum(w.nodes.shape[0] for w in wls)),
-                   "pairs_per_step": int(pairs), "surrogate": False},
+                   "pairs_per_step": int(pairs), "surrogate": False, "tiles_by_system": tiles_by_system},
         "latency_ms": {"instance_p50": float(np.percentile(lat_g, 50)), "instance_p99": float(np.percentile(lat_g, 99)),
                        "batch_p50": float(np.percentile(batch_lat, 50)), "batch_p99": float(np.percentile(batch_lat, 99))},
         "streams_value": pairs * args.steps / el_s,

@@ -204,6 +204,8 @@ dp_ctx* dp_create(const dp_system_desc* sys, int32_t device) {
   ok = ok && hipMemset(c->d_sticky, 0, sizeof(long long) * DPS_COUNT) == hipSuccess;
   ok = ok && dev_alloc(&c->d_wshift, 2) == DP_OK;
   ok = ok && hipMemset(c->d_wshift, 0, sizeof(int32_t) * 2) == hipSuccess;
+  ok = ok && dev_alloc(&c->d_lneed, 1) == DP_OK;
+  ok = ok && hipMemset(c->d_lneed, 0, sizeof(int32_t)) == hipSuccess;
   ok = ok && dev_alloc(&c->d_total, 1) == DP_OK;
   ok = ok && hipMemset(c->d_counters, 0, sizeof(long long) * DPC_COUNT) == hipSuccess;
   ok = ok && hipMemset(c->d_total, 0, sizeof(int32_t)) == hipSuccess;
@@ -225,6 +227,7 @@ void dp_destroy(dp_ctx* c) {
   dp_mlp_free(c);
   dev_free(&c->d_sys);
   dev_free(&c->d_wshift);
+  dev_free(&c->d_lneed);
   dev_free(&c->d_trace);
   dev_free(&c->d_node_obs);
   dev_free(&c->d_node_nobs);

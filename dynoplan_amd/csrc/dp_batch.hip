@@ -1,15 +1,20 @@
 // dp_batch.hip -- many independent planning instances in one launch.
 //
-// A batch holds n expansion contexts (any mix of systems on one device).  In
-// graph mode the pipelines of all instances are captured once into one
-// hipGraph, one branch per instance forked from a master stream, and replayed
-// by a single hipGraphLaunch while nothing the graph bakes in has changed
-// (tracked by a global state epoch that every reallocation and every change
-// of nodes count, primitives, world, visited set or surrogate bumps).  In
-// stream mode the same branches are launched directly, one stream per
-// instance.  A one-thread kernel at the start of the batch and one at the end
-// of each branch stamp the GPU's constant-rate clock, which gives each
-// instance's completion latency without host round trips.
+// A batch holds n expansion contexts (any mix of systems on one device).
+// Each instance's pipeline up to its conflict lists runs on its own stream,
+// forked from a master stream.  The ordered walks, the latency-bound stage of
+// each instance (a chain of dependent nodes that keeps few waves busy), then
+// run as one work queue per system class on the master stream: waves claim
+// (instance, node) items interleaved over the instances, so the chains of
+// all instances advance together (dp_walk_queue, k_grid.hip).  In graph mode
+// the whole batch is captured once into one hipGraph and replayed by a single
+// hipGraphLaunch while nothing the graph bakes in has changed (tracked by a
+// global state epoch that every reallocation and every change of nodes
+// count, primitives, world, visited set or surrogate bumps).  In stream mode
+// the same work is launched directly.  A one-thread kernel at the start of
+// the batch stamps the GPU's constant-rate clock, and the wave that finishes
+// an instance's last node stamps it again, which gives each instance's
+// completion latency without host round trips.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -38,7 +43,8 @@ struct dp_batch {
   hipGraphExec_t exec = nullptr;
   uint64_t epoch = 0;
   std::vector<dp_expand_params> params;
-  unsigned long long* d_stamp = nullptr;  // [n] branch ends, [n] batch start
+  unsigned long long* d_stamp = nullptr;  // [n] instance completions, [n] batch start
+  dp_walk_queue* walk = nullptr;
   int wall_khz = 0;
   bool stamped = false;
   int64_t captures = 0, launches = 0;
@@ -89,8 +95,8 @@ static int prepare(dp_batch* b, const dp_expand_params* params) {
   return DP_OK;
 }
 
-// Start stamp, fork, one pipeline per instance on its own stream, end
-// stamps, join back into the master stream.
+// Start stamp, fork, one pipeline per instance on its own stream up to its
+// conflict lists, join back into the master stream, then the walk queue.
 static int launch_branches(dp_batch* b, const dp_expand_params* params, bool capturing) {
   const int32_t n = (int32_t)b->ctx.size();
   k_stamp<<<1, 1, 0, b->stream>>>(b->d_stamp + n);
@@ -100,16 +106,20 @@ static int launch_branches(dp_batch* b, const dp_expand_params* params, bool cap
     dp_ctx* c = b->ctx[i];
     B_HIP(hipStreamWaitEvent(c->stream, b->fork, 0));
     c->no_events = capturing;
+    c->defer_walk = true;
     int rc = dp_launch_expand(c, &params[i], c->stream);
     c->no_events = false;
+    c->defer_walk = false;
     if (rc) return rc;
     c->last_pairs = -1;
-    k_stamp<<<1, 1, 0, c->stream>>>(b->d_stamp + i);
-    B_HIP(hipGetLastError());
     B_HIP(hipEventRecord(b->join[i], c->stream));
     B_HIP(hipStreamWaitEvent(b->stream, b->join[i], 0));
   }
-  return DP_OK;
+  // the walk arguments are known once every pipeline is launched; a capture
+  // reuses those of the eager pass before it
+  int rc = capturing ? DP_OK : dp_walk_queue_prepare(b->walk, b->ctx.data(), b->last);
+  if (!rc) rc = dp_walk_queue_launch(b->walk, b->d_stamp, b->stream);
+  return rc;
 }
 
 static int capture(dp_batch* b, const dp_expand_params* params) {
@@ -172,6 +182,7 @@ dp_batch* dp_batch_create(dp_ctx* const* ctxs, int32_t n, int32_t mode) {
   ok = ok && hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
   for (int32_t i = 0; i < n && ok; ++i) ok = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) == hipSuccess;
   ok = ok && hipMalloc(&b->d_stamp, sizeof(unsigned long long) * (n + 1)) == hipSuccess;
+  ok = ok && (b->walk = dp_walk_queue_create(n)) != nullptr;
   ok = ok && hipDeviceGetAttribute(&b->wall_khz, hipDeviceAttributeWallClockRate, b->device) == hipSuccess;
   if (!ok || b->wall_khz <= 0) {
     dp_set_error("batch creation failed (stream, events, stamps or clock rate)");
@@ -190,6 +201,7 @@ void dp_batch_destroy(dp_batch* b) {
   for (hipEvent_t e : b->join)
     if (e) (void)hipEventDestroy(e);
   if (b->d_stamp) (void)hipFree(b->d_stamp);
+  dp_walk_queue_destroy(b->walk);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }

@@ -24,6 +24,13 @@
 // Pair flag (limit_branching_factor == 0): a FREE pair after its node's
 // first FREE pair, CAPPED before the walk.
 #define DP_F_PRECAP 0x40
+// Pair flag (planar systems, limit_branching_factor > 0): a FREE pair past
+// its node's list window.  It stays a contender (later pairs test against
+// it) but gets no stored list of earlier conflicts; the walk scans the
+// contender grid for it if its node's cap does not fall before it.  (The
+// quadrotor's check stage uses this bit for its own state before it writes
+// the final flags, and never sets it for the walk.)
+#define DP_F_NOLIST 0x08
 // FP64 window of a quadrotor node, in FREE pairs: enough for the
 // cap of limit NOVEL pairs with a wide margin for duplicates.
 #define DP_FP64_WINDOW(limit) (3 * (limit) + 32)
@@ -282,6 +289,7 @@ struct dp_ctx {
   long long* d_counters = nullptr;  // [DPC_COUNT]
   long long* d_sticky = nullptr;    // [DPS_COUNT] never reset
   int32_t* d_wshift = nullptr;      // [2] quadrotor FP64 window growth (k_quad_window)
+  int32_t* d_lneed = nullptr;       // [1] planar list window need (k_list_window)
   int64_t n_batches = 0;            // dp_expand calls
   int64_t host_completions = 0;     // batches finished by host-launched work in dp_get_counts
   // test-only limits (dp_debug_set_limits): inline conflict slots used
@@ -299,6 +307,7 @@ struct dp_ctx {
   hipEvent_t ev[DP_EVENTS] = {};
   bool timing_valid = false;
   bool no_events = false;           // set while a multi-instance graph is captured
+  bool defer_walk = false;          // set while a multi-instance batch launches: no walk (dp_walk_queue)
   int32_t poll_cap = 16;            // resolution: longest pause between polls (64-cycle sleeps)
   unsigned long long* d_trace = nullptr;  // [4 N_cap] resolution per node: start, end, waits, polls
   int64_t trace_cap = 0;
@@ -357,6 +366,18 @@ int dp_launch_more_rounds(dp_ctx* c, int rounds, hipStream_t s);
 // Resolution from scratch after a window overflow: every FREE pair
 // undecided again, resolution counters cleared, lists and walk rebuilt.
 int dp_launch_resolve_again(dp_ctx* c, hipStream_t s);
+// The ordered walks of a multi-instance batch as one work queue: contexts
+// expanded with defer_walk set stop before their walk, and one launch per
+// system class walks every instance, interleaved, stamping each instance's
+// completion clock into stamps[i].  prepare (host, synchronous: it first
+// waits for `prev`, the stream of the previous launch, if the arrays change)
+// uploads the contexts' walk arguments; it runs after their pipelines were
+// launched and before any capture of the queue launch.
+struct dp_walk_queue;
+dp_walk_queue* dp_walk_queue_create(int32_t n);
+void dp_walk_queue_destroy(dp_walk_queue* q);
+int dp_walk_queue_prepare(dp_walk_queue* q, dp_ctx* const* ctx, hipStream_t prev);
+int dp_walk_queue_launch(dp_walk_queue* q, unsigned long long* stamps, hipStream_t s);
 
 // ---- k_radius.hip -------------------------------------------------------
 int dp_noalloc_init_device(int device);

@@ -979,7 +979,7 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
     rc = dp_launch_resolve(c, p, r_dup, s);
     if (rc) return rc;
     DP_EV(c->ev[3], s);
-    c->timing_valid = !c->no_events;
+    c->timing_valid = !c->no_events && !c->defer_walk;
     return DP_OK;
   }
   int lpp = c->tiles.lanes_per_pair > 0 ? c->tiles.lanes_per_pair
@@ -1036,12 +1036,12 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
     DP_HIP_RET(hipGetLastError());
     DP_EV(c->ev[7], s);
     DP_EV(c->ev[3], s);
-    c->timing_valid = !c->no_events;
+    c->timing_valid = !c->no_events && !c->defer_walk;
     return DP_OK;
   }
   rc = dp_launch_resolve(c, p, r_dup, s);
   if (rc) return rc;
   DP_EV(c->ev[3], s);
-  c->timing_valid = !c->no_events;
+  c->timing_valid = !c->no_events && !c->defer_walk;
   return DP_OK;
 }

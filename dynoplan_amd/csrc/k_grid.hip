@@ -22,6 +22,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <vector>
 
 #include "dp_internal.h"
 
@@ -226,10 +228,20 @@ __global__ __launch_bounds__(256) void k_cell_list(int64_t ncells, dp_grid_geom 
   const int lane = threadIdx.x % 64;
   // one cell per thread (the grid covers every cell)
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int kind = 0;  // 0: empty, 1: listed, 2: settled here
+  int kind = 0;  // 0: empty or no list needed, 1: listed, 2: settled here
   if (c < ncells) {
     const int32_t b = cstart[c], e = cstart[c + 1];
-    if (e > b) kind = window > 0 || e - b != 1 || cell_block_count(G, cstart, c) != 1 ? 1 : 2;
+    if (e > b) kind = e - b != 1 || cell_block_count(G, cstart, c) != 1 ? 1 : 2;
+    if (kind == 1 && window > 0) {
+      // a cell whose contenders are all goal-forced or past their node's
+      // list window needs no wave; the latter are scanned when reached
+      bool any = false;
+      for (int32_t k = b; k < e && !any; ++k) {
+        const int8_t fl = flags[cidx[k]];
+        any = !(fl & 2) && !(fl & DP_F_NOLIST);
+      }
+      if (!any) kind = 0;
+    }
     if (kind == 2) {
       const int32_t q = cidx[b];
       const int8_t fl = flags[q];
@@ -336,11 +348,11 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
           for (int k = 0; k < XL; ++k) x_l[k] = cx[(int64_t)k * pcap + g0 + lane];
         }
         q_l = cidx[g0 + lane];
-        list_l = !(flags[q_l] & 2);  // goal-forced: NOVEL without a list
-        if (list_l && window > 0 && q_l - app[pnode[q_l]] >= window) {
-          conf_cnt[q_l] = -1;  // past the window: scanned when reached
-          list_l = false;
-        }
+        const int8_t fl = flags[q_l];
+        list_l = !(fl & 2);  // goal-forced: NOVEL without a list
+        // past the list window: scanned when reached (k_list_window set
+        // its negative list length)
+        if (window > 0 && (fl & DP_F_NOLIST)) list_l = false;
       }
       const uint64_t lists = __ballot(list_l);
       int cnt_l = 0;           // list length of this lane's contender so far
@@ -450,6 +462,50 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
         if (cnt_l == 0) flags[q_l] = (int8_t)(flags[q_l] | DP_F_NUC);
       }
     }
+  }
+}
+
+// List window (planar systems, limit > 0): the contenders of a node after
+// its first W in canonical order are flagged DP_F_NOLIST, and their list
+// length is set to -(1 + rank) (rank among the node's contenders, at most
+// 32766).  The walk reaches one of them only if more than W - limit
+// contenders before it were duplicates; it then records 1 + rank in
+// need[0], and later batches of the context use W = max(window,
+// need + need / 4 + 16) (a fixed window leaves need alone: need == null).
+// One wave per node.
+__global__ __launch_bounds__(64) void k_list_window(int32_t N, const int32_t* __restrict__ app,
+                                                    const int32_t* __restrict__ total,
+                                                    const int8_t* __restrict__ verdict, int8_t* __restrict__ flags,
+                                                    int16_t* __restrict__ conf_cnt, int32_t window,
+                                                    const int32_t* __restrict__ need) {
+  const int32_t node = blockIdx.x;
+  if (node >= N) return;
+  const int lane = threadIdx.x;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  if (need) {
+    const int32_t nd = need[0];
+    const int64_t w = (int64_t)nd + nd / 4 + 16;
+    if (nd > 0 && w > window) window = w < INT32_MAX ? (int32_t)w : INT32_MAX;
+  }
+  const int32_t tot = *total;
+  const int32_t b = app[node] < tot ? app[node] : tot;
+  const int32_t e = app[node + 1] < tot ? app[node + 1] : tot;
+  int32_t rank = 0;
+  for (int32_t cb = b; cb < e; cb += 64) {
+    const int32_t p = cb + lane;
+    bool con = false;
+    int8_t fl = 0;
+    if (p < e && verdict[p] == DP_V_FREE) {
+      fl = flags[p];
+      con = (fl & 2) || !(fl & 1);
+    }
+    const uint64_t m = __ballot(con);
+    const int32_t r = rank + __popcll(m & lt);
+    if (con && r >= window) {
+      flags[p] = (int8_t)(fl | DP_F_NOLIST);
+      conf_cnt[p] = (int16_t)(-1 - (r < 32766 ? r : 32766));
+    }
+    rank += __popcll(m);
   }
 }
 
@@ -578,6 +634,7 @@ struct round_args {
   long long* counters;
   long long* sticky;  // [DPS_COUNT] never reset
   int32_t* wshift;    // quadrotor FP64 window: [0] growth for later batches, [1] the growth this batch used
+  int32_t* lneed;     // planar list window: [0] largest 1 + rank of a listless contender a walk reached
 };
 
 // Pause between two polls of a contender's verdict: grows with the number of
@@ -786,6 +843,14 @@ __device__ bool advance_node(const round_args& a, int32_t node, int32_t& cur, in
         ext_block = !ext_dup && (undec != 0 || more);
       }
       __syncthreads();  // the arrays are rewritten by the next chunk
+    }
+    if (a.lneed && __ballot(cnt < 0)) {
+      // a contender past the list window was reached before the cap: later
+      // batches list at least this many contenders per node
+      int need = cnt < 0 && (fl & DP_F_NOLIST) ? -cnt : 0;
+#pragma unroll
+      for (int d = 32; d > 0; d >>= 1) need = max(need, __shfl_xor(need, d, 64));
+      if (lane == 0 && need > 0) atomicMax(a.lneed, need);
     }
     if (cnt < 0) eval(p, 0);
     unsigned long long t_walk = 0;
@@ -1055,12 +1120,7 @@ __global__ __launch_bounds__(64) void k_round(round_args a) {
 // a wave that exhausts it leaves its node to the round kernel and raises a
 // flag the host checks.
 template <int NX>
-__global__ __launch_bounds__(64) void k_resolve_ticketed(round_args a) {
-  __shared__ int32_t s_node;
-  if (threadIdx.x == 0) s_node = (int32_t)atomicAdd((unsigned long long*)&a.counters[DPC_TICKET], 1ull);
-  __syncthreads();
-  const int32_t node = s_node;
-  if (node >= a.N) return;
+__device__ void walk_node(const round_args& a, int32_t node) {
   if (a.trace && threadIdx.x == 0) {
     unsigned long long* t = a.trace + (int64_t)DP_TRACE_FIELDS * node;
     t[0] = (unsigned long long)wall_clock64();
@@ -1082,6 +1142,54 @@ __global__ __launch_bounds__(64) void k_resolve_ticketed(round_args a) {
   }
   if (threadIdx.x == 0 && spins) atomicAdd((unsigned long long*)&a.counters[DPC_ROUNDS], (unsigned long long)spins);
   if (a.trace && threadIdx.x == 0) a.trace[(int64_t)DP_TRACE_FIELDS * node + 1] = (unsigned long long)wall_clock64();
+}
+
+template <int NX>
+__global__ __launch_bounds__(64) void k_resolve_ticketed(round_args a) {
+  __shared__ int32_t s_node;
+  if (threadIdx.x == 0) s_node = (int32_t)atomicAdd((unsigned long long*)&a.counters[DPC_TICKET], 1ull);
+  __syncthreads();
+  const int32_t node = s_node;
+  if (node >= a.N) return;
+  walk_node<NX>(a, node);
+}
+
+// The ordered walks of many instances of one system in one launch.  Waves
+// claim (instance, node) items from one queue in ticket order; the host
+// interleaves the instances in short runs of nodes, so every instance's
+// chain of dependent nodes advances at the same time instead of one
+// instance after another.  Within an instance the tickets ascend with the
+// node, so the argument of k_resolve_ticketed holds per instance: a wave
+// only waits for nodes of its instance with smaller tickets, all claimed
+// earlier by resident or finished waves.  The wave that finishes an
+// instance's last node stamps the instance's completion time.
+template <int NX>
+__global__ __launch_bounds__(64) void k_resolve_queue(const round_args* __restrict__ args,
+                                                      const int2* __restrict__ items, int32_t n_items,
+                                                      unsigned long long* __restrict__ queue,
+                                                      int32_t* __restrict__ done,
+                                                      unsigned long long* __restrict__ stamps) {
+  __shared__ int32_t s_t;
+  if (threadIdx.x == 0) s_t = (int32_t)atomicAdd(queue, 1ull);
+  __syncthreads();
+  const int32_t t = __builtin_amdgcn_readfirstlane(s_t);
+  if (t >= n_items) return;
+  const int2 it = items[t];
+  const int32_t inst = __builtin_amdgcn_readfirstlane(it.x);
+  const int32_t node = __builtin_amdgcn_readfirstlane(it.y);
+  const round_args& a = args[inst];
+  walk_node<NX>(a, node);
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&done[inst], 1) + 1 == a.N) stamps[inst] = (unsigned long long)wall_clock64();
+  }
+}
+
+__global__ void k_queue_reset(unsigned long long* __restrict__ queues, int32_t nq, int32_t* __restrict__ done,
+                              int32_t n) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nq) queues[i] = 0ull;
+  if (i < n) done[i] = 0;
 }
 
 // Append the NOVEL final states of the batch to the visited set in canonical
@@ -1157,6 +1265,7 @@ static round_args make_round_args(dp_ctx* c) {
   a.counters = c->d_counters;
   a.sticky = c->d_sticky;
   a.wshift = c->d_wshift;
+  a.lneed = c->fp64_window > 0 ? nullptr : c->d_lneed;
   return a;
 }
 
@@ -1198,15 +1307,24 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
   c->round_limit = p->limit_branching_factor;
   c->round_r_dup = r_dup;
   c->round_params = *p;
-  // Every contender gets a stored list (window 0).  Quadrotor pairs past
-  // their node's FP64 window are not contenders (k_quad_window).
-  const int32_t window = 0;
+  // Planar systems with a cap: only the first DP_FP64_WINDOW(limit) FREE
+  // pairs of each node get a stored list (or the test limit).  Quadrotor
+  // pairs past their node's FP64 window are not contenders at all
+  // (k_quad_window), so every quadrotor contender gets a list.
+  const int32_t lim = p->limit_branching_factor;
+  const int32_t window = lim > 0 && c->desc.system != DP_SYS_QUADROTOR && !c->select_mode
+                             ? (c->fp64_window > 0 ? c->fp64_window : DP_FP64_WINDOW(lim))
+                             : 0;
+  const int32_t* lneed = c->fp64_window > 0 ? nullptr : c->d_lneed;
   const int pb = dp_grid_blocks(c->pair_cap, 256, 8192);
   const int64_t pool_cap = c->pool_limit > 0 ? std::min<int64_t>(c->pool_limit, c->conf_pool_cap) : c->conf_pool_cap;
   rc = dispatch_nx(c->desc.nx, [&](auto NXc) -> int {
     constexpr int NX = decltype(NXc)::value;
     if (p->limit_branching_factor == 0 && N > 0)
       k_precap_first<<<N, 64, 0, s>>>(N, c->d_app, c->d_total, c->d_verdict, c->d_flags, c->d_res);
+    if (window > 0 && N > 0)
+      k_list_window<<<N, 64, 0, s>>>(N, c->d_app, c->d_total, c->d_verdict, c->d_flags, c->d_conf_cnt, window,
+                                     lneed);
     dp_fill_i32(c->d_cont_start, nc + 1, 0, s);
     k_cont_count<NX><<<pb, 256, 0, s>>>(c->grid, c->d_total, c->d_verdict, c->d_flags, c->d_final, c->pair_cap,
                                         c->d_cont_start);
@@ -1228,6 +1346,7 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
     if (p->limit_branching_factor > 0)
       k_pre_nodes<<<N, 64, 0, s>>>(N, c->d_app, c->d_total, c->d_verdict, c->d_flags, c->d_res,
                                    p->limit_branching_factor);
+    if (c->defer_walk) return DP_OK;  // a multi-instance batch walks every instance in one queue
     round_args ra = make_round_args(c);
     ra.wait = DP_WAIT_POLLS;
     k_resolve_ticketed<NX><<<N, 64, 0, s>>>(ra);
@@ -1274,6 +1393,124 @@ int dp_launch_more_rounds(dp_ctx* c, int rounds, hipStream_t s) {
     return DP_OK;
   });
   if (rc) return rc;
+  DP_HIP_RET(hipGetLastError());
+  return DP_OK;
+}
+
+// ---- the ordered walks of a multi-instance batch as one work queue ----
+
+struct dp_walk_queue {
+  struct klass {
+    int nx;
+    int32_t first, count;  // its items: d_items[first, first + count)
+  };
+  int32_t n = 0;  // instances
+  round_args* d_args = nullptr;
+  int2* d_items = nullptr;
+  int64_t items_cap = 0;
+  unsigned long long* d_queues = nullptr;  // one ticket per class
+  int32_t* d_done = nullptr;               // finished nodes per instance
+  std::vector<round_args> h_args;
+  std::vector<int2> h_items;
+  std::vector<klass> classes;
+  bool uploaded = false;
+};
+
+namespace {
+constexpr int kWalkClasses = 4;  // nx 3, 4, 5, 13
+constexpr int32_t kWalkRun = 8;  // nodes of one instance per run of the interleave
+}  // namespace
+
+dp_walk_queue* dp_walk_queue_create(int32_t n) {
+  if (n <= 0) return nullptr;
+  dp_walk_queue* q = new dp_walk_queue;
+  q->n = n;
+  bool ok = hipMalloc(&q->d_args, sizeof(round_args) * n) == hipSuccess;
+  ok = ok && hipMalloc(&q->d_queues, sizeof(unsigned long long) * kWalkClasses) == hipSuccess;
+  ok = ok && hipMalloc(&q->d_done, sizeof(int32_t) * n) == hipSuccess;
+  if (!ok) {
+    dp_walk_queue_destroy(q);
+    return nullptr;
+  }
+  return q;
+}
+
+void dp_walk_queue_destroy(dp_walk_queue* q) {
+  if (!q) return;
+  if (q->d_args) (void)hipFree(q->d_args);
+  if (q->d_items) (void)hipFree(q->d_items);
+  if (q->d_queues) (void)hipFree(q->d_queues);
+  if (q->d_done) (void)hipFree(q->d_done);
+  delete q;
+}
+
+int dp_walk_queue_prepare(dp_walk_queue* q, dp_ctx* const* ctx, hipStream_t prev) {
+  const int32_t n = q->n;
+  std::vector<round_args> args(n);
+  for (int32_t i = 0; i < n; ++i) {
+    std::memset(&args[i], 0, sizeof(round_args));
+    args[i] = make_round_args(ctx[i]);
+    args[i].wait = DP_WAIT_POLLS;
+  }
+  std::vector<int2> items;
+  std::vector<dp_walk_queue::klass> classes;
+  for (int nx : {3, 4, 5, 13}) {
+    std::vector<int32_t> mine;
+    int32_t nmax = 0;
+    for (int32_t i = 0; i < n; ++i)
+      if (ctx[i]->desc.nx == nx) {
+        mine.push_back(i);
+        nmax = std::max(nmax, ctx[i]->N);
+      }
+    if (mine.empty()) continue;
+    dp_walk_queue::klass k{nx, (int32_t)items.size(), 0};
+    for (int32_t b = 0; b < nmax; b += kWalkRun)
+      for (int32_t i : mine)
+        for (int32_t v = b; v < std::min(b + kWalkRun, ctx[i]->N); ++v) items.push_back(make_int2(i, v));
+    k.count = (int32_t)items.size() - k.first;
+    classes.push_back(k);
+  }
+  const bool same = q->uploaded && items.size() == q->h_items.size() &&
+                    std::memcmp(items.data(), q->h_items.data(), sizeof(int2) * items.size()) == 0 &&
+                    std::memcmp(args.data(), q->h_args.data(), sizeof(round_args) * n) == 0;
+  if (same) return DP_OK;
+  // the previous batch may still read the arrays
+  if (prev) DP_HIP_RET(hipStreamSynchronize(prev));
+  if ((int64_t)items.size() > q->items_cap) {
+    if (q->d_items) DP_HIP_RET(hipFree(q->d_items));
+    q->d_items = nullptr;
+    q->items_cap = 0;
+    DP_HIP_RET(hipMalloc(&q->d_items, sizeof(int2) * items.size()));
+    q->items_cap = (int64_t)items.size();
+    dp_bump_epoch();
+  }
+  DP_HIP_RET(hipMemcpy(q->d_args, args.data(), sizeof(round_args) * n, hipMemcpyHostToDevice));
+  if (!items.empty())
+    DP_HIP_RET(hipMemcpy(q->d_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice));
+  q->h_args.swap(args);
+  q->h_items.swap(items);
+  q->classes.swap(classes);
+  q->uploaded = true;
+  return DP_OK;
+}
+
+int dp_walk_queue_launch(dp_walk_queue* q, unsigned long long* stamps, hipStream_t s) {
+  if (!q->uploaded) {
+    dp_set_error("walk queue launched before it was prepared");
+    return DP_ERR_STATE;
+  }
+  const int32_t m = std::max<int32_t>(q->n, kWalkClasses);
+  k_queue_reset<<<(m + 255) / 256, 256, 0, s>>>(q->d_queues, kWalkClasses, q->d_done, q->n);
+  for (size_t k = 0; k < q->classes.size(); ++k) {
+    const dp_walk_queue::klass& K = q->classes[k];
+    int rc = dispatch_nx(K.nx, [&](auto NXc) -> int {
+      constexpr int NX = decltype(NXc)::value;
+      k_resolve_queue<NX><<<K.count, 64, 0, s>>>(q->d_args, q->d_items + K.first, K.count, q->d_queues + k,
+                                                 q->d_done, stamps);
+      return DP_OK;
+    });
+    if (rc) return rc;
+  }
   DP_HIP_RET(hipGetLastError());
   return DP_OK;
 }

@@ -256,10 +256,12 @@ int dp_health(dp_ctx* ctx, dp_health_info* out);
 /* Test-only limits (no counterpart in the reference) that force the rare
  * paths of the resolution: inline earlier-conflict slots used per pair
  * (1..64; 0 restores 64), overflow pool entries (0: the full pool; a list
- * that does not fit is scanned from the grid when reached) and the quadrotor
- * FP64 window in FREE pairs (0: 3 * limit + 32, doubled for a context's
- * later batches each time one of its batches overflowed it, up to 2^10
- * times).  Results do not depend on them. */
+ * that does not fit is scanned from the grid when reached) and the window in
+ * FREE pairs per node: the quadrotor's FP64 window (0: 3 * limit + 32,
+ * doubled for a context's later batches each time one of its batches
+ * overflowed it, up to 2^10 times) and, for planar systems, the pairs that
+ * get stored conflict lists (0: 3 * limit + 32; a pair past it is scanned
+ * from the grid when reached).  Results do not depend on them. */
 int dp_debug_set_limits(dp_ctx* ctx, int32_t conf_slots, int64_t pool_entries, int32_t fp64_window);
 
 /* Copy the per-pair results of the last dp_expand into caller buffers, in

@@ -79,6 +79,27 @@ def test_unicycle1_rare_list_paths(need_gpu):
         assert_parity(gpu, ora, f"slots={slots} pool={pool}")
 
 
+@pytest.mark.parametrize("name,window", [("unicycle1", 1), ("unicycle1", 6), ("car", 3), ("unicycle2", 0)])
+def test_list_window(need_gpu, name, window):
+    """Stored conflict lists only for the first `window` FREE pairs of each
+    node (0: the default 3 * limit + 32): with tiny windows most walked pairs
+    lie past it and are resolved by scanning the contender grid.  Results
+    are the oracle's either way."""
+    wl = {"unicycle1": lambda: synth.unicycle1_small(seed=15, N_nodes=256, n_visited=2000),
+          "car": lambda: synth.car_trailer(seed=16, P=800, N_nodes=200, n_boxes=60, n_visited=3000, extent=10.0),
+          "unicycle2": lambda: synth.unicycle2_small(seed=17, N_nodes=256)}[name]()
+    p = synth.make_params(goal=wl.params.goal[:wl.desc.nx], delta=0.6, limit_branching_factor=8)
+    ora, co = O.expand(wl, params=p)
+    with expander_for(wl) as ex:
+        ex.debug_set_limits(fp64_window=window)
+        cg = ex.expand(wl.nodes, wl.g_parent, p)
+        gpu = ex.fetch()
+    print(name, "window", window, "counts", cg)
+    assert co["dup_batch"] > 0 and co["capped"] > 0
+    assert_parity(gpu, ora, f"{name} list window {window}")
+    counts_consistent(cg)
+
+
 def test_unicycle1_goal_hits(need_gpu):
     """Goal placed on a frontier node's neighbourhood: intermediate-goal
     checks fire and forced-novel pairs appear."""
