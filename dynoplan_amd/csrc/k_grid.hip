@@ -183,6 +183,18 @@ __device__ __forceinline__ void scan_earlier(const dp_dev_sys& S, const dp_grid_
 // pool ran out), for which the resolution scans the grid.  The order of a
 // list's entries carries no meaning.
 
+// Contenders of one cell that one wave of k_conf_waves takes.  A wave's
+// serial work is its contenders times the candidate batches of the block
+// (K / batch), so the contenders of a crowded block are split over several
+// waves, about 32 contender-batches each, in at most 255 parts.
+__device__ __forceinline__ int32_t cell_part_size(int32_t m, int32_t K, int32_t batch) {
+  const int32_t nb = (K + batch - 1) / batch;
+  int32_t per = nb > 0 ? 32 / nb : 64;
+  per = per < 1 ? 1 : (per > 64 ? 64 : per);
+  const int32_t least = (m + 254) / 255;
+  return per < least ? least : per;
+}
+
 // Contenders in the 27-cell block around `cell` (the runs k_conf_waves
 // reads, with the same neighbour rules).
 __device__ __forceinline__ int32_t cell_block_count(const dp_grid_geom& G, const int32_t* __restrict__ cstart,
@@ -220,7 +232,7 @@ __device__ __forceinline__ int32_t cell_block_count(const dp_grid_geom& G, const
 // contender can conflict: an empty list, and NUC unless goal-forced) and is
 // not listed, when no window applies.
 __global__ __launch_bounds__(256) void k_cell_list(int64_t ncells, dp_grid_geom G, const int32_t* __restrict__ cstart,
-                                                   const int32_t* __restrict__ cidx, int32_t window,
+                                                   const int32_t* __restrict__ cidx, int32_t window, int32_t batch,
                                                    int8_t* __restrict__ flags, int16_t* __restrict__ conf_cnt,
                                                    int32_t* __restrict__ list, long long* __restrict__ counters) {
   __shared__ int32_t s_cnt;
@@ -229,9 +241,11 @@ __global__ __launch_bounds__(256) void k_cell_list(int64_t ncells, dp_grid_geom 
   // one cell per thread (the grid covers every cell)
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int kind = 0;  // 0: empty or no list needed, 1: listed, 2: settled here
+  int32_t parts = 0;
   if (c < ncells) {
     const int32_t b = cstart[c], e = cstart[c + 1];
-    if (e > b) kind = e - b != 1 || cell_block_count(G, cstart, c) != 1 ? 1 : 2;
+    const int32_t K = e > b ? cell_block_count(G, cstart, c) : 0;
+    if (e > b) kind = e - b != 1 || K != 1 ? 1 : 2;
     if (kind == 1 && window > 0) {
       // a cell whose contenders are all goal-forced or past their node's
       // list window needs no wave; the latter are scanned when reached
@@ -241,6 +255,10 @@ __global__ __launch_bounds__(256) void k_cell_list(int64_t ncells, dp_grid_geom 
         any = !(fl & 2) && !(fl & DP_F_NOLIST);
       }
       if (!any) kind = 0;
+    }
+    if (kind == 1) {
+      const int32_t per = cell_part_size(e - b, K, batch);
+      parts = (e - b + per - 1) / per;
     }
     if (kind == 2) {
       const int32_t q = cidx[b];
@@ -253,15 +271,23 @@ __global__ __launch_bounds__(256) void k_cell_list(int64_t ncells, dp_grid_geom 
   }
   if (threadIdx.x == 0) s_cnt = 0;
   __syncthreads();
-  const uint64_t m = __ballot(kind == 1);
+  // items: cell | part << 24 (a grid has at most 2^24 cells)
+  int incl = parts;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int v = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += v;
+  }
+  const int wtot = __shfl(incl, 63, 64);
   int woff = 0;
-  if (lane == 0 && m) woff = atomicAdd(&s_cnt, __popcll(m));
+  if (lane == 0 && wtot) woff = atomicAdd(&s_cnt, wtot);
   __syncthreads();
   if (threadIdx.x == 0)
     s_base = s_cnt ? (long long)atomicAdd((unsigned long long*)&counters[DPC_CELLS], (unsigned long long)s_cnt) : 0;
   __syncthreads();
   woff = __shfl(woff, 0, 64);
-  if (kind == 1) list[s_base + woff + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)c;
+  for (int32_t k = 0; k < parts; ++k)
+    list[s_base + woff + (incl - parts) + k] = (int32_t)((uint32_t)c | ((uint32_t)k << 24));
 }
 
 // One wave per non-empty grid cell: the 27 cells around it
@@ -287,10 +313,17 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
   const uint64_t ltmask = (1ull << lane) - 1ull;
   const long long ncl = counters[DPC_CELLS];
   const double T = dp_pos_bound(S, r);
+  // quadrotor: |q.q'| below cos(r / w1) - 1e-9 puts the attitude term alone
+  // above r (acos is decreasing with slope <= -1, and the margin dwarfs the
+  // series' error), so the pair is rejected without the acos
+  const double cq = NX == 13 && S.w[1] > 0.0 && r / S.w[1] < DP_HALF_PI ? cos(r / S.w[1]) - 1e-9 : -1.0;
   const int64_t d01 = (int64_t)G.dims[0] * G.dims[1];
   const long long nwaves = (long long)gridDim.x * (blockDim.x / 64);
+  constexpr int kU = NX == 13 ? 2 : 4;  // candidates per lane in flight
   for (long long w = (long long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; w < ncl; w += nwaves) {
-    const int64_t cell = cells[w];
+    const uint32_t item = (uint32_t)cells[w];
+    const int64_t cell = item & 0xFFFFFFu;
+    const int32_t part = (int32_t)(item >> 24);
     const int32_t c0 = (int32_t)(cell % G.dims[0]), c1 = (int32_t)((cell / G.dims[0]) % G.dims[1]);
     const int32_t c2 = (int32_t)(cell / d01);
     // run k (lanes 0..8): rows c1 + k%3 - 1, layers c2 + k/3 - 1 (headings, or z for the
@@ -330,23 +363,21 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
       rp[k] = __builtin_amdgcn_readlane(incl - n, k);
     }
     const int K = __builtin_amdgcn_readlane(incl, 8);
-    const int32_t b = cstart[cell], e = cstart[cell + 1];
+    const int32_t b0 = cstart[cell], e0 = cstart[cell + 1];
+    const int32_t per = cell_part_size(e0 - b0, K, 64 * kU);
+    const int32_t b = b0 + part * per, e = e0 < b + per ? e0 : b + per;
     // the cell's contenders, 64 at a time, one per lane
     for (int32_t g0 = b; g0 < e; g0 += 64) {
       const int m = e - g0 < 64 ? e - g0 : 64;
       int32_t q_l = INT32_MAX;
       bool list_l = false;
-      // planar systems hold the group's states one per lane (broadcast by
-      // readlane below); the quadrotor's 13 components are read uniformly
-      constexpr int XL = NX == 13 ? 1 : NX;
-      double x_l[XL];
+      // the group's states, one per lane (broadcast by readlane below)
+      double x_l[NX];
 #pragma unroll
-      for (int k = 0; k < XL; ++k) x_l[k] = 0.0;
+      for (int k = 0; k < NX; ++k) x_l[k] = 0.0;
       if (lane < m) {
-        if constexpr (NX != 13) {
 #pragma unroll
-          for (int k = 0; k < XL; ++k) x_l[k] = cx[(int64_t)k * pcap + g0 + lane];
-        }
+        for (int k = 0; k < NX; ++k) x_l[k] = cx[(int64_t)k * pcap + g0 + lane];
         q_l = cidx[g0 + lane];
         const int8_t fl = flags[q_l];
         list_l = !(fl & 2);  // goal-forced: NOVEL without a list
@@ -360,14 +391,13 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
       // Candidates in order, kU per lane in flight, each batch tested
       // against every contender of the group.  Entries at list position
       // < DP_CONF_SLOTS go inline; the pool pass writes the others.
-      constexpr int kU = NX == 13 ? 2 : 4;
       auto pass = [&](bool pool_pass, uint64_t who) {
         for (int j0 = 0; j0 < K; j0 += 64 * kU) {
-          // candidate index, position and (planar) heading; the other
-          // components are read only for candidates that pass the rejections
+          // candidate index and state, every component loaded together with
+          // the batch (the contender loop below then runs without loads)
           int32_t o[kU];
           int64_t tt[kU];
-          double y[kU][3];
+          double y[kU][NX];
 #pragma unroll
           for (int u = 0; u < kU; ++u) {
             const int j = j0 + 64 * u + lane;
@@ -384,24 +414,19 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
               tt[u] = (int64_t)sk + (j - pk);
               o[u] = cidx[tt[u]];
 #pragma unroll
-              for (int d = 0; d < 3; ++d) y[u][d] = cx[(int64_t)d * pcap + tt[u]];
+              for (int d = 0; d < NX; ++d) y[u][d] = cx[(int64_t)d * pcap + tt[u]];
             }
           }
           for (uint64_t rem = who; rem; rem &= rem - 1ull) {
             const int c = __ffsll((long long)rem) - 1;
             const int32_t q = __builtin_amdgcn_readlane(q_l, c);
             double x[NX];
-            if constexpr (NX == 13) {
 #pragma unroll
-              for (int k = 0; k < NX; ++k) x[k] = cx[(int64_t)k * pcap + g0 + c];
-            } else {
-#pragma unroll
-              for (int k = 0; k < NX; ++k) {
-                const uint64_t bits = (uint64_t)__double_as_longlong(x_l[k < XL ? k : 0]);
-                const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)bits, c);
-                const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(bits >> 32), c);
-                x[k] = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-              }
+            for (int k = 0; k < NX; ++k) {
+              const uint64_t bits = (uint64_t)__double_as_longlong(x_l[k]);
+              const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)bits, c);
+              const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(bits >> 32), c);
+              x[k] = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
             }
             int cc = __builtin_amdgcn_readlane(cnt_l, c);
             const long long off = pool_pass ? (long long)__builtin_amdgcn_readlane((int)off_l, c) : 0;
@@ -421,13 +446,14 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
                   pass = !(s2 > T) && !(S.w[1] * dp_so2(x[2], y[u][2]) > r);
                 }
                 if (pass) {
-                  double yf[NX];
-#pragma unroll
-                  for (int d = 0; d < 3; ++d) yf[d] = y[u][d];
-#pragma unroll
-                  for (int d = 3; d < NX; ++d) yf[d] = cx[(int64_t)d * pcap + tt[u]];
-                  if constexpr (NX == 13) hit = !(dp_pos_term(S, x, yf) > r) && dp_within(S, x, yf, r);
-                  else hit = dp_within_planar_fast<NX>(S, x, yf, r, T);
+                  if constexpr (NX == 13) {
+                    double dot = x[3] * y[u][3];
+                    dot = dot + x[4] * y[u][4];
+                    dot = dot + x[5] * y[u][5];
+                    dot = dot + x[6] * y[u][6];
+                    hit = !(fabs(dot) < cq) && !(dp_pos_term(S, x, y[u]) > r) && dp_within(S, x, y[u], r);
+                  }
+                  else hit = dp_within_planar_fast<NX>(S, x, y[u], r, T);
                 }
               }
               const uint64_t mk = __ballot(hit);
@@ -1301,8 +1327,10 @@ int dp_launch_visited_rebuild(dp_ctx* c, double radius, hipStream_t s) {
 int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStream_t s) {
   const int64_t nc = c->grid.ncells;
   const int32_t N = c->N;
+  // the scatter cursors, then the conflict-wave items: at most one per
+  // contender
   int rc = dp_ensure_i32(&c->d_cont_start, &c->cont_start_cap, nc + 1);
-  if (!rc) rc = dp_ensure_i32(&c->d_cont_cursor, &c->cont_cursor_cap, nc + 1);
+  if (!rc) rc = dp_ensure_i32(&c->d_cont_cursor, &c->cont_cursor_cap, std::max<int64_t>(nc + 1, c->pair_cap));
   if (rc) return rc;
   c->round_limit = p->limit_branching_factor;
   c->round_r_dup = r_dup;
@@ -1335,8 +1363,8 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
     // earlier-conflict lists; the scatter cursors are free again and hold
     // the list of non-empty cells
     k_cell_list<<<(int)((nc + 255) / 256), 256, 0, s>>>(nc, c->grid, c->d_cont_start, c->d_cont_idx, window,
-                                                                c->d_flags, c->d_conf_cnt, c->d_cont_cursor,
-                                                                c->d_counters);
+                                                                64 * (NX == 13 ? 2 : 4), c->d_flags, c->d_conf_cnt,
+                                                                c->d_cont_cursor, c->d_counters);
     k_conf_waves<NX><<<(int)std::min<int64_t>((nc + 3) / 4, 8192), 256, 0, s>>>(
         c->d_sys, c->grid, c->d_flags, c->d_pnode, c->d_app, window, c->pair_cap, c->d_cont_start, c->d_cont_idx, c->d_cont_x, c->d_cont_cursor,
         r_dup, c->d_conf_cnt, c->d_conf, c->d_conf_ov, c->d_conf_pool, pool_cap, c->conf_slots, c->d_counters);
