@@ -186,10 +186,10 @@ __device__ __forceinline__ void scan_earlier(const dp_dev_sys& S, const dp_grid_
 // Contenders of one cell that one wave of k_conf_waves takes.  A wave's
 // serial work is its contenders times the candidate batches of the block
 // (K / batch), so the contenders of a crowded block are split over several
-// waves, about 32 contender-batches each, in at most 255 parts.
+// waves, about 16 contender-batches each, in at most 255 parts.
 __device__ __forceinline__ int32_t cell_part_size(int32_t m, int32_t K, int32_t batch) {
   const int32_t nb = (K + batch - 1) / batch;
-  int32_t per = nb > 0 ? 32 / nb : 64;
+  int32_t per = nb > 0 ? 16 / nb : 64;
   per = per < 1 ? 1 : (per > 64 ? 64 : per);
   const int32_t least = (m + 254) / 255;
   return per < least ? least : per;

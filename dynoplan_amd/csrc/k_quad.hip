@@ -109,8 +109,7 @@ constexpr int kQWaves = 4;
 // obstacles listed per lane for the FP64 re-check (int16 in LDS)
 constexpr int kQ64Cand = 128;
 
-__device__ __forceinline__ bool collides64(const dp_dev_sys& S, const double* obs, int nb, int ns, const double* s) {
-  const double r = S.radius;
+__device__ __forceinline__ bool collides64(double r, const double* obs, int nb, int ns, const double* s) {
   for (int o = 0; o < nb; ++o)
     if (dp_sphere_box(s[0], s[1], s[2], r, obs + 6 * o)) return true;
   const double* sp = obs + 6 * nb;
@@ -493,14 +492,27 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
           near_bound_any = rnear <= rbad;
         }
       }
-      // the velocity increments of the next four steps are in flight while
-      // a state is checked: each step waits for a load issued four steps
-      // earlier instead of one (the loop is bound by load latency)
+      // The velocity increments of the next four steps are in flight while
+      // a state is checked, in four registers that the loop below (unrolled
+      // by four) names statically: the increment of step k sits in slot
+      // (k - 1) % 4 and is replaced by that of step k + 4 right after use,
+      // so each step waits for a load issued four steps earlier.  Indices
+      // are clamped to the primitive's last state, so every load is valid
+      // and unconditional (a conditional load would make the wait for it
+      // exact only at the loop's end).
       const float4* dvp = reinterpret_cast<const float4*>(a.dv32) + s0;
-      const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
       const int32_t Lr = fast ? 0 : L1;
-      float4 d1 = 1 < Lr ? dvp[1] : z4, d2 = 2 < Lr ? dvp[2] : z4, d3 = 3 < Lr ? dvp[3] : z4,
-             d4 = 4 < Lr ? dvp[4] : z4;
+      const int32_t Lm = L1 - 1;
+      // (issued in the loop's own order, slot 3 first, so that the wait
+      // for each slot is the same on entry and around the loop)
+      float4 dq[4];
+      dq[3] = dvp[4 < Lm ? 4 : Lm];
+#pragma unroll
+      for (int u = 0; u < 3; ++u) {
+        __builtin_amdgcn_sched_barrier(0);  // keep the issue order
+        dq[u] = dvp[1 + u < Lm ? 1 + u : Lm];
+      }
+      __builtin_amdgcn_sched_barrier(0);
       // Long pairs: the staged obstacles that can come within r + 2 near_eps
       // of this pair's own states (its position box: the primitive's offset
       // box plus k dt v0 for k up to L1 - 1, widened by 1e-3), one bit each;
@@ -536,19 +548,23 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
           if (hit) smask |= 1ull << t;
         }
       }
-      for (int k = 0; k < (fast ? 0 : L1); ++k) {
-        if (k > 0) {
-          const float dv[3] = {d1.x, d1.y, d1.z};
+      bool stop = false;
+      for (int k0 = 0; k0 < Lr && !stop; k0 += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + u;
+        const int slot = (u + 3) & 3;
+        const bool live = !stop && k < Lr;
+        if (live && k > 0) {
+          const float dv[3] = {dq[slot].x, dq[slot].y, dq[slot].z};
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
             s[c] = s[c] + K.dt * s[3 + c];
             s[3 + c] = s[3 + c] + dv[c];
           }
-          d1 = d2;
-          d2 = d3;
-          d3 = d4;
-          d4 = k + 4 < L1 ? dvp[k + 4] : z4;
         }
+        dq[slot] = dvp[k + 4 < Lm ? k + 4 : Lm];
+        if (!live) continue;
         // bounds with margin (attitude and rate: the primitive's first
         // offending / near state)
         bool bad = k >= rbad, nb_ = k >= rnear;
@@ -563,7 +579,8 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
         near_bound_any = near_bound_any || nb_;
         if (bad) {
           first_bad = k;
-          break;  // later states cannot change a bounds verdict
+          stop = true;  // later states cannot change a bounds verdict
+          continue;
         }
         if (first_col == INT_MAX && !short_chunk) {
           bool col = false, nc = false;
@@ -597,6 +614,7 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
           if (nc) near_col_before = true;
           if (col) first_col = k;
         }
+      }
       }
       int8_t vd;
       int32_t fk;
@@ -681,13 +699,15 @@ __device__ __forceinline__ void quad64_rot(const dp_quad_args& a, int64_t st, do
   for (int c = 0; c < 3; ++c) s[10 + c] = a.rot64[(int64_t)(4 + c) * a.S + st];
 }
 
-// Bounds of the position and velocity components (is_state_valid on them).
-__device__ __forceinline__ bool trans_valid64(const dp_dev_sys& S, const double* s) {
+// Bounds of the position and velocity components (is_state_valid on them),
+// against lo[6] / hi[6] (position then velocity) held in registers; every
+// comparison is evaluated (no short circuit, so no branch per bound).
+__device__ __forceinline__ bool trans_valid64(const double* lo, const double* hi, const double* s) {
   bool ok = true;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    ok = ok && (s[c] >= S.lo[c] && s[c] <= S.hi[c]);
-    ok = ok && (s[7 + c] >= S.lo[7 + c] && s[7 + c] <= S.hi[7 + c]);
+    ok = ok & (s[c] >= lo[c]) & (s[c] <= hi[c]);
+    ok = ok & (s[7 + c] >= lo[3 + c]) & (s[7 + c] <= hi[3 + c]);
   }
   return ok;
 }
@@ -707,6 +727,17 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
   // per-lane candidate lists [blockDim.x][kQ64Cand]
   extern __shared__ double q64_lds[];
   const dp_dev_sys& S = *a.sys;
+  // read once into registers: the rollout loops store to global memory,
+  // after which a field of *a.sys would be read again on every step
+  const double dt = S.dt, rad = S.radius, w0 = S.w[0];
+  double blo[6], bhi[6];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    blo[c] = S.lo[c];
+    bhi[c] = S.hi[c];
+    blo[3 + c] = S.lo[7 + c];
+    bhi[3 + c] = S.hi[7 + c];
+  }
   const int64_t n64 = quad64_count(a);
   if ((int64_t)blockIdx.x * blockDim.x >= n64) return;  // uniform per block
   const int nobs_d = 6 * a.nb + 4 * a.ns;
@@ -731,39 +762,61 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
     int first_bad = INT_MAX, first_col = INT_MAX;
     const int32_t rbad = a.rbad64[j];
     int ncand = 0;  // obstacles listed for the second rollout (-1: more than kQ64Cand, test all)
+    // The velocity increments of the next four steps are in flight while a
+    // state is processed, in four statically named slots of loops unrolled
+    // by four (step k's increment in slot (k - 1) % 4, replaced by step
+    // k + 4's right after use); indices clamp to the last state, so every
+    // load is valid and unconditional, and the slots are first loaded in
+    // the loops' own order (3, 0, 1, 2): each step then waits for a load
+    // issued four steps earlier.
     const double4* dvp = reinterpret_cast<const double4*>(a.dv64) + s0;
+    const int32_t Lm = L1 - 1;
+    auto ld = [&](int idx) {
+      const double4 t = dvp[idx < Lm ? idx : Lm];
+      return make_double3(t.x, t.y, t.z);
+    };
+    auto advance = [&](const double3& d) {
+      const double dv[3] = {d.x, d.y, d.z};
+      double o[13];
+      dp_quad_trans_step64(dt, s, dv, o);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        s[c] = o[c];
+        s[7 + c] = o[7 + c];
+      }
+    };
+    double3 dq[4];
     if (full) {
       // first rollout: bounds of every state and the box of the positions
       quad64_start(S, a, i, j, s);
       double bl[3] = {s[0], s[1], s[2]}, bh[3] = {s[0], s[1], s[2]};
-      double3 d1 = make_double3(0.0, 0.0, 0.0), d2 = d1, d3 = d1, d4 = d1;
-      if (1 < L1) d1 = make_double3(dvp[1].x, dvp[1].y, dvp[1].z);
-      if (2 < L1) d2 = make_double3(dvp[2].x, dvp[2].y, dvp[2].z);
-      if (3 < L1) d3 = make_double3(dvp[3].x, dvp[3].y, dvp[3].z);
-      if (4 < L1) d4 = make_double3(dvp[4].x, dvp[4].y, dvp[4].z);
-      for (int k = 0; k < L1; ++k) {
-        if (k > 0) {
-          const double dv[3] = {d1.x, d1.y, d1.z};
-          double o[13];
-          dp_quad_trans_step64(S.dt, s, dv, o);
+      dq[3] = ld(4);
+#pragma unroll
+      for (int u = 0; u < 3; ++u) {
+        __builtin_amdgcn_sched_barrier(0);  // keep the issue order
+        dq[u] = ld(1 + u);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      bool stop = false;
+      for (int k0 = 0; k0 < L1 && !stop; k0 += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int k = k0 + u;
+          const int slot = (u + 3) & 3;
+          const bool live = !stop && k < L1;
+          if (live && k > 0) advance(dq[slot]);
+          dq[slot] = ld(k + 4);
+          if (!live) continue;
+          if (k >= rbad || !trans_valid64(blo, bhi, s)) {
+            first_bad = k;
+            stop = true;
+            continue;
+          }
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            s[c] = o[c];
-            s[7 + c] = o[7 + c];
+            bl[c] = fmin(bl[c], s[c]);
+            bh[c] = fmax(bh[c], s[c]);
           }
-          d1 = d2;
-          d2 = d3;
-          d3 = d4;
-          if (k + 4 < L1) d4 = make_double3(dvp[k + 4].x, dvp[k + 4].y, dvp[k + 4].z);
-        }
-        if (k >= rbad || !trans_valid64(S, s)) {
-          first_bad = k;
-          break;
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          bl[c] = fmin(bl[c], s[c]);
-          bh[c] = fmax(bh[c], s[c]);
         }
       }
       if (first_bad == INT_MAX) {
@@ -804,54 +857,56 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
 #pragma unroll
     for (int g = 0; g < 8; ++g) gi[g] = g < nng ? a.goalidx[(int64_t)g * a.P + j] : -1;
     int32_t gnext = nng > 0 ? gi[0] : -1;
-    double3 d1 = make_double3(0.0, 0.0, 0.0), d2 = d1, d3 = d1, d4 = d1;
-    if (1 < L1) d1 = make_double3(dvp[1].x, dvp[1].y, dvp[1].z);
-    if (2 < L1) d2 = make_double3(dvp[2].x, dvp[2].y, dvp[2].z);
-    if (3 < L1) d3 = make_double3(dvp[3].x, dvp[3].y, dvp[3].z);
-    if (4 < L1) d4 = make_double3(dvp[4].x, dvp[4].y, dvp[4].z);
-    for (int k = 0; k < L1 && first_bad == INT_MAX; ++k) {
-      if (k > 0) {
-        const double dv[3] = {d1.x, d1.y, d1.z};
-        double o[13];
-        dp_quad_trans_step64(S.dt, s, dv, o);
+    dq[3] = ld(4);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          s[c] = o[c];
-          s[7 + c] = o[7 + c];
-        }
-        d1 = d2;
-        d2 = d3;
-        d3 = d4;
-        if (k + 4 < L1) d4 = make_double3(dvp[k + 4].x, dvp[k + 4].y, dvp[k + 4].z);
-      }
+    for (int u = 0; u < 3; ++u) {
+      __builtin_amdgcn_sched_barrier(0);  // keep the issue order
+      dq[u] = ld(1 + u);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const int32_t L2 = first_bad == INT_MAX ? L1 : 0;
+    for (int k0 = 0; k0 < L2; k0 += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + u;
+      const int slot = (u + 3) & 3;
+      const bool live = k < L2;
+      if (live && k > 0) advance(dq[slot]);
+      dq[slot] = ld(k + 4);
+      if (!live) continue;
       if (full && first_col == INT_MAX) {
         bool col;
         if (ncand < 0) {
-          col = collides64(S, obs, a.nb, a.ns, s);
+          col = collides64(rad, obs, a.nb, a.ns, s);
         } else {
           col = false;
           for (int e = 0; e < ncand && !col; ++e) {
             const int o = cand[e];
-            col = o < a.nb ? dp_sphere_box(s[0], s[1], s[2], S.radius, obs + 6 * o)
-                           : dp_sphere_sphere(s[0], s[1], s[2], S.radius, obs + 6 * a.nb + 4 * (o - a.nb));
+            col = o < a.nb ? dp_sphere_box(s[0], s[1], s[2], rad, obs + 6 * o)
+                           : dp_sphere_sphere(s[0], s[1], s[2], rad, obs + 6 * a.nb + 4 * (o - a.nb));
           }
         }
         if (col) first_col = k;
       }
       // check_goal at the float-rule indices, in nn order (dbastar.cpp:492-512)
       while (chosen == -1 && nn < nng && gnext == k) {
-        quad64_rot(a, s0 + k, s);
-        if (dp_within(S, s, a.goal, a.goal_bound)) {
-          chosen = k;
-          // the final state is the goal state: stored now, read back below
+        // dp_within's own first exit, the position term above the bound,
+        // taken before the attitude is loaded: no load in the common case
+        if (!(w0 * dp_l2_3(s, a.goal) > a.goal_bound)) {
+          quad64_rot(a, s0 + k, s);
+          if (dp_within(S, s, a.goal, a.goal_bound)) {
+            chosen = k;
+            // the final state is the goal state: stored now, read back below
 #pragma unroll
-          for (int c = 0; c < 13; ++c) a.fin[(int64_t)c * a.pcap + q] = s[c];
+            for (int c = 0; c < 13; ++c) a.fin[(int64_t)c * a.pcap + q] = s[c];
+          }
         }
         ++nn;
 #pragma unroll
         for (int g = 1; g < 8; ++g)
           if (nn == g) gnext = gi[g];
       }
+    }
     }
     if (first_bad == INT_MAX) quad64_rot(a, s0 + L1 - 1, s);  // the full final state
     int8_t vd = DP_V_FREE;
@@ -908,6 +963,17 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
 // rollout so that neither kernel's registers limit the other's occupancy.
 __global__ __launch_bounds__(256) void k_quad_dupv(dp_quad_args a) {
   const dp_dev_sys& S = *a.sys;
+  // read once into registers: the rollout loops store to global memory,
+  // after which a field of *a.sys would be read again on every step
+  const double dt = S.dt, rad = S.radius, w0 = S.w[0];
+  double blo[6], bhi[6];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    blo[c] = S.lo[c];
+    bhi[c] = S.hi[c];
+    blo[3 + c] = S.lo[7 + c];
+    bhi[3 + c] = S.hi[7 + c];
+  }
   const int64_t n64 = quad64_count(a);
   const double T = dp_pos_bound(S, a.r_dup);
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n64; t += (int64_t)gridDim.x * blockDim.x) {
