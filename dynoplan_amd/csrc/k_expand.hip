@@ -429,12 +429,18 @@ template <int NX, int LPP>
 __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
   static_assert(LPP >= 2 && 64 % LPP == 0, "a pair takes 2 to 64 lanes of a wave");
   extern __shared__ double smem[];
-  const dp_dev_sys& S = *a.sys;
+  // the system constants in LDS: read from global memory after the loop's
+  // stores, each would be a load the wave waits for
+  __shared__ dp_dev_sys s_sys;
+  const dp_dev_sys& S = s_sys;
   const int nobs = a.nobs;
   if ((int64_t)blockIdx.x * kWaves * (64 / LPP) >= (int64_t)*a.total) return;  // uniform per block
   double* s_obs = smem;  // [nobs][4]
   int16_t* s_cand = (int16_t*)(smem + 4 * nobs);
   for (int t = threadIdx.x; t < 4 * nobs; t += blockDim.x) s_obs[t] = a.obs[t];
+  static_assert(sizeof(dp_dev_sys) % sizeof(double) == 0, "copied as doubles");
+  for (int t = threadIdx.x; t < (int)(sizeof(dp_dev_sys) / sizeof(double)); t += blockDim.x)
+    reinterpret_cast<double*>(&s_sys)[t] = reinterpret_cast<const double*>(a.sys)[t];
   __syncthreads();
 
   constexpr int SEG = 64 / LPP;
@@ -447,6 +453,15 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
   const int32_t total = *a.total;
   const double Tdup = dp_pos_bound(S, a.r_dup);
   long long c_b = 0, c_c = 0, c_f = 0, c_k = 0, c_x = 0;
+  // bounds and footprint held in registers: read inside the loops below
+  // (after the loop's stores) every bound would be a load and a wait
+  double blo[NX], bhi[NX];
+#pragma unroll
+  for (int c = 0; c < NX; ++c) {
+    blo[c] = S.lo[c];
+    bhi[c] = S.hi[c];
+  }
+  const double body_hl = S.body_hl, body_hw = S.body_hw, trailer_hl = S.trailer_hl, trailer_hw = S.trailer_hw;
 
   for (int64_t q0 = ((int64_t)blockIdx.x * kWaves + wave) * SEG; q0 < total;
        q0 += (int64_t)gridDim.x * kWaves * SEG) {
@@ -479,7 +494,7 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
         const double o = c == 0 ? ox : (c == 1 ? oy : 0.0);
         const double mn = a.srange[(int64_t)c * a.P + j], mx = a.srange[((int64_t)NX + c) * a.P + j];
         const double tmn = c < 2 ? mn + o : mn, tmx = c < 2 ? mx + o : mx;
-        clear = clear && tmn >= S.lo[c] && tmx <= S.hi[c];
+        clear = clear & (tmn >= blo[c]) & (tmx <= bhi[c]);
       }
     }
     int first_bad = INT_MAX;
@@ -491,7 +506,7 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
         double s[NX];
         load_translated<NX>(a, sidx, ox, oy, s);
 #pragma unroll
-        for (int c = 0; c < NX; ++c) bad = bad || !(s[c] >= S.lo[c] && s[c] <= S.hi[c]);
+        for (int c = 0; c < NX; ++c) bad = bad | !((s[c] >= blo[c]) & (s[c] <= bhi[c]));
       }
       const uint64_t m = (__ballot(bad) >> shift) & segbits;
       if (first_bad == INT_MAX && m) first_bad = kb + __ffsll((long long)m) - 1;
@@ -542,12 +557,12 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
           for (int bi = 0; bi < a.nbox && !col; ++bi) {
             const double c = a.geo[(int64_t)(4 * bi + 2) * a.S + sidx];
             const double s = a.geo[(int64_t)(4 * bi + 3) * a.S + sidx];
-            double cx = x, cy = y, hl = S.body_hl, hw = S.body_hw;
+            double cx = x, cy = y, hl = body_hl, hw = body_hw;
             if (bi > 0) {
               cx = x + a.geo[(int64_t)(4 * bi) * a.S + sidx];
               cy = y + a.geo[(int64_t)(4 * bi + 1) * a.S + sidx];
-              hl = S.trailer_hl;
-              hw = S.trailer_hw;
+              hl = trailer_hl;
+              hw = trailer_hw;
             }
             for (int t = 0; t < ncand; ++t) {
               const double* b = s_obs + 4 * cand[t];
@@ -989,7 +1004,7 @@ int dp_launch_expand(dp_ctx* c, const dp_expand_params* p, hipStream_t s) {
   auto lds_of = [&](int l) {
     return sizeof(double) * 4 * c->nb + sizeof(int16_t) * (size_t)kWaves * (64 / l) * (c->nb > 0 ? c->nb : 1);
   };
-  while (lpp < 64 && lds_of(lpp) > (size_t)DP_LDS_MAX) lpp *= 2;
+  while (lpp < 64 && lds_of(lpp) > (size_t)DP_LDS_MAX - sizeof(dp_dev_sys)) lpp *= 2;
   const int seg = 64 / lpp;
   const size_t lds = lds_of(lpp);
   // The pair count is on the device; size the grid from the bound N*P.

@@ -727,6 +727,12 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
   // per-lane candidate lists [blockDim.x][kQ64Cand]
   extern __shared__ double q64_lds[];
   const dp_dev_sys& S = *a.sys;
+  const int64_t n64 = quad64_count(a);
+  if ((int64_t)blockIdx.x * blockDim.x >= n64) return;  // uniform per block
+  const int nobs_d = 6 * a.nb + 4 * a.ns;
+  double* obs = q64_lds;
+  for (int t = threadIdx.x; t < nobs_d; t += blockDim.x) obs[t] = a.obs[t];
+  __syncthreads();
   // read once into registers: the rollout loops store to global memory,
   // after which a field of *a.sys would be read again on every step
   const double dt = S.dt, rad = S.radius, w0 = S.w[0];
@@ -738,12 +744,6 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
     blo[3 + c] = S.lo[7 + c];
     bhi[3 + c] = S.hi[7 + c];
   }
-  const int64_t n64 = quad64_count(a);
-  if ((int64_t)blockIdx.x * blockDim.x >= n64) return;  // uniform per block
-  const int nobs_d = 6 * a.nb + 4 * a.ns;
-  double* obs = q64_lds;
-  for (int t = threadIdx.x; t < nobs_d; t += blockDim.x) obs[t] = a.obs[t];
-  __syncthreads();
   int16_t* cand = reinterpret_cast<int16_t*>(q64_lds + nobs_d) + (int64_t)threadIdx.x * kQ64Cand;
   long long c_b = 0, c_c = 0, c_f = 0, c_k = 0, c_r = 0;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n64; t += (int64_t)gridDim.x * blockDim.x) {
@@ -963,17 +963,6 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
 // rollout so that neither kernel's registers limit the other's occupancy.
 __global__ __launch_bounds__(256) void k_quad_dupv(dp_quad_args a) {
   const dp_dev_sys& S = *a.sys;
-  // read once into registers: the rollout loops store to global memory,
-  // after which a field of *a.sys would be read again on every step
-  const double dt = S.dt, rad = S.radius, w0 = S.w[0];
-  double blo[6], bhi[6];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    blo[c] = S.lo[c];
-    bhi[c] = S.hi[c];
-    blo[3 + c] = S.lo[7 + c];
-    bhi[3 + c] = S.hi[7 + c];
-  }
   const int64_t n64 = quad64_count(a);
   const double T = dp_pos_bound(S, a.r_dup);
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n64; t += (int64_t)gridDim.x * blockDim.x) {
