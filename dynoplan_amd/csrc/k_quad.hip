@@ -500,9 +500,11 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
       // are clamped to the primitive's last state, so every load is valid
       // and unconditional (a conditional load would make the wait for it
       // exact only at the loop's end).
-      const float4* dvp = reinterpret_cast<const float4*>(a.dv32) + s0;
+      // Pairs decided from their primitive's record (the large majority)
+      // all read the table's first entry: one cache line for the wave.
+      const float4* dvp = reinterpret_cast<const float4*>(a.dv32) + (fast ? 0 : s0);
       const int32_t Lr = fast ? 0 : L1;
-      const int32_t Lm = L1 - 1;
+      const int32_t Lm = fast ? 0 : L1 - 1;
       // (issued in the loop's own order, slot 3 first, so that the wait
       // for each slot is the same on entry and around the loop)
       float4 dq[4];
