@@ -591,22 +591,25 @@ int dp_reverse_primitives(const dp_prims* in, int32_t system, double noise, uint
  * No counterpart in the reference, which plans one instance per process.  A
  * batch groups n contexts (independent instances, any mix of systems, one
  * device); dp_batch_expand expands the uploaded nodes of all of them.
- * DP_BATCH_GRAPH: one hipGraph launch whose branches are the instances'
- * pipelines, captured on the first call and again whenever a context's node
- * count, primitives, world, visited set, buffer capacities or surrogate, or
- * the params, changed since (replayed otherwise; a capture call also runs the
- * batch once eagerly).  DP_BATCH_STREAMS: the same pipelines launched
- * directly, one stream per instance.  Results are read per context
- * (dp_get_counts, dp_fetch_pairs); per-stage dp_last_timing is not kept for
- * graph launches. */
+ * Each instance's pipeline up to its conflict lists runs on its own stream;
+ * the ordered walks of all instances then run as one work queue per system
+ * (waves take (instance, node) items interleaved over the instances).
+ * DP_BATCH_GRAPH: all of it as one hipGraph launch, captured on the first
+ * call and again whenever a context's node count, primitives, world, visited
+ * set, buffer capacities or surrogate, or the params, changed since
+ * (replayed otherwise; a capture call also runs the batch once eagerly).
+ * DP_BATCH_STREAMS: the same work launched directly.  Results are read per
+ * context (dp_get_counts, dp_fetch_pairs); per-stage dp_last_timing is not
+ * kept for batch launches. */
 enum { DP_BATCH_GRAPH = 0, DP_BATCH_STREAMS = 1 };
 typedef struct dp_batch dp_batch;
 dp_batch* dp_batch_create(dp_ctx* const* ctxs, int32_t n, int32_t mode);
 void dp_batch_destroy(dp_batch* b);
 /* params[n], one per context; stream NULL = the batch's own stream. */
 int dp_batch_expand(dp_batch* b, const dp_expand_params* params, void* stream);
-/* Completion time of each instance of the last dp_batch_expand, ms[n] after
- * the batch started on the GPU (constant-rate GPU clock).  Waits for it. */
+/* Completion time of each instance of the last dp_batch_expand (its walk's
+ * last node), ms[n] after the batch started on the GPU (constant-rate GPU
+ * clock).  Waits for it. */
 int dp_batch_latency(dp_batch* b, float* ms);
 /* Graph captures and launches so far. */
 int dp_batch_stats(const dp_batch* b, int64_t* captures, int64_t* launches);
