@@ -285,6 +285,16 @@ def bench_multi(args):
     torch.cuda.synchronize()
     el_q = time.perf_counter() - t
     progress(f"sequential: {args.steps} steps in {el_q:.3f} s")
+    # one instance at a time with its counts read before the next, as
+    # separate planner processes expanding one batch each would do
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        for e, p in zip(exs, params):
+            e.expand_async(p, one)
+            e.counts()
+    torch.cuda.synchronize()
+    el_r = time.perf_counter() - t
+    progress(f"one at a time with counts: {args.steps} steps in {el_r:.3f} s")
     abytes = 0.0
     for e, w in zip(exs, wls):
         abytes += algorithmic_bytes(w, e.fetch()["prim"])
@@ -313,6 +323,7 @@ def bench_multi(args):
         "streams_latency_ms": {"instance_p50": float(np.percentile(lat_s, 50)),
                                "instance_p99": float(np.percentile(lat_s, 99))},
         "sequential_value": pairs * args.steps / el_q,
+        "one_at_a_time_value": pairs * args.steps / el_r,
         "graph_stats": st_g,
         "integrity": integrity,
         "roofline": {"bound": "hbm", "kernel": "whole batch (all kernels)", "achieved": achieved,
