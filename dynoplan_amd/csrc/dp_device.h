@@ -197,11 +197,34 @@ __device__ __forceinline__ bool dp_within_planar_fast(const dp_dev_sys& S, const
   return dp_within(S, x, y, r);
 }
 
-// dp_within_col with two exact rejections ahead of it for planar systems:
-// the squared position difference above T (dp_pos_bound), or the heading
+// dp_within(S, x, y, r) for the quadrotor, with a rejection that skips the
+// FP64 square roots for most candidates: the position, velocity and rate
+// terms estimated with FP32 square roots of the FP64 squared sums (as
+// dp_l2_3 forms them).  Conversions, the weights in FP32, sqrtf and two FP32
+// additions keep the estimate within a relative 1e-6 of the FP64 sum
+// A + C + D of dp_within (all terms non-negative), so an estimate above
+// r * (1 + 1e-5) proves (A + C) + D > r, where dp_within returns false.
+// Otherwise dp_within decides.  The answer is always dp_within's.
+__device__ __forceinline__ bool dp_within_quad_fast(const dp_dev_sys& S, const double* x, const double* y, double r) {
+  auto sq3 = [](const double* a, const double* b) {
+    const double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+    double s = d0 * d0;
+    s = s + d1 * d1;
+    s = s + d2 * d2;
+    return s;
+  };
+  const float est = (float)S.w[0] * sqrtf((float)sq3(x, y)) + (float)S.w[2] * sqrtf((float)sq3(x + 7, y + 7)) +
+                    (float)S.w[3] * sqrtf((float)sq3(x + 10, y + 10));
+  if (est > (float)(r * (1.0 + 1e-5))) return false;
+  return dp_within(S, x, y, r);
+}
+
+// dp_within_col with exact rejections ahead of it: the squared position
+// difference above T (dp_pos_bound), and for planar systems the heading
 // term w1 * so2 alone above r (dp_distance adds non-negative terms to it, and
-// IEEE addition is monotone).  Neither changes the answer; both skip the
-// square root for most candidates of a grid scan.
+// IEEE addition is monotone); then the FP32-estimate paths above.  None
+// changes the answer; they skip the square roots for most candidates of a
+// grid scan.
 template <int NX>
 __device__ __forceinline__ bool dp_within_col_pf(const dp_dev_sys& S, const double* x, const double* col,
                                                  int64_t stride, int64_t i, double r, double T) {
@@ -216,7 +239,19 @@ __device__ __forceinline__ bool dp_within_col_pf(const dp_dev_sys& S, const doub
     for (int k = 0; k < NX; ++k) y[k] = col[(int64_t)k * stride + i];
     return dp_within_planar_fast<NX>(S, x, y, r, T);
   } else {
-    return dp_within_col<NX>(S, x, col, stride, i, r);
+    // the squared position difference first (exact against T, dp_pos_bound),
+    // the other components only when it passes
+    double y[13];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) y[k] = col[(int64_t)k * stride + i];
+    const double d0 = x[0] - y[0], d1 = x[1] - y[1], d2 = x[2] - y[2];
+    double s = d0 * d0;
+    s = s + d1 * d1;
+    s = s + d2 * d2;
+    if (s > T) return false;
+#pragma unroll
+    for (int k = 3; k < 13; ++k) y[k] = col[(int64_t)k * stride + i];
+    return dp_within_quad_fast(S, x, y, r);
   }
 }
 

@@ -451,7 +451,7 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
                     dot = dot + x[4] * y[u][4];
                     dot = dot + x[5] * y[u][5];
                     dot = dot + x[6] * y[u][6];
-                    hit = !(fabs(dot) < cq) && !(dp_pos_term(S, x, y[u]) > r) && dp_within(S, x, y[u], r);
+                    hit = !(fabs(dot) < cq) && dp_within_quad_fast(S, x, y[u], r);
                   }
                   else hit = dp_within_planar_fast<NX>(S, x, y[u], r, T);
                 }
