@@ -216,11 +216,17 @@ __global__ __launch_bounds__(64) void k_app_blk(const dp_dev_sys* __restrict__ S
   int32_t cnt = 0;  // lane b < nb: node i0 + b
   for (int32_t w = w0; w < w1; ++w) {
     const int32_t r = w * 64 + lane;
-    const int32_t j = r < P ? (row ? row[r] : r) : 0;
+    // every lane loads (past P, the last primitive's key, then unused): a
+    // load under a condition would be waited for on its own, component
+    // after component
+    const int32_t rc = r < P ? r : P - 1;
+    const int32_t j = row ? row[rc] : rc;
     float k[NX];
 #pragma unroll
+    for (int c = 0; c < NX; ++c) k[c] = keys32[(int64_t)c * P + j];
+#pragma unroll
     for (int c = 0; c < NX; ++c) {
-      k[c] = r < P ? keys32[(int64_t)c * P + j] : 0.0f;
+      k[c] = r < P ? k[c] : 0.0f;
       s_k[c * 64 + lane] = k[c];
     }
     // stage 1 for every node of the group; survivors queued
