@@ -1070,9 +1070,17 @@ __global__ __launch_bounds__(64) void k_quad_window(int32_t N, const int32_t* __
   // (one atomic per node reserves their place in the compact list)
   int32_t rank = 0, need_n = 0;
   long long nd = 0;
+  // each chunk's flags are loaded while the previous chunk is processed
+  // (indices clamped to the node's last pair: unconditional loads, and the
+  // one past the last chunk goes unused); a lane reads and writes only the
+  // pairs p = b + lane (mod 64), so the second pass sees its own stores
+  const int32_t el = e > b ? e - 1 : b;
+  int8_t fl_n = e > b ? flags[b + lane < el ? b + lane : el] : (int8_t)0;
   for (int32_t cb = b; cb < e; cb += 64) {
     const int32_t p = cb + lane;
-    const int8_t fl = p < e ? flags[p] : (int8_t)0;
+    const int8_t fl_c = fl_n;
+    fl_n = flags[cb + 64 + lane < el ? cb + 64 + lane : el];
+    const int8_t fl = p < e ? fl_c : (int8_t)0;
     const uint64_t m = __ballot((fl & QF_NEED64) != 0);
     const bool defer = (fl & QF_NEED64) && rank + __popcll(m & lt) >= window;
     if (defer) {
@@ -1091,9 +1099,12 @@ __global__ __launch_bounds__(64) void k_quad_window(int32_t N, const int32_t* __
     if (lane == 0) base = (int32_t)atomicAdd((unsigned long long*)&counters[DPC_Q64], (unsigned long long)need_n);
     base = __shfl(base, 0, 64);
     // second pass: the pairs in canonical order (deferred ones now carry DP_F_DEFER)
+    fl_n = flags[b + lane < el ? b + lane : el];
     for (int32_t cb = b; cb < e; cb += 64) {
       const int32_t p = cb + lane;
-      const int8_t fl = p < e ? flags[p] : (int8_t)0;
+      const int8_t fl_c = fl_n;
+      fl_n = flags[cb + 64 + lane < el ? cb + 64 + lane : el];
+      const int8_t fl = p < e ? fl_c : (int8_t)0;
       const bool need = (fl & (QF_NEAR | QF_NEED64)) != 0;
       const uint64_t nm = __ballot(need);
       if (need) q64[base + __popcll(nm & lt)] = p;
