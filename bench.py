@@ -53,7 +53,7 @@ def workloads():
 # stream), per system family.
 KERNEL_GROUPS = {
     "check_main": {"quad": "k_quad32", "planar": "k_check_planar"},
-    "check_post": {"quad": "k_quad_window + k_quad64 + k_quad_dupv", "planar": "(none)"},
+    "check_post": {"quad": "k_quad_window + k_quad64 (with the visited duplicate test)", "planar": "(none)"},
     "lists": {"quad": "k_cont_count + k_cont_scatter + k_cell_list + k_conf_waves (+ fills, scans)",
               "planar": "k_cont_count + k_cont_scatter + k_cell_list + k_conf_waves (+ fills, scans)"},
     "walk": {"quad": "k_round_init + k_pre_nodes + k_resolve_ticketed",

@@ -738,6 +738,7 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
   // read once into registers: the rollout loops store to global memory,
   // after which a field of *a.sys would be read again on every step
   const double dt = S.dt, rad = S.radius, w0 = S.w[0];
+  const double Tdup = dp_pos_bound(S, a.r_dup);
   double blo[6], bhi[6];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -950,7 +951,13 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
     a.hout[q] = hq;
     a.goal_out[q] = chosen;
     a.res[q] = DP_R_UNDECIDED;
-    a.flags[q] = (int8_t)(chosen != -1 ? 2 : 0);  // the duplicate bit is set by k_quad_dupv
+    // duplicate of the visited set: the 9 runs of the final state's 27-cell
+    // block, with the state still in registers
+    bool dupv = false;
+    if (a.nv > 0)
+      for (int k = 0; k < 9 && !dupv; ++k)
+        dupv = dp_grid_run_any<13>(S, a.grid, a.vstart, a.vis, a.vstride, x, k, a.r_dup, Tdup);
+    a.flags[q] = (int8_t)((chosen != -1 ? 2 : 0) | (dupv ? 1 : 0));
     c_f += !a.redo;
   }
   if (c_b) atomicAdd((unsigned long long*)&a.counters[DPC_BOUNDS], (unsigned long long)c_b);
@@ -958,26 +965,6 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
   if (c_f) atomicAdd((unsigned long long*)&a.counters[DPC_FREE], (unsigned long long)c_f);
   if (c_k) atomicAdd((unsigned long long*)&a.counters[DPC_CHECKED], (unsigned long long)c_k);
   if (c_r) atomicAdd((unsigned long long*)&a.counters[DPC_RECHECK], (unsigned long long)c_r);
-}
-
-// Duplicate-of-visited test of the FREE pairs' final states, after the FP64
-// pass: one thread per pair, the 9 runs of the 27-cell block, apart from the
-// rollout so that neither kernel's registers limit the other's occupancy.
-__global__ __launch_bounds__(256) void k_quad_dupv(dp_quad_args a) {
-  const dp_dev_sys& S = *a.sys;
-  const int64_t n64 = quad64_count(a);
-  const double T = dp_pos_bound(S, a.r_dup);
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n64; t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t q = quad64_pair(a, t);
-    if (a.verdict[q] != DP_V_FREE || (a.flags[q] & DP_F_DEFER)) continue;
-    double x[13];
-#pragma unroll
-    for (int c = 0; c < 13; ++c) x[c] = a.fin[(int64_t)c * a.pcap + q];
-    bool dupv = false;
-    for (int k = 0; k < 9 && !dupv; ++k)
-      dupv = dp_grid_run_any<13>(S, a.grid, a.vstart, a.vis, a.vstride, x, k, a.r_dup, T);
-    if (dupv) a.flags[q] = (int8_t)(a.flags[q] | 1);
-  }
 }
 
 // Per node: its pairs reordered for the FP32 pass, those whose primitive
@@ -1268,10 +1255,9 @@ int32_t dp_fp64_window(const dp_ctx* c, int32_t limit) {
 int dp_launch_quad_deferred(dp_ctx* c, hipStream_t s) {
   const dp_expand_params* p = &c->round_params;
   dp_quad_args a = make_quad_args(c, p, p->delta_factor_goal * (double)p->delta, c->round_r_dup);
-  a.redo = 1;  // both kernels over every pair of the batch
+  a.redo = 1;  // over every pair of the batch, the deferred ones rolled out
   const int t64 = c->tiles.quad64_threads;
   k_quad64<<<dp_grid_blocks(c->pair_cap, t64, c->tiles.quad64_blocks), t64, quad64_lds(c, t64), s>>>(a);
-  if (c->nv > 0) k_quad_dupv<<<dp_grid_blocks(c->pair_cap, 256, 8192), 256, 0, s>>>(a);
   DP_HIP_RET(hipGetLastError());
   return DP_OK;
 }
@@ -1339,7 +1325,6 @@ int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, doub
   }
   const int t64 = c->tiles.quad64_threads;
   k_quad64<<<dp_grid_blocks(c->pair_cap, t64, c->tiles.quad64_blocks), t64, quad64_lds(c, t64), s>>>(a);
-  if (c->nv > 0) k_quad_dupv<<<dp_grid_blocks(c->pair_cap, 256, 8192), 256, 0, s>>>(a);
   DP_HIP_RET(hipGetLastError());
   return DP_OK;
 }
