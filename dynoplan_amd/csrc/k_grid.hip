@@ -573,44 +573,95 @@ __global__ __launch_bounds__(64) void k_precap_first(int32_t N, const int32_t* _
 // is NOVEL unless capped, every FREE pair is CAPPED; a pair that is NOVEL
 // unless capped, with fewer than limit possibly NOVEL pairs before it, is
 // NOVEL.
-__global__ __launch_bounds__(64) void k_pre_nodes(int32_t N, const int32_t* __restrict__ app,
-                                                  const int32_t* __restrict__ total,
-                                                  const int8_t* __restrict__ verdict,
-                                                  const int8_t* __restrict__ flags, int8_t* __restrict__ res,
-                                                  int32_t limit) {
-  const int32_t node = blockIdx.x;
-  if (node >= N) return;
-  const int lane = threadIdx.x;
-  const uint64_t lt = (1ull << lane) - 1ull;
-  const int32_t tot = *total;
-  const int32_t b = app[node] < tot ? app[node] : tot;
-  const int32_t e = app[node + 1] < tot ? app[node + 1] : tot;
-  int nuc = 0, possible = 0;
-  for (int32_t cb = b; cb < e; cb += 64) {
-    const int32_t p = cb + lane;
-    const bool valid = p < e && verdict[p] == DP_V_FREE;
-    const int8_t fl = valid ? flags[p] : 0;
-    const int r = valid ? res[p] : DP_R_NONE;
-    const bool dupv = (fl & 1) && !(fl & 2);
-    const bool not_nov = valid && (dupv || r == DP_R_CAPPED);
-    const bool nuc_p = valid && (r == DP_R_NOVEL || (fl & (2 | DP_F_NUC)) != 0);
-    const uint64_t nuc_m = __ballot(nuc_p);
-    const uint64_t pos_m = __ballot(valid && !not_nov);
-    const int nuc_before = nuc + __popcll(nuc_m & lt);
-    const int pos_before = possible + __popcll(pos_m & lt);
-    if (limit > 0 && nuc_before >= limit) {
-      if (valid && r != DP_R_CAPPED) res[p] = DP_R_CAPPED;
-    } else if (nuc_p && r != DP_R_NOVEL && (limit <= 0 || pos_before < limit)) {
-      res[p] = DP_R_NOVEL;
+__global__ __launch_bounds__(256) void k_pre_nodes(int32_t N, const int32_t* __restrict__ app,
+                                                   const int32_t* __restrict__ total,
+                                                   const int8_t* __restrict__ verdict,
+                                                   const int8_t* __restrict__ flags, int8_t* __restrict__ res,
+                                                   int32_t limit, int32_t* __restrict__ cursor,
+                                                   int32_t* __restrict__ count, int32_t* __restrict__ node_novel,
+                                                   long long* __restrict__ counters) {
+  // four nodes per workgroup, one per wave; the finished nodes' counts are
+  // summed in LDS and added with one atomic per counter and workgroup (one
+  // per node, all at the same moment, queued on the same few addresses)
+  __shared__ long long s_add[4];
+  if (threadIdx.x < 4) s_add[threadIdx.x] = 0;
+  __syncthreads();
+  const int32_t node = blockIdx.x * 4 + (int32_t)(threadIdx.x / 64);
+  const int lane = threadIdx.x % 64;
+  if (node < N) {
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const int32_t tot = *total;
+    const int32_t b = app[node] < tot ? app[node] : tot;
+    const int32_t e = app[node + 1] < tot ? app[node + 1] : tot;
+    int nuc = 0, possible = 0;
+    // statuses this pass leaves the node's FREE pairs in
+    int c_nov = 0, c_goal = 0, c_cap = 0;
+    bool undec = false;
+    for (int32_t cb = b; cb < e; cb += 64) {
+      const int32_t p = cb + lane;
+      const bool valid = p < e && verdict[p] == DP_V_FREE;
+      const int8_t fl = valid ? flags[p] : 0;
+      const int r = valid ? res[p] : DP_R_NONE;
+      const bool dupv = (fl & 1) && !(fl & 2);
+      const bool not_nov = valid && (dupv || r == DP_R_CAPPED);
+      const bool nuc_p = valid && (r == DP_R_NOVEL || (fl & (2 | DP_F_NUC)) != 0);
+      const uint64_t nuc_m = __ballot(nuc_p);
+      const uint64_t pos_m = __ballot(valid && !not_nov);
+      const int nuc_before = nuc + __popcll(nuc_m & lt);
+      const int pos_before = possible + __popcll(pos_m & lt);
+      int st = r;
+      if (limit > 0 && nuc_before >= limit) {
+        if (valid && r != DP_R_CAPPED) res[p] = DP_R_CAPPED;
+        st = DP_R_CAPPED;
+      } else if (nuc_p && r != DP_R_NOVEL && (limit <= 0 || pos_before < limit)) {
+        res[p] = DP_R_NOVEL;
+        st = DP_R_NOVEL;
+      }
+      if (valid) {
+        c_nov += st == DP_R_NOVEL;
+        c_goal += st == DP_R_NOVEL && (fl & 2);
+        c_cap += st == DP_R_CAPPED;
+        undec = undec || (st != DP_R_NOVEL && st != DP_R_CAPPED);
+      }
+      nuc += __popcll(nuc_m);
+      possible += __popcll(pos_m);
+      if (limit > 0 && nuc >= limit) {
+        // every later pair has at least limit such pairs before it: capped
+        for (int32_t t = cb + 64 + lane; t < e; t += 64)
+          if (verdict[t] == DP_V_FREE) {
+            if (res[t] != DP_R_CAPPED) res[t] = DP_R_CAPPED;
+            ++c_cap;
+          }
+        break;
+      }
     }
-    nuc += __popcll(nuc_m);
-    possible += __popcll(pos_m);
-    if (limit > 0 && nuc >= limit) {
-      // every later pair has at least limit such pairs before it: capped
-      for (int32_t t = cb + 64 + lane; t < e; t += 64)
-        if (verdict[t] == DP_V_FREE && res[t] != DP_R_CAPPED) res[t] = DP_R_CAPPED;
-      break;
+    // Every FREE pair decided here: the node is finished as its walk would
+    // finish it (its statuses counted, its cursor at the end), and its wave
+    // in the walk returns at once.
+    if (__ballot(undec) == 0) {
+#pragma unroll
+      for (int d = 32; d > 0; d >>= 1) {
+        c_nov += __shfl_xor(c_nov, d, 64);
+        c_goal += __shfl_xor(c_goal, d, 64);
+        c_cap += __shfl_xor(c_cap, d, 64);
+      }
+      if (lane == 0) {
+        cursor[node] = e;
+        count[node] = c_nov;
+        if (node_novel) node_novel[node] = c_nov;
+        if (b < e) atomicAdd((unsigned long long*)&s_add[0], 1ull);
+        atomicAdd((unsigned long long*)&s_add[1], (unsigned long long)c_nov);
+        atomicAdd((unsigned long long*)&s_add[2], (unsigned long long)c_cap);
+        atomicAdd((unsigned long long*)&s_add[3], (unsigned long long)c_goal);
+      }
     }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_add[0]) atomicAdd((unsigned long long*)&counters[DPC_NODES_LEFT], (unsigned long long)-s_add[0]);
+    if (s_add[1]) atomicAdd((unsigned long long*)&counters[DPC_NOVEL], (unsigned long long)s_add[1]);
+    if (s_add[2]) atomicAdd((unsigned long long*)&counters[DPC_CAPPED], (unsigned long long)s_add[2]);
+    if (s_add[3]) atomicAdd((unsigned long long*)&counters[DPC_GOAL], (unsigned long long)s_add[3]);
   }
 }
 
@@ -1372,8 +1423,9 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
     k_round_init<<<dp_grid_blocks(N, 256, 1024), 256, 0, s>>>(N, c->d_app, c->d_total, c->d_node_cursor,
                                                               c->d_node_count, c->d_node_novel, c->d_counters);
     if (p->limit_branching_factor > 0)
-      k_pre_nodes<<<N, 64, 0, s>>>(N, c->d_app, c->d_total, c->d_verdict, c->d_flags, c->d_res,
-                                   p->limit_branching_factor);
+      k_pre_nodes<<<(N + 3) / 4, 256, 0, s>>>(N, c->d_app, c->d_total, c->d_verdict, c->d_flags, c->d_res,
+                                   p->limit_branching_factor, c->d_node_cursor, c->d_node_count, c->d_node_novel,
+                                   c->d_counters);
     if (c->defer_walk) return DP_OK;  // a multi-instance batch walks every instance in one queue
     round_args ra = make_round_args(c);
     ra.wait = DP_WAIT_POLLS;
