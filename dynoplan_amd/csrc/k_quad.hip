@@ -1034,15 +1034,20 @@ __global__ void k_quad_chunk_nodes(int32_t N, const int32_t* __restrict__ chunk_
 // unless `fixed`: the resolution raises wshift[0] when a node overflows, so a
 // context's later batches stay on the device; wshift[1] records the growth
 // this batch used.
-__global__ __launch_bounds__(64) void k_quad_window(int32_t N, const int32_t* __restrict__ app,
-                                                    const int32_t* __restrict__ total, int8_t* __restrict__ verdict,
-                                                    int32_t* __restrict__ firstk, int32_t* __restrict__ goal_out,
-                                                    int8_t* __restrict__ res, int8_t* __restrict__ flags,
-                                                    int32_t window, int32_t fixed, int32_t* __restrict__ wshift,
-                                                    int32_t* __restrict__ q64, long long* __restrict__ counters) {
-  const int32_t node = blockIdx.x;
-  if (node >= N) return;
-  const int lane = threadIdx.x;
+__global__ __launch_bounds__(256) void k_quad_window(int32_t N, const int32_t* __restrict__ app,
+                                                     const int32_t* __restrict__ total, int8_t* __restrict__ verdict,
+                                                     int32_t* __restrict__ firstk, int32_t* __restrict__ goal_out,
+                                                     int8_t* __restrict__ res, int8_t* __restrict__ flags,
+                                                     int32_t window, int32_t fixed, int32_t* __restrict__ wshift,
+                                                     int32_t* __restrict__ q64, long long* __restrict__ counters) {
+  // four nodes per workgroup, one per wave: their places in the compact list
+  // are reserved with one atomic per workgroup (one per node, all issued
+  // together on one address, queue behind each other)
+  __shared__ int32_t s_need[4];
+  __shared__ int32_t s_base;
+  const int wv = threadIdx.x / 64;
+  const int32_t node = blockIdx.x * 4 + wv;
+  const int lane = threadIdx.x % 64;
   if (!fixed) {
     const int32_t sh = wshift[0] < 10 ? wshift[0] : 10;
     const int64_t w = (int64_t)window << sh;
@@ -1051,10 +1056,9 @@ __global__ __launch_bounds__(64) void k_quad_window(int32_t N, const int32_t* __
   }
   const uint64_t lt = (1ull << lane) - 1ull;
   const int32_t tot = *total;
-  const int32_t b = app[node] < tot ? app[node] : tot;
-  const int32_t e = app[node + 1] < tot ? app[node + 1] : tot;
+  const int32_t b = node < N ? (app[node] < tot ? app[node] : tot) : 0;
+  const int32_t e = node < N ? (app[node + 1] < tot ? app[node + 1] : tot) : 0;
   // first pass: deferrals, and the node's count of pairs for the FP64 pass
-  // (one atomic per node reserves their place in the compact list)
   int32_t rank = 0, need_n = 0;
   long long nd = 0;
   // each chunk's flags are loaded while the previous chunk is processed
@@ -1081,10 +1085,16 @@ __global__ __launch_bounds__(64) void k_quad_window(int32_t N, const int32_t* __
     need_n += __popcll(__ballot((fl & (QF_NEAR | QF_NEED64)) != 0 && !defer));
     rank += __popcll(m);
   }
+  if (lane == 0) s_need[wv] = need_n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int32_t sum = s_need[0] + s_need[1] + s_need[2] + s_need[3];
+    s_base = sum ? (int32_t)atomicAdd((unsigned long long*)&counters[DPC_Q64], (unsigned long long)sum) : 0;
+  }
+  __syncthreads();
   if (need_n > 0) {
-    int32_t base = 0;
-    if (lane == 0) base = (int32_t)atomicAdd((unsigned long long*)&counters[DPC_Q64], (unsigned long long)need_n);
-    base = __shfl(base, 0, 64);
+    int32_t base = s_base;
+    for (int k = 0; k < wv; ++k) base += s_need[k];
     // second pass: the pairs in canonical order (deferred ones now carry DP_F_DEFER)
     fl_n = flags[b + lane < el ? b + lane : el];
     for (int32_t cb = b; cb < e; cb += 64) {
@@ -1316,7 +1326,7 @@ int dp_launch_quad(dp_ctx* c, const dp_expand_params* p, double goal_bound, doub
     const int32_t window = dp_fp64_window(c, p->limit_branching_factor);
     const int32_t fixed = c->fp64_window > 0 || p->limit_branching_factor < 0 ? 1 : 0;
     if (c->N > 0)
-      k_quad_window<<<c->N, 64, 0, s>>>(c->N, c->d_app, c->d_total, c->d_verdict, c->d_firstk, c->d_goal, c->d_res,
+      k_quad_window<<<(c->N + 3) / 4, 256, 0, s>>>(c->N, c->d_app, c->d_total, c->d_verdict, c->d_firstk, c->d_goal, c->d_res,
                                         c->d_flags, window, fixed, c->d_wshift, c->d_q64, c->d_counters);
   } else {
     int rc = dp_record_event(c, 5, s);
