@@ -337,6 +337,11 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
   const int per_wave = 6 * a.nb + 4 * a.ns;
   float* cb = qsm + wave * per_wave;  // compacted boxes (6 floats) then spheres (4 floats)
+  // the workgroup's counts, added to the batch counters once per workgroup
+  // (once per wave, thousands of waves queued on three addresses at the end)
+  __shared__ unsigned long long s_cnt[3];
+  if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0ull;
+  __syncthreads();
   const quad_f32 K = make_quad_f32(S);
   const float r = (float)S.radius;
   // margins under which a verdict is re-checked in FP64: near_eps for
@@ -653,9 +658,15 @@ __global__ __launch_bounds__(256) void k_quad32(dp_quad_args a) {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-  if (c_b) atomicAdd((unsigned long long*)&a.counters[DPC_BOUNDS], (unsigned long long)c_b);
-  if (c_c) atomicAdd((unsigned long long*)&a.counters[DPC_COLLIDE], (unsigned long long)c_c);
-  if (c_k) atomicAdd((unsigned long long*)&a.counters[DPC_CHECKED], (unsigned long long)c_k);
+  if (c_b) atomicAdd(&s_cnt[0], (unsigned long long)c_b);
+  if (c_c) atomicAdd(&s_cnt[1], (unsigned long long)c_c);
+  if (c_k) atomicAdd(&s_cnt[2], (unsigned long long)c_k);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_cnt[0]) atomicAdd((unsigned long long*)&a.counters[DPC_BOUNDS], s_cnt[0]);
+    if (s_cnt[1]) atomicAdd((unsigned long long*)&a.counters[DPC_COLLIDE], s_cnt[1]);
+    if (s_cnt[2]) atomicAdd((unsigned long long*)&a.counters[DPC_CHECKED], s_cnt[2]);
+  }
 }
 
 // ---- FP64 pass -------------------------------------------------------------------
@@ -734,6 +745,9 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
   const int nobs_d = 6 * a.nb + 4 * a.ns;
   double* obs = q64_lds;
   for (int t = threadIdx.x; t < nobs_d; t += blockDim.x) obs[t] = a.obs[t];
+  // the workgroup's counts, added to the batch counters once per workgroup
+  __shared__ unsigned long long s_cnt[5];
+  if (threadIdx.x < 5) s_cnt[threadIdx.x] = 0ull;
   __syncthreads();
   // read once into registers: the rollout loops store to global memory,
   // after which a field of *a.sys would be read again on every step
@@ -960,11 +974,17 @@ __global__ __launch_bounds__(256) void k_quad64(dp_quad_args a) {
     a.flags[q] = (int8_t)((chosen != -1 ? 2 : 0) | (dupv ? 1 : 0));
     c_f += !a.redo;
   }
-  if (c_b) atomicAdd((unsigned long long*)&a.counters[DPC_BOUNDS], (unsigned long long)c_b);
-  if (c_c) atomicAdd((unsigned long long*)&a.counters[DPC_COLLIDE], (unsigned long long)c_c);
-  if (c_f) atomicAdd((unsigned long long*)&a.counters[DPC_FREE], (unsigned long long)c_f);
-  if (c_k) atomicAdd((unsigned long long*)&a.counters[DPC_CHECKED], (unsigned long long)c_k);
-  if (c_r) atomicAdd((unsigned long long*)&a.counters[DPC_RECHECK], (unsigned long long)c_r);
+  if (c_b) atomicAdd(&s_cnt[0], (unsigned long long)c_b);
+  if (c_c) atomicAdd(&s_cnt[1], (unsigned long long)c_c);
+  if (c_f) atomicAdd(&s_cnt[2], (unsigned long long)c_f);
+  if (c_k) atomicAdd(&s_cnt[3], (unsigned long long)c_k);
+  if (c_r) atomicAdd(&s_cnt[4], (unsigned long long)c_r);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int slot[5] = {DPC_BOUNDS, DPC_COLLIDE, DPC_FREE, DPC_CHECKED, DPC_RECHECK};
+    for (int k = 0; k < 5; ++k)
+      if (s_cnt[k]) atomicAdd((unsigned long long*)&a.counters[slot[k]], s_cnt[k]);
+  }
 }
 
 // Per node: its pairs reordered for the FP32 pass, those whose primitive
