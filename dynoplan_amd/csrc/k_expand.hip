@@ -447,6 +447,9 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
   static_assert(sizeof(dp_dev_sys) % sizeof(double) == 0, "copied as doubles");
   for (int t = threadIdx.x; t < (int)(sizeof(dp_dev_sys) / sizeof(double)); t += blockDim.x)
     reinterpret_cast<double*>(&s_sys)[t] = reinterpret_cast<const double*>(a.sys)[t];
+  // the workgroup's counts, added to the batch counters once per workgroup
+  __shared__ unsigned long long s_cnt[5];
+  if (threadIdx.x < 5) s_cnt[threadIdx.x] = 0ull;
   __syncthreads();
 
   constexpr int SEG = 64 / LPP;
@@ -691,11 +694,17 @@ __global__ __launch_bounds__(256) void k_check_planar(dp_check_args a) {
     }
   }
   if (k0 == 0) {
-    if (c_x) atomicAdd((unsigned long long*)&a.counters[DPC_CONSTRAINT], (unsigned long long)c_x);
-    if (c_b) atomicAdd((unsigned long long*)&a.counters[DPC_BOUNDS], (unsigned long long)c_b);
-    if (c_c) atomicAdd((unsigned long long*)&a.counters[DPC_COLLIDE], (unsigned long long)c_c);
-    if (c_f) atomicAdd((unsigned long long*)&a.counters[DPC_FREE], (unsigned long long)c_f);
-    if (c_k) atomicAdd((unsigned long long*)&a.counters[DPC_CHECKED], (unsigned long long)c_k);
+    if (c_x) atomicAdd(&s_cnt[0], (unsigned long long)c_x);
+    if (c_b) atomicAdd(&s_cnt[1], (unsigned long long)c_b);
+    if (c_c) atomicAdd(&s_cnt[2], (unsigned long long)c_c);
+    if (c_f) atomicAdd(&s_cnt[3], (unsigned long long)c_f);
+    if (c_k) atomicAdd(&s_cnt[4], (unsigned long long)c_k);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int slot[5] = {DPC_CONSTRAINT, DPC_BOUNDS, DPC_COLLIDE, DPC_FREE, DPC_CHECKED};
+    for (int k = 0; k < 5; ++k)
+      if (s_cnt[k]) atomicAdd((unsigned long long*)&a.counters[slot[k]], s_cnt[k]);
   }
 }
 
