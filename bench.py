@@ -56,8 +56,8 @@ KERNEL_GROUPS = {
     "check_post": {"quad": "k_quad_window + k_quad64 (with the visited duplicate test)", "planar": "(none)"},
     "lists": {"quad": "k_cont_count + k_cont_scatter + k_cell_list + k_conf_waves (+ fills, scans)",
               "planar": "k_cont_count + k_cont_scatter + k_cell_list + k_conf_waves (+ fills, scans)"},
-    "walk": {"quad": "k_round_init + k_pre_nodes + k_resolve_ticketed",
-             "planar": "k_round_init + k_pre_nodes + k_resolve_ticketed"},
+    "walk": {"quad": "k_pre_nodes + k_resolve_ticketed",
+             "planar": "k_pre_nodes + k_resolve_ticketed"},
 }
 
 

@@ -572,7 +572,10 @@ __global__ __launch_bounds__(64) void k_precap_first(int32_t N, const int32_t* _
 // One wave per node, its FREE pairs in order: after the limit-th pair that
 // is NOVEL unless capped, every FREE pair is CAPPED; a pair that is NOVEL
 // unless capped, with fewer than limit possibly NOVEL pairs before it, is
-// NOVEL.
+// NOVEL.  The kernel also sets up every node's walk state (the work of
+// k_round_init): the cursor at the node's first pair, or at its end when
+// every FREE pair was decided here, and the node counted as left to walk
+// only in the first case.
 __global__ __launch_bounds__(256) void k_pre_nodes(int32_t N, const int32_t* __restrict__ app,
                                                    const int32_t* __restrict__ total,
                                                    const int8_t* __restrict__ verdict,
@@ -580,9 +583,9 @@ __global__ __launch_bounds__(256) void k_pre_nodes(int32_t N, const int32_t* __r
                                                    int32_t limit, int32_t* __restrict__ cursor,
                                                    int32_t* __restrict__ count, int32_t* __restrict__ node_novel,
                                                    long long* __restrict__ counters) {
-  // four nodes per workgroup, one per wave; the finished nodes' counts are
-  // summed in LDS and added with one atomic per counter and workgroup (one
-  // per node, all at the same moment, queued on the same few addresses)
+  // four nodes per workgroup, one per wave; the counts are summed in LDS and
+  // added with one atomic per counter and workgroup (one per node, all at
+  // the same moment, queued on the same few addresses)
   __shared__ long long s_add[4];
   if (threadIdx.x < 4) s_add[threadIdx.x] = 0;
   __syncthreads();
@@ -597,59 +600,70 @@ __global__ __launch_bounds__(256) void k_pre_nodes(int32_t N, const int32_t* __r
     // statuses this pass leaves the node's FREE pairs in
     int c_nov = 0, c_goal = 0, c_cap = 0;
     bool undec = false;
-    for (int32_t cb = b; cb < e; cb += 64) {
-      const int32_t p = cb + lane;
-      const bool valid = p < e && verdict[p] == DP_V_FREE;
-      const int8_t fl = valid ? flags[p] : 0;
-      const int r = valid ? res[p] : DP_R_NONE;
-      const bool dupv = (fl & 1) && !(fl & 2);
-      const bool not_nov = valid && (dupv || r == DP_R_CAPPED);
-      const bool nuc_p = valid && (r == DP_R_NOVEL || (fl & (2 | DP_F_NUC)) != 0);
-      const uint64_t nuc_m = __ballot(nuc_p);
-      const uint64_t pos_m = __ballot(valid && !not_nov);
-      const int nuc_before = nuc + __popcll(nuc_m & lt);
-      const int pos_before = possible + __popcll(pos_m & lt);
-      int st = r;
-      if (limit > 0 && nuc_before >= limit) {
-        if (valid && r != DP_R_CAPPED) res[p] = DP_R_CAPPED;
-        st = DP_R_CAPPED;
-      } else if (nuc_p && r != DP_R_NOVEL && (limit <= 0 || pos_before < limit)) {
-        res[p] = DP_R_NOVEL;
-        st = DP_R_NOVEL;
+    // kPre chunks of 64 pairs per step, all their loads issued before the
+    // first chunk is evaluated (indices clamped into the node, so the loads
+    // are unconditional); past the limit-th such pair the evaluation below
+    // caps every later FREE pair
+    constexpr int kPre = 4;
+    for (int32_t c0 = b; c0 < e; c0 += 64 * kPre) {
+      int8_t v_u[kPre], f_u[kPre], r_u[kPre];
+#pragma unroll
+      for (int u = 0; u < kPre; ++u) {
+        const int32_t p = c0 + 64 * u + lane < e ? c0 + 64 * u + lane : e - 1;
+        v_u[u] = verdict[p];
+        f_u[u] = flags[p];
+        r_u[u] = res[p];
       }
-      if (valid) {
-        c_nov += st == DP_R_NOVEL;
-        c_goal += st == DP_R_NOVEL && (fl & 2);
-        c_cap += st == DP_R_CAPPED;
-        undec = undec || (st != DP_R_NOVEL && st != DP_R_CAPPED);
-      }
-      nuc += __popcll(nuc_m);
-      possible += __popcll(pos_m);
-      if (limit > 0 && nuc >= limit) {
-        // every later pair has at least limit such pairs before it: capped
-        for (int32_t t = cb + 64 + lane; t < e; t += 64)
-          if (verdict[t] == DP_V_FREE) {
-            if (res[t] != DP_R_CAPPED) res[t] = DP_R_CAPPED;
-            ++c_cap;
-          }
-        break;
+#pragma unroll
+      for (int u = 0; u < kPre; ++u) {
+        const int32_t p = c0 + 64 * u + lane;
+        const bool valid = p < e && v_u[u] == DP_V_FREE;
+        const int8_t fl = valid ? f_u[u] : 0;
+        const int r = valid ? r_u[u] : DP_R_NONE;
+        const bool dupv = (fl & 1) && !(fl & 2);
+        const bool not_nov = valid && (dupv || r == DP_R_CAPPED);
+        const bool nuc_p = valid && (r == DP_R_NOVEL || (fl & (2 | DP_F_NUC)) != 0);
+        const uint64_t nuc_m = __ballot(nuc_p);
+        const uint64_t pos_m = __ballot(valid && !not_nov);
+        const int nuc_before = nuc + __popcll(nuc_m & lt);
+        const int pos_before = possible + __popcll(pos_m & lt);
+        int st = r;
+        if (limit > 0 && nuc_before >= limit) {
+          if (valid && r != DP_R_CAPPED) res[p] = DP_R_CAPPED;
+          st = DP_R_CAPPED;
+        } else if (nuc_p && r != DP_R_NOVEL && (limit <= 0 || pos_before < limit)) {
+          res[p] = DP_R_NOVEL;
+          st = DP_R_NOVEL;
+        }
+        if (valid) {
+          c_nov += st == DP_R_NOVEL;
+          c_goal += st == DP_R_NOVEL && (fl & 2);
+          c_cap += st == DP_R_CAPPED;
+          undec = undec || (st != DP_R_NOVEL && st != DP_R_CAPPED);
+        }
+        nuc += __popcll(nuc_m);
+        possible += __popcll(pos_m);
       }
     }
     // Every FREE pair decided here: the node is finished as its walk would
     // finish it (its statuses counted, its cursor at the end), and its wave
-    // in the walk returns at once.
-    if (__ballot(undec) == 0) {
+    // in the walk returns at once.  Otherwise the walk starts at its first
+    // pair with nothing counted.
+    const bool finished = __ballot(undec) == 0;
+    if (finished) {
 #pragma unroll
       for (int d = 32; d > 0; d >>= 1) {
         c_nov += __shfl_xor(c_nov, d, 64);
         c_goal += __shfl_xor(c_goal, d, 64);
         c_cap += __shfl_xor(c_cap, d, 64);
       }
-      if (lane == 0) {
-        cursor[node] = e;
-        count[node] = c_nov;
-        if (node_novel) node_novel[node] = c_nov;
-        if (b < e) atomicAdd((unsigned long long*)&s_add[0], 1ull);
+    }
+    if (lane == 0) {
+      cursor[node] = finished ? e : b;
+      count[node] = finished ? c_nov : 0;
+      if (node_novel) node_novel[node] = finished ? c_nov : 0;
+      if (b < e && !finished) atomicAdd((unsigned long long*)&s_add[0], 1ull);
+      if (finished) {
         atomicAdd((unsigned long long*)&s_add[1], (unsigned long long)c_nov);
         atomicAdd((unsigned long long*)&s_add[2], (unsigned long long)c_cap);
         atomicAdd((unsigned long long*)&s_add[3], (unsigned long long)c_goal);
@@ -658,7 +672,7 @@ __global__ __launch_bounds__(256) void k_pre_nodes(int32_t N, const int32_t* __r
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    if (s_add[0]) atomicAdd((unsigned long long*)&counters[DPC_NODES_LEFT], (unsigned long long)-s_add[0]);
+    if (s_add[0]) atomicAdd((unsigned long long*)&counters[DPC_NODES_LEFT], (unsigned long long)s_add[0]);
     if (s_add[1]) atomicAdd((unsigned long long*)&counters[DPC_NOVEL], (unsigned long long)s_add[1]);
     if (s_add[2]) atomicAdd((unsigned long long*)&counters[DPC_CAPPED], (unsigned long long)s_add[2]);
     if (s_add[3]) atomicAdd((unsigned long long*)&counters[DPC_GOAL], (unsigned long long)s_add[3]);
@@ -1420,12 +1434,13 @@ int dp_launch_resolve(dp_ctx* c, const dp_expand_params* p, double r_dup, hipStr
         c->d_sys, c->grid, c->d_flags, c->d_pnode, c->d_app, window, c->pair_cap, c->d_cont_start, c->d_cont_idx, c->d_cont_x, c->d_cont_cursor,
         r_dup, c->d_conf_cnt, c->d_conf, c->d_conf_ov, c->d_conf_pool, pool_cap, c->conf_slots, c->d_counters);
     if (dp_record_event(c, 7, s)) return DP_ERR_HIP;
-    k_round_init<<<dp_grid_blocks(N, 256, 1024), 256, 0, s>>>(N, c->d_app, c->d_total, c->d_node_cursor,
-                                                              c->d_node_count, c->d_node_novel, c->d_counters);
     if (p->limit_branching_factor > 0)
       k_pre_nodes<<<(N + 3) / 4, 256, 0, s>>>(N, c->d_app, c->d_total, c->d_verdict, c->d_flags, c->d_res,
-                                   p->limit_branching_factor, c->d_node_cursor, c->d_node_count, c->d_node_novel,
-                                   c->d_counters);
+                                               p->limit_branching_factor, c->d_node_cursor, c->d_node_count,
+                                               c->d_node_novel, c->d_counters);
+    else
+      k_round_init<<<dp_grid_blocks(N, 256, 1024), 256, 0, s>>>(N, c->d_app, c->d_total, c->d_node_cursor,
+                                                                c->d_node_count, c->d_node_novel, c->d_counters);
     if (c->defer_walk) return DP_OK;  // a multi-instance batch walks every instance in one queue
     round_args ra = make_round_args(c);
     ra.wait = DP_WAIT_POLLS;
