@@ -320,7 +320,19 @@ __global__ __launch_bounds__(256) void k_conf_waves(const dp_dev_sys* __restrict
   const int64_t d01 = (int64_t)G.dims[0] * G.dims[1];
   const long long nwaves = (long long)gridDim.x * (blockDim.x / 64);
   constexpr int kU = NX == 13 ? 2 : 4;  // candidates per lane in flight
-  for (long long w = (long long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; w < ncl; w += nwaves) {
+  // Workgroups are dealt to the 8 XCDs in turn and each XCD has its own L2;
+  // items listed next to each other are cells next to each other, whose
+  // 27-cell blocks read mostly the same contender runs, so each XCD takes
+  // one eighth of the items, contiguous (when the grid is a multiple of 8
+  // workgroups; otherwise every wave strides over all items)
+  long long w0 = (long long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, w_end = ncl, w_step = nwaves;
+  if ((gridDim.x & 7) == 0) {
+    const long long range = (ncl + 7) / 8, x = blockIdx.x & 7;
+    w0 = x * range + (long long)(blockIdx.x >> 3) * (blockDim.x / 64) + threadIdx.x / 64;
+    w_end = x * range + range < ncl ? x * range + range : ncl;
+    w_step = nwaves / 8;
+  }
+  for (long long w = w0; w < w_end; w += w_step) {
     const uint32_t item = (uint32_t)cells[w];
     const int64_t cell = item & 0xFFFFFFu;
     const int32_t part = (int32_t)(item >> 24);
