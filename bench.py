@@ -28,8 +28,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
 # FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy (profiles/ holds the runs):
 # workload -> (kernel group, bytes, source).
 MEASURED_TRAFFIC = {
-    "quadrotor": ("check_main", 39.1e6, "profiles/k_quad32_quadrotor_hbm_traffic.txt"),
-    "unicycle1_scale": ("check_main", 1.383e9, "profiles/k_check_planar_unicycle1_scale_hbm_traffic.txt"),
+    "quadrotor": ("check_main", 37.9e6, "profiles/k_quad32_quadrotor_hbm_traffic.txt"),
+    "unicycle1_scale": ("check_main", 1.315e9, "profiles/k_check_planar_unicycle1_scale_hbm_traffic.txt"),
 }
 
 
